@@ -1,0 +1,165 @@
+// common.h -- shared host/device declarations of libelfel_hip.so (gfx950).
+//
+// The library integrates finite elements and assembles the global sparse
+// matrix on one MI355X.  Layout in HBM (all 0-based int32 indices):
+//   conn   [E][nen]        element-major node numbers (the Julia layout)
+//   xy     [Nn] double2    node coordinates
+//   dof    [Nn][ncomp]     dof numbers (FEField.dofnums, 0-based)
+//   geom   [E][nen][2]     per-element packed coordinates written by the
+//                          gather kernel (FP64 absolute, or FP32 relative to
+//                          the element's first node for the mixed kernels)
+//   CSR/CSC rowptr [nmaj+1], colind [nnz], values [nnz] double.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/elfel_hip.h"
+
+#define ELFEL_MAXQ 4     // quadrature points per element (triangle 1/3, square 1x1/2x2)
+#define ELFEL_MAXBF 6    // basis functions of the scalar element (T3 3, Q4 4, T6 6)
+
+// ---- quadrature / basis tables --------------------------------------------
+// Tabulated once on the host from the reference formulas (RefShapes.jl,
+// FElements.jl) and handed to the kernels by value: every index into the
+// table is wave-uniform, so the loads are scalar (SGPR) loads from the kernel
+// argument segment.
+struct QTable {
+    int nq, nbf;
+    double w[ELFEL_MAXQ];
+    double N[ELFEL_MAXQ][ELFEL_MAXBF];
+    double G[ELFEL_MAXQ][ELFEL_MAXBF][2];
+    double Np[ELFEL_MAXQ][3];  // T3 basis at the same points (Taylor-Hood pressure)
+    float wf[ELFEL_MAXQ];
+    float Nf[ELFEL_MAXQ][ELFEL_MAXBF];
+    float Gf[ELFEL_MAXQ][ELFEL_MAXBF][2];
+    float Npf[ELFEL_MAXQ][3];
+};
+
+// Coefficients of the weak form (uniform values + optional per-element arrays).
+struct Coef {
+    double kappa, Q, mu;
+    double D[9];  // column-major
+    const double *kappa_e, *Q_e, *scale_e;
+};
+
+// ---- device buffers ---------------------------------------------------------
+struct Buf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Field {
+    int nen = 0, ncomp = 0;
+    int64_t E = 0, Nn = 0, ndofs = 0;
+    bool has_mesh = false, has_dofs = false;
+    Buf conn, xy, dof;
+    uint64_t mesh_version = 0, dof_version = 0;
+};
+
+// Pattern cached for the REUSE method (built once per mesh + dof map).
+struct Pattern {
+    bool valid = false;
+    int kind = -1, fe = -1, layout = -1;
+    uint64_t key_mesh0 = 0, key_dof0 = 0, key_mesh1 = 0, key_dof1 = 0;
+    int64_t nrow = 0, ncol = 0, nnz = 0;
+    int64_t ntiles = 0, ntasks = 0, ngroups = 0;
+    int ncolmax = 0;             // bytes of pos per task
+    int maxrank = 0;
+    Buf ptr, ind;                // CSR pattern (int32)
+    Buf tile;                    // int4 per tile: task0, ntasks, v0, nvals | (row0 in .w)
+    Buf tile_rows;               // int2 per tile: row0, nrows
+    Buf task_elem;               // int32 per task
+    Buf task_info;               // uint32 per task
+    Buf task_pos;                // uint8 [ntasks][ncolmax]
+};
+
+struct Timing {
+    bool on = false;
+    double ms[ELFEL_NPHASES] = {0};
+    int64_t n[ELFEL_NPHASES] = {0};
+    std::vector<hipEvent_t> pool;
+    struct Pending { int phase; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+};
+
+struct elfel_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    Field f[2];
+    Buf kappa_e, Q_e, scale_e;
+    int64_t coef_E = 0;
+    bool has_kappa_e = false, has_Q_e = false, has_scale_e = false;
+    // scratch
+    Buf geom;                     // gathered element geometry
+    Buf keys[2], vals[2];         // COO keys/values (double-buffered for the sort)
+    Buf hist, scan_tmp, flags, scal;
+    Buf fkeys[2], fvals[2];
+    Buf stage64;                  // int64 staging for copy-out
+    // result
+    Buf ptr, ind, val, F;
+    int64_t nmaj = 0, nmin = 0, nnz = -1, nrow = 0, ncol = 0, nF = 0;
+    int layout = 0;
+    bool has_result = false, has_F = false;
+    bool result_is_pattern = false;   // ptr/ind live in the cached pattern
+    Pattern pat;
+    // device scalars: [0] nnz (int32), [1] error flags, [2..3] min det (double bits), [4..5] fallback count
+    Buf dscal;
+    Timing tm;
+    // solver scratch
+    Buf pcg;
+};
+
+// ---- error handling -------------------------------------------------------
+int set_err(elfel_hip_ctx *c, int code, const std::string &msg);
+#define HIPCHK(c, call)                                                        \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess)                                                  \
+            return set_err((c), ELFEL_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Grows a buffer to at least `bytes` (contents are not preserved).
+int ensure(elfel_hip_ctx *c, Buf &b, size_t bytes);
+void release(Buf &b);
+
+// ---- timing -----------------------------------------------------------------
+void tm_begin(elfel_hip_ctx *c, int phase, hipEvent_t *a);
+void tm_end(elfel_hip_ctx *c, int phase, hipEvent_t a);
+void tm_collect(elfel_hip_ctx *c);
+
+// ---- device error flags (dscal[1]) ----------------------------------------
+enum { DERR_RANGE = 1, DERR_GEOM = 2 };
+
+// ---- quadrature tables (tables.cpp) ---------------------------------------
+int make_qtable(int fe, int rule, QTable *t);
+
+// ---- sort.hip ---------------------------------------------------------------
+int bits_for(int64_t n);  // bits needed to hold 0..n-1
+int radix_sort_pairs(elfel_hip_ctx *c, uint64_t *&keys, uint64_t *&vals, uint64_t *&keys_alt,
+                     uint64_t *&vals_alt, int64_t n, int nbits);
+int exclusive_scan_u32(elfel_hip_ctx *c, uint32_t *data, int64_t n, uint32_t *total_dev);
+// From sorted (key, value) pairs build CSR: key = (major << minor_bits) | minor.
+int build_csr_from_sorted(elfel_hip_ctx *c, const uint64_t *keys, const double *vals, int64_t n,
+                          int minor_bits, int64_t nmaj, int32_t *ptr, int32_t *ind, double *val,
+                          int32_t *nnz_dev, bool values);
+// Sorted (row, value) pairs -> dense vector (rows without entries = 0).
+int build_dense_from_sorted(elfel_hip_ctx *c, const uint64_t *keys, const double *vals, int64_t n,
+                            int64_t ndofs, double *out);
+
+// ---- plain.hip (FP64, unfused arithmetic) -------------------------------------
+int plain_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf);
+int convert_indices(elfel_hip_ctx *c, const int64_t *src_dev, int32_t *dst, int64_t n, int64_t lo,
+                    int64_t hi);
+
+// ---- reuse.hip ----------------------------------------------------------------
+int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf);
+
+// ---- solve.hip ----------------------------------------------------------------
+int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int precision, double tol,
+              int maxit, int *iters, int *outer, double *relres);
+
+static inline int ceil_div(int64_t a, int64_t b) { return (int)((a + b - 1) / b); }

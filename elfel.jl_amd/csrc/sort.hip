@@ -1,0 +1,311 @@
+// sort.hip -- deterministic COO -> CSR build: stable LSD radix sort of 64-bit
+// keys with 64-bit payloads, then an in-order segmented reduce.
+//
+// This replaces SparseArrays.sparse called by finish!(SysmatAssemblerSparse)
+// (src/Assemblers.jl:121-123).  Stability of every pass keeps equal keys in
+// COO stream order, and each segment is summed sequentially from its first
+// entry, so duplicates are combined left to right exactly as the stream
+// presents them; explicit zeros stay structural entries.
+//
+// Radix pass (8-bit digits, tiles of 4096 keys, 256 threads = 4 wave64):
+//   k_rs_hist    per-tile digit histogram (per-wave LDS sub-histograms)
+//   scan         exclusive scan of the digit-major [256][ntiles] table
+//   k_rs_scatter stable rank: per round of 256 keys, lanes holding the same
+//                digit are matched with 8 ballots; ranks = popcount of lower
+//                peers + counts of earlier waves + earlier rounds of the tile.
+#include "common.h"
+
+#define RS_BLOCK 256
+#define RS_ITEMS 16
+#define RS_TILE (RS_BLOCK * RS_ITEMS)
+#define RS_RADIX 256
+
+#define SC_BLOCK 256
+#define SC_ITEMS 16
+#define SC_CHUNK (SC_BLOCK * SC_ITEMS)
+
+int bits_for(int64_t n)
+{
+    int b = 0;
+    while (b < 63 && ((int64_t)1 << b) < n) b++;
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan of uint32 (in place), chunk per block, recursive on sums.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane)
+{
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(SC_BLOCK) void k_scan_chunks(uint32_t *data, int64_t n, uint32_t *sums)
+{
+    __shared__ uint32_t s[SC_CHUNK];
+    __shared__ uint32_t wsum[SC_BLOCK / 64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int64_t base = (int64_t)blockIdx.x * SC_CHUNK;
+    for (int i = tid; i < SC_CHUNK; i += SC_BLOCK) {
+        int64_t g = base + i;
+        s[i] = g < n ? data[g] : 0u;
+    }
+    __syncthreads();
+    uint32_t loc[SC_ITEMS];
+    uint32_t tsum = 0;
+    for (int k = 0; k < SC_ITEMS; k++) {
+        loc[k] = tsum;
+        tsum += s[tid * SC_ITEMS + k];
+    }
+    uint32_t inc = wave_incl_scan(tsum, lane);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (int k = 0; k < w; k++) woff += wsum[k];
+    uint32_t excl = woff + inc - tsum;
+    for (int k = 0; k < SC_ITEMS; k++) s[tid * SC_ITEMS + k] = excl + loc[k];
+    if (tid == SC_BLOCK - 1) sums[blockIdx.x] = excl + tsum;
+    __syncthreads();
+    for (int i = tid; i < SC_CHUNK; i += SC_BLOCK) {
+        int64_t g = base + i;
+        if (g < n) data[g] = s[i];
+    }
+}
+
+__global__ void k_scan_add(uint32_t *data, int64_t n, const uint32_t *sums)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) data[g] += sums[g / SC_CHUNK];
+}
+
+__global__ void k_copy_u32(const uint32_t *src, uint32_t *dst) { *dst = *src; }
+
+// total_dev (optional) receives the sum of all n inputs.
+static int scan_rec(elfel_hip_ctx *c, uint32_t *data, int64_t n, uint32_t *tmp, int64_t tmp_cap,
+                    uint32_t *total_dev)
+{
+    if (n <= 0) return 0;
+    int64_t nb = (n + SC_CHUNK - 1) / SC_CHUNK;
+    if (nb + 1 > tmp_cap) return set_err(c, ELFEL_EINVAL, "scan scratch too small");
+    uint32_t *sums = tmp;
+    hipLaunchKernelGGL(k_scan_chunks, dim3((unsigned)nb), dim3(SC_BLOCK), 0, c->stream, data, n, sums);
+    HIPCHK(c, hipGetLastError());
+    if (nb > 1) {
+        // sums[nb] receives the grand total via the recursive call.
+        int rc = scan_rec(c, sums, nb, tmp + nb + 1, tmp_cap - nb - 1, total_dev);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_scan_add, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, data, n, sums);
+        HIPCHK(c, hipGetLastError());
+    } else if (total_dev) {
+        hipLaunchKernelGGL(k_copy_u32, dim3(1), dim3(1), 0, c->stream, sums, total_dev);
+        HIPCHK(c, hipGetLastError());
+    }
+    return 0;
+}
+
+int exclusive_scan_u32(elfel_hip_ctx *c, uint32_t *data, int64_t n, uint32_t *total_dev)
+{
+    // Scratch: nb + nb/4096 + ... < n/4000 + 64 entries.
+    int64_t cap = n / (SC_CHUNK - 1) + 64;
+    int rc = ensure(c, c->scan_tmp, sizeof(uint32_t) * cap);
+    if (rc) return rc;
+    return scan_rec(c, data, n, (uint32_t *)c->scan_tmp.p, cap, total_dev);
+}
+
+// ---------------------------------------------------------------------------
+// Radix sort
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint64_t *__restrict__ keys, int64_t n,
+                                                      int shift, uint32_t *__restrict__ hist, int ntiles)
+{
+    __shared__ uint32_t h[4][RS_RADIX];
+    const int tid = threadIdx.x, w = tid >> 6;
+    for (int i = tid; i < 4 * RS_RADIX; i += RS_BLOCK) (&h[0][0])[i] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; r++) {
+        int64_t idx = base + r * RS_BLOCK + tid;
+        if (idx < n) {
+            uint32_t d = (uint32_t)(keys[idx] >> shift) & (RS_RADIX - 1);
+            atomicAdd(&h[w][d], 1u);
+        }
+    }
+    __syncthreads();
+    hist[(int64_t)tid * ntiles + blockIdx.x] = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
+}
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint64_t *__restrict__ kin,
+                                                         const uint64_t *__restrict__ vin,
+                                                         uint64_t *__restrict__ kout,
+                                                         uint64_t *__restrict__ vout, int64_t n, int shift,
+                                                         const uint32_t *__restrict__ hist, int ntiles)
+{
+    __shared__ uint32_t base[RS_RADIX];
+    __shared__ uint32_t wc[4][RS_RADIX];
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    base[tid] = hist[(int64_t)tid * ntiles + blockIdx.x];
+    for (int ww = 0; ww < 4; ww++) wc[ww][tid] = 0;
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const int64_t idx = t0 + r * RS_BLOCK + tid;
+        const bool valid = idx < n;
+        uint64_t k = 0, v = 0;
+        if (valid) { k = kin[idx]; v = vin[idx]; }
+        const uint32_t d = (uint32_t)(k >> shift) & (RS_RADIX - 1);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        uint32_t acc = 0;
+        for (int ww = 0; ww < 4; ww++) {
+            uint32_t cnt = wc[ww][tid];
+            wc[ww][tid] = acc;
+            acc += cnt;
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t dst = base[d] + wc[w][d] + rank;
+            kout[dst] = k;
+            vout[dst] = v;
+        }
+        __syncthreads();
+        base[tid] += acc;
+        for (int ww = 0; ww < 4; ww++) wc[ww][tid] = 0;
+        __syncthreads();
+    }
+}
+
+int radix_sort_pairs(elfel_hip_ctx *c, uint64_t *&keys, uint64_t *&vals, uint64_t *&keys_alt,
+                     uint64_t *&vals_alt, int64_t n, int nbits)
+{
+    if (n <= 1 || nbits <= 0) return 0;
+    if (n >= ((int64_t)1 << 32)) return set_err(c, ELFEL_ENOTSUP, "sort of more than 2^32 keys");
+    const int ntiles = ceil_div(n, RS_TILE);
+    const int64_t nh = (int64_t)RS_RADIX * ntiles;
+    int rc = ensure(c, c->hist, sizeof(uint32_t) * nh);
+    if (rc) return rc;
+    uint32_t *hist = (uint32_t *)c->hist.p;
+    for (int shift = 0; shift < nbits; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_BLOCK), 0, c->stream, keys, n, shift, hist, ntiles);
+        HIPCHK(c, hipGetLastError());
+        rc = exclusive_scan_u32(c, hist, nh, nullptr);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_BLOCK), 0, c->stream, keys, vals, keys_alt,
+                           vals_alt, n, shift, hist, ntiles);
+        HIPCHK(c, hipGetLastError());
+        uint64_t *t = keys; keys = keys_alt; keys_alt = t;
+        t = vals; vals = vals_alt; vals_alt = t;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Segmented reduce of sorted pairs into CSR
+// ---------------------------------------------------------------------------
+__global__ void k_mark_heads(const uint64_t *__restrict__ keys, int64_t n, uint32_t *__restrict__ flags)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+// For each segment head i (segment index s = seg[i]): sum the segment left to
+// right, write column/value, and fill ptr for the majors ending at this head.
+__global__ void k_seg_reduce(const uint64_t *__restrict__ keys, const double *__restrict__ vals, int64_t n,
+                             const uint32_t *__restrict__ seg, int minor_bits, int64_t nmaj,
+                             int32_t *__restrict__ ptr, int32_t *__restrict__ ind, double *__restrict__ val,
+                             int32_t *__restrict__ nnz_dev, int with_values)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = keys[i];
+    const bool head = (i == 0) || (keys[i - 1] != k);
+    const uint64_t mmask = (minor_bits >= 64) ? ~0ull : ((1ull << minor_bits) - 1);
+    if (head) {
+        const uint32_t s = seg[i];
+        if (with_values) {
+            double acc = vals[i];
+            for (int64_t j = i + 1; j < n && keys[j] == k; j++) acc += vals[j];
+            val[s] = acc;
+        }
+        ind[s] = (int32_t)(k & mmask);
+        const int64_t m = (int64_t)(k >> minor_bits);
+        const int64_t mp = (i == 0) ? -1 : (int64_t)(keys[i - 1] >> minor_bits);
+        for (int64_t r = mp + 1; r <= m; r++) ptr[r] = (int32_t)s;
+    }
+    if (i == n - 1) {
+        const int32_t nnz = (int32_t)(seg[i] + (head ? 1u : 0u));
+        const int64_t m = (int64_t)(k >> minor_bits);
+        for (int64_t r = m + 1; r <= nmaj; r++) ptr[r] = nnz;
+        *nnz_dev = nnz;
+    }
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_f64(double *p, int64_t n, double v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int build_csr_from_sorted(elfel_hip_ctx *c, const uint64_t *keys, const double *vals, int64_t n,
+                          int minor_bits, int64_t nmaj, int32_t *ptr, int32_t *ind, double *val,
+                          int32_t *nnz_dev, bool values)
+{
+    if (n == 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(ceil_div(nmaj + 1, 256)), dim3(256), 0, c->stream, ptr, nmaj + 1, 0);
+        HIPCHK(c, hipGetLastError());
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, c->stream, nnz_dev, (int64_t)1, 0);
+        HIPCHK(c, hipGetLastError());
+        return 0;
+    }
+    int rc = ensure(c, c->flags, sizeof(uint32_t) * n);
+    if (rc) return rc;
+    uint32_t *seg = (uint32_t *)c->flags.p;
+    hipLaunchKernelGGL(k_mark_heads, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, keys, n, seg);
+    HIPCHK(c, hipGetLastError());
+    rc = exclusive_scan_u32(c, seg, n, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_seg_reduce, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, keys, vals, n, seg,
+                       minor_bits, nmaj, ptr, ind, val, nnz_dev, values ? 1 : 0);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+__global__ void k_dense_reduce(const uint64_t *__restrict__ keys, const double *__restrict__ vals, int64_t n,
+                               double *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = keys[i];
+    if (i > 0 && keys[i - 1] == k) return;
+    double acc = vals[i];
+    for (int64_t j = i + 1; j < n && keys[j] == k; j++) acc += vals[j];
+    out[k] = 0.0 + acc;  // start! zeros the vector, assemble! adds (src/Assemblers.jl:194-221)
+}
+
+int build_dense_from_sorted(elfel_hip_ctx *c, const uint64_t *keys, const double *vals, int64_t n,
+                            int64_t ndofs, double *out)
+{
+    hipLaunchKernelGGL(k_fill_f64, dim3(ceil_div(ndofs, 256)), dim3(256), 0, c->stream, out, ndofs, 0.0);
+    HIPCHK(c, hipGetLastError());
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_dense_reduce, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, keys, vals, n, out);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}

@@ -1,0 +1,246 @@
+"""ctypes binding of libelfel_hip.so (the C ABI declared in include/elfel_hip.h).
+
+There is no CPU fallback: if the library is missing or no HIP device is present
+the constructors raise, so nothing silently runs elsewhere.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.environ.get("ELFEL_HIP_LIB", os.path.join(os.path.dirname(_PKG), "libelfel_hip.so"))
+
+# status codes
+OK, EINVAL, ERANGE, ENOMEM, EHIP, ESTATE, EGEOM, ENOTSUP = 0, -1, -2, -3, -4, -5, -6, -7
+# enums
+HEAT, ELAST, STOKES_GEN, STOKES_VECLAP = 0, 1, 2, 3
+T3, T6, Q4 = 0, 1, 2
+FP64, MIXED = 0, 1
+SORT, REUSE = 0, 1
+CSR, CSC = 0, 1
+PH_GATHER, PH_ELEMENT, PH_SORT, PH_REDUCE, PH_NUMERIC, PH_SYMBOLIC, PH_TOTAL, PH_SOLVE = range(8)
+NPHASES = 8
+PHASE_NAMES = ["gather", "element", "sort", "reduce", "numeric", "symbolic", "total", "solve"]
+
+EXPORTED = [
+    "elfel_hip_init", "elfel_hip_free", "elfel_hip_last_error", "elfel_hip_device_name",
+    "elfel_hip_sparse", "elfel_hip_copy_sparse", "elfel_hip_result_info", "elfel_hip_vector",
+    "elfel_hip_set_mesh", "elfel_hip_set_dofs", "elfel_hip_set_coefficients", "elfel_hip_assemble",
+    "elfel_hip_copy_vector", "elfel_hip_synchronize", "elfel_hip_geometry_stats", "elfel_hip_pcg",
+    "elfel_hip_timing", "elfel_hip_timing_read",
+]
+
+
+class ElfelHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class ElfelRangeError(ElfelHipError, IndexError):
+    """Index out of range: Elfel raises BoundsError / ArgumentError here."""
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("fe", ctypes.c_int32), ("rule", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("method", ctypes.c_int32), ("layout", ctypes.c_int32),
+        ("kappa", ctypes.c_double), ("Q", ctypes.c_double), ("mu", ctypes.c_double),
+        ("D", ctypes.c_double * 9),
+    ]
+
+
+_lib = None
+
+
+def load():
+    """Loads libelfel_hip.so once; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise ElfelHipError(ESTATE, f"{LIBPATH} is not built; run __graft_entry__.build() or make -C elfel.jl_amd/csrc")
+    lib = ctypes.CDLL(LIBPATH)
+    P = ctypes.c_void_p
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    f64p = ctypes.POINTER(ctypes.c_double)
+    lib.elfel_hip_init.argtypes = [ctypes.c_int, ctypes.POINTER(P)]
+    lib.elfel_hip_free.argtypes = [P]
+    lib.elfel_hip_last_error.argtypes = [P]
+    lib.elfel_hip_last_error.restype = ctypes.c_char_p
+    lib.elfel_hip_device_name.argtypes = [P, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.elfel_hip_sparse.argtypes = [P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, i64p, i64p, f64p,
+                                     ctypes.c_int, i64p]
+    lib.elfel_hip_copy_sparse.argtypes = [P, ctypes.c_int, i64p, i64p, f64p]
+    lib.elfel_hip_result_info.argtypes = [P, i64p, i64p, i64p, ctypes.POINTER(ctypes.c_int)]
+    lib.elfel_hip_vector.argtypes = [P, ctypes.c_int64, ctypes.c_int64, i64p, f64p, f64p]
+    lib.elfel_hip_set_mesh.argtypes = [P, ctypes.c_int, ctypes.c_int32, ctypes.c_int64, i64p, ctypes.c_int64, f64p]
+    lib.elfel_hip_set_dofs.argtypes = [P, ctypes.c_int, ctypes.c_int32, ctypes.c_int64, i64p, ctypes.c_int64]
+    lib.elfel_hip_set_coefficients.argtypes = [P, ctypes.c_int64, f64p, f64p, f64p]
+    lib.elfel_hip_assemble.argtypes = [P, ctypes.POINTER(Problem), i64p]
+    lib.elfel_hip_copy_vector.argtypes = [P, f64p, ctypes.c_int64]
+    lib.elfel_hip_synchronize.argtypes = [P]
+    lib.elfel_hip_geometry_stats.argtypes = [P, f64p, i64p]
+    lib.elfel_hip_pcg.argtypes = [P, ctypes.c_int64, f64p, f64p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), f64p]
+    lib.elfel_hip_timing.argtypes = [P, ctypes.c_int]
+    lib.elfel_hip_timing_read.argtypes = [P, f64p, i64p]
+    for name in EXPORTED:
+        if name != "elfel_hip_last_error":
+            getattr(lib, name).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a, t):
+    if a is None:
+        return ctypes.cast(None, ctypes.POINTER(t))
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+class Context:
+    """One device context = one HIP stream on one GPU (elfel_hip_init)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        rc = self.lib.elfel_hip_init(device, ctypes.byref(h))
+        if rc != OK:
+            raise ElfelHipError(rc, f"elfel_hip_init({device}) failed: no usable HIP device")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.elfel_hip_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != OK:
+            msg = self.lib.elfel_hip_last_error(self.h).decode(errors="replace")
+            cls = ElfelRangeError if rc == ERANGE else ElfelHipError
+            raise cls(rc, f"{what}: {msg}")
+
+    def device_name(self):
+        buf = ctypes.create_string_buffer(256)
+        cus = ctypes.c_int(0)
+        self._chk(self.lib.elfel_hip_device_name(self.h, buf, 256, ctypes.byref(cus)), "device_name")
+        return buf.value.decode(), cus.value
+
+    # ---- generic drop-in --------------------------------------------------
+    def sparse(self, I, J, V, nrow, ncol, layout=CSC):
+        I, J, V = _i64(I), _i64(J), _f64(V)
+        if not (I.size == J.size == V.size):
+            raise ValueError("row, col and val must have the same length")
+        nnz = ctypes.c_int64(0)
+        self._chk(self.lib.elfel_hip_sparse(self.h, nrow, ncol, I.size, _ptr(I, ctypes.c_int64),
+                                            _ptr(J, ctypes.c_int64), _ptr(V, ctypes.c_double), layout,
+                                            ctypes.byref(nnz)), "sparse")
+        return nnz.value
+
+    def result_info(self):
+        nr, nc, nnz, lay = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        self._chk(self.lib.elfel_hip_result_info(self.h, ctypes.byref(nr), ctypes.byref(nc), ctypes.byref(nnz),
+                                                 ctypes.byref(lay)), "result_info")
+        return nr.value, nc.value, nnz.value, lay.value
+
+    def copy_sparse(self, base=0):
+        nr, nc, nnz, lay = self.result_info()
+        nmaj = nc if lay == CSC else nr
+        ptr = np.empty(nmaj + 1, np.int64)
+        ind = np.empty(nnz, np.int64)
+        val = np.empty(nnz, np.float64)
+        self._chk(self.lib.elfel_hip_copy_sparse(self.h, base, _ptr(ptr, ctypes.c_int64), _ptr(ind, ctypes.c_int64),
+                                                 _ptr(val, ctypes.c_double)), "copy_sparse")
+        return ptr, ind, val, (nr, nc), lay
+
+    def vector(self, idx, val, ndofs):
+        idx, val = _i64(idx), _f64(val)
+        out = np.empty(ndofs, np.float64)
+        self._chk(self.lib.elfel_hip_vector(self.h, ndofs, idx.size, _ptr(idx, ctypes.c_int64),
+                                            _ptr(val, ctypes.c_double), _ptr(out, ctypes.c_double)), "vector")
+        return out
+
+    # ---- device-resident problem ----------------------------------------------
+    def set_mesh(self, field, conn, xy):
+        conn, xy = _i64(conn), _f64(xy)
+        E, nen = conn.shape
+        self._chk(self.lib.elfel_hip_set_mesh(self.h, field, nen, E, _ptr(conn, ctypes.c_int64), xy.shape[0],
+                                              _ptr(xy, ctypes.c_double)), "set_mesh")
+
+    def set_dofs(self, field, dofnums, ndofs):
+        d = _i64(dofnums)
+        if d.ndim == 1:
+            d = d.reshape(-1, 1)
+        self._chk(self.lib.elfel_hip_set_dofs(self.h, field, d.shape[1], d.shape[0], _ptr(d, ctypes.c_int64), ndofs),
+                  "set_dofs")
+
+    def set_coefficients(self, E, kappa_e=None, Q_e=None, scale_e=None):
+        ks = [None if a is None else _f64(a) for a in (kappa_e, Q_e, scale_e)]
+        for a in ks:
+            if a is not None and a.size != E:
+                raise ValueError("per-element coefficient array has the wrong length")
+        self._keep_coef = ks
+        self._chk(self.lib.elfel_hip_set_coefficients(self.h, E, *[_ptr(a, ctypes.c_double) for a in ks]),
+                  "set_coefficients")
+
+    def assemble(self, kind, fe, rule, precision=FP64, method=SORT, layout=CSR, kappa=1.0, Q=0.0, mu=1.0,
+                 D=None, wait=True):
+        pb = Problem()
+        pb.kind, pb.fe, pb.rule = kind, fe, rule
+        pb.precision, pb.method, pb.layout = precision, method, layout
+        pb.kappa, pb.Q, pb.mu = kappa, Q, mu
+        if D is not None:
+            Dc = np.asarray(D, np.float64).reshape(3, 3).ravel(order="F")
+            for i in range(9):
+                pb.D[i] = Dc[i]
+        self._pb = pb
+        nnz = ctypes.c_int64(0)
+        self._chk(self.lib.elfel_hip_assemble(self.h, ctypes.byref(pb), ctypes.byref(nnz) if wait else None),
+                  "assemble")
+        return nnz.value if wait else None
+
+    def copy_vector(self, n):
+        F = np.empty(n, np.float64)
+        self._chk(self.lib.elfel_hip_copy_vector(self.h, _ptr(F, ctypes.c_double), n), "copy_vector")
+        return F
+
+    def synchronize(self):
+        self._chk(self.lib.elfel_hip_synchronize(self.h), "synchronize")
+
+    def geometry_stats(self):
+        md, nf = ctypes.c_double(), ctypes.c_int64()
+        self._chk(self.lib.elfel_hip_geometry_stats(self.h, ctypes.byref(md), ctypes.byref(nf)), "geometry_stats")
+        return md.value, nf.value
+
+    def pcg(self, nfree, F, x, precision=FP64, tol=1e-10, maxit=100000):
+        F = _f64(F)
+        x = _f64(x).copy()
+        it, outer, rr = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._chk(self.lib.elfel_hip_pcg(self.h, nfree, _ptr(F, ctypes.c_double), _ptr(x, ctypes.c_double), precision,
+                                         tol, maxit, ctypes.byref(it), ctypes.byref(outer), ctypes.byref(rr)), "pcg")
+        return x, it.value, outer.value, rr.value
+
+    def timing(self, enable=True):
+        self._chk(self.lib.elfel_hip_timing(self.h, 1 if enable else 0), "timing")
+
+    def timing_read(self):
+        ms = np.zeros(NPHASES)
+        n = np.zeros(NPHASES, np.int64)
+        self._chk(self.lib.elfel_hip_timing_read(self.h, _ptr(ms, ctypes.c_double), _ptr(n, ctypes.c_int64)),
+                  "timing_read")
+        return {PHASE_NAMES[i]: (float(ms[i]), int(n[i])) for i in range(NPHASES)}

@@ -1,0 +1,148 @@
+/*
+ * elfel_hip.h -- C ABI of libelfel_hip.so, the MI355X (gfx950) element
+ * integration + global assembly library behind Elfel.jl's assembler API.
+ *
+ * Plain C: no C++ or torch types cross this boundary.  Every function returns
+ * an int status (0 = ELFEL_OK, < 0 = error class); elfel_hip_last_error()
+ * gives the text.  No C++ exception crosses the ABI.  One context = one HIP
+ * stream on one device; calls on a context are serialised by the caller
+ * (Elfel's assemblers are mutable and single-threaded, src/Assemblers.jl).
+ *
+ * Index conventions follow Julia: connectivity, dof numbers and COO indices
+ * are 1-based Int64, laid out exactly as the Julia arrays are in memory
+ * (Vector{SVector{nen,Int64}} -> conn[e*nen + k]; Vector{SVector{2,Float64}}
+ * -> xy[2*n + d]; FEField.dofnums -> dofnums[ncomp*n + c]).  Sparse outputs
+ * are copied as 1-based Int64 (ready for SparseMatrixCSC) or 0-based.
+ */
+#ifndef ELFEL_HIP_H
+#define ELFEL_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct elfel_hip_ctx elfel_hip_ctx;
+
+/* status codes */
+enum {
+    ELFEL_OK = 0,
+    ELFEL_EINVAL = -1,   /* bad argument (null pointer, bad size, bad enum) */
+    ELFEL_ERANGE = -2,   /* index out of range: sparse() ArgumentError / BoundsError */
+    ELFEL_ENOMEM = -3,   /* device allocation failed */
+    ELFEL_EHIP = -4,     /* HIP runtime error */
+    ELFEL_ESTATE = -5,   /* call out of order (no mesh, no dofs, no result) */
+    ELFEL_EGEOM = -6,    /* element with non-positive Jacobian determinant */
+    ELFEL_ENOTSUP = -7   /* combination not supported */
+};
+
+/* weak forms */
+enum { ELFEL_HEAT = 0, ELFEL_ELAST = 1, ELFEL_STOKES_GEN = 2, ELFEL_STOKES_VECLAP = 3 };
+/* scalar finite elements */
+enum { ELFEL_T3 = 0, ELFEL_T6 = 1, ELFEL_Q4 = 2 };
+/* arithmetic of the element kernels */
+enum { ELFEL_FP64 = 0, ELFEL_MIXED = 1 };
+/* global build method */
+enum { ELFEL_SORT = 0,    /* COO in stream order -> stable radix sort -> in-order segmented reduce */
+       ELFEL_REUSE = 1 }; /* cached pattern + fused row-owner kernel (no sort, no atomics) */
+/* sparse layout of the result */
+enum { ELFEL_CSR = 0, ELFEL_CSC = 1 };
+/* timing phases (elfel_hip_timing_read) */
+enum { ELFEL_PH_GATHER = 0, ELFEL_PH_ELEMENT = 1, ELFEL_PH_SORT = 2, ELFEL_PH_REDUCE = 3,
+       ELFEL_PH_NUMERIC = 4, ELFEL_PH_SYMBOLIC = 5, ELFEL_PH_TOTAL = 6, ELFEL_PH_SOLVE = 7,
+       ELFEL_NPHASES = 8 };
+
+/* ---- context -------------------------------------------------------------
+ * No counterpart in the reference (Julia needs no device context); the Julia
+ * assembler object that owns it is SysmatAssemblerHIP. */
+int elfel_hip_init(int device, elfel_hip_ctx **ctx);
+int elfel_hip_free(elfel_hip_ctx *ctx);
+const char *elfel_hip_last_error(const elfel_hip_ctx *ctx);
+int elfel_hip_device_name(elfel_hip_ctx *ctx, char *buf, int len, int *num_cus);
+
+/* ---- generic drop-in for SysmatAssemblerSparse ----------------------------
+ * Replaces finish!(self::SysmatAssemblerSparse) = sparse(row, col, val, nrow,
+ * ncol) at src/Assemblers.jl:121-123, for a COO stream gathered on the host by
+ * assemble!(a, r, c, v) (:85-90), assemble!(a, lma) (:97-102) and
+ * assemble!(a, transpose(lma)) (:109-114).  Duplicates are summed in stream
+ * order; explicit zeros stay structural; out-of-range indices -> ELFEL_ERANGE. */
+int elfel_hip_sparse(elfel_hip_ctx *ctx, int64_t nrow, int64_t ncol, int64_t n,
+                     const int64_t *row, const int64_t *col, const double *val,
+                     int layout, int64_t *nnz);
+
+/* Copies the matrix built by the last elfel_hip_sparse / elfel_hip_assemble.
+ * base = 1: 1-based Int64 (SparseMatrixCSC colptr/rowval when layout = CSC);
+ * base = 0: 0-based.  ptr has nmajor+1 entries, ind and val nnz entries. */
+int elfel_hip_copy_sparse(elfel_hip_ctx *ctx, int base, int64_t *ptr, int64_t *ind, double *val);
+int elfel_hip_result_info(elfel_hip_ctx *ctx, int64_t *nrow, int64_t *ncol, int64_t *nnz, int *layout);
+
+/* ---- generic drop-in for SysvecAssembler ----------------------------------
+ * Replaces start!/assemble!/finish! of SysvecAssembler (src/Assemblers.jl:
+ * 194-230): out[i-1] = sum of val[k] with idx[k] == i, added in stream order
+ * from 0.0.  Index outside 1..ndofs -> ELFEL_ERANGE (BoundsError at :218). */
+int elfel_hip_vector(elfel_hip_ctx *ctx, int64_t ndofs, int64_t n, const int64_t *idx,
+                     const double *val, double *out);
+
+/* ---- device-resident problem description ----------------------------------
+ * field 0: the primary (velocity / temperature / displacement) space;
+ * field 1: the pressure space of a Taylor-Hood pair (T3 on the corners).
+ * Replaces what FEIterator gathers per element (src/FEIterators.jl:54-81,
+ * 161-194): base incidence relation (conn), "geom" attribute (xy) and
+ * FEField.dofnums (src/FEFields.jl:15).  Dof 0 (unnumbered) -> ELFEL_ERANGE. */
+int elfel_hip_set_mesh(elfel_hip_ctx *ctx, int field, int32_t nen, int64_t E, const int64_t *conn,
+                       int64_t Nn, const double *xy);
+int elfel_hip_set_dofs(elfel_hip_ctx *ctx, int field, int32_t ncomp, int64_t Nn,
+                       const int64_t *dofnums, int64_t ndofs);
+/* Per-element coefficients (any pointer may be NULL = uniform value of the
+ * problem): conductivity, heat source, scale of the material matrix D. */
+int elfel_hip_set_coefficients(elfel_hip_ctx *ctx, int64_t E, const double *kappa_e,
+                               const double *Q_e, const double *scale_e);
+
+typedef struct {
+    int32_t kind;       /* ELFEL_HEAT / ELFEL_ELAST / ELFEL_STOKES_GEN / ELFEL_STOKES_VECLAP */
+    int32_t fe;         /* ELFEL_T3 / ELFEL_T6 / ELFEL_Q4 (velocity element for Stokes) */
+    int32_t rule;       /* triangle: npts (1 or 3); square: Gauss order (1 or 2) */
+    int32_t precision;  /* ELFEL_FP64 / ELFEL_MIXED */
+    int32_t method;     /* ELFEL_SORT / ELFEL_REUSE */
+    int32_t layout;     /* ELFEL_CSR / ELFEL_CSC */
+    double kappa, Q;    /* heat: uniform conductivity and source */
+    double mu;          /* Stokes vector-Laplacian viscosity */
+    double D[9];        /* elasticity / Stokes-gen material matrix, column-major */
+} elfel_hip_problem;
+
+/* Integrates every element and builds the global matrix (and the load vector
+ * for heat) on the device.  Replaces the user integrate! loops with
+ * LocalMatrixAssembler + SysmatAssemblerSparse (test/test_heat.jl:36-71,
+ * test/test_elasticity.jl:149-181, examples/stokes/colliding_flow/
+ * ht_p2_p1_gen.jl:46-90, test/test_stokes.jl:629-668).  nnz may be NULL:
+ * then the call does not wait for the device. */
+int elfel_hip_assemble(elfel_hip_ctx *ctx, const elfel_hip_problem *pb, int64_t *nnz);
+/* Load vector of the last heat assembly (finish!(SysvecAssembler), :228-230). */
+int elfel_hip_copy_vector(elfel_hip_ctx *ctx, double *F, int64_t n);
+/* Blocks until the context's stream is idle. */
+int elfel_hip_synchronize(elfel_hip_ctx *ctx);
+/* Smallest Jacobian determinant seen at any quadrature point by the last
+ * assembly, and the number of elements routed to the FP64 branch of the
+ * mixed kernels. */
+int elfel_hip_geometry_stats(elfel_hip_ctx *ctx, double *min_det, int64_t *n_fp64_fallback);
+
+/* ---- solve --------------------------------------------------------------
+ * Replaces the driver's solve! (test/test_heat.jl:73-76: T[1:nu] =
+ * K[1:nu,1:nu] \ (F - K*T)[1:nu]) for SPD problems with Jacobi-PCG on the
+ * matrix of the last assembly (CSR).  x holds the data values on entry
+ * (x[nfree:]) and the solution on return.  precision = ELFEL_FP64: FP64 PCG;
+ * ELFEL_MIXED: FP32 PCG inner iterations inside FP64 iterative refinement.
+ * Convergence: ||b - A x|| / ||b|| <= tol in FP64 on the free block. */
+int elfel_hip_pcg(elfel_hip_ctx *ctx, int64_t nfree, const double *F, double *x, int precision,
+                  double tol, int maxit, int *iters, int *outer, double *relres);
+
+/* ---- timing ---------------------------------------------------------------
+ * hipEvent timing of each phase on the context's stream, accumulated while
+ * enabled.  ms[ELFEL_NPHASES], counts[ELFEL_NPHASES] (launch counts). */
+int elfel_hip_timing(elfel_hip_ctx *ctx, int enable);
+int elfel_hip_timing_read(elfel_hip_ctx *ctx, double *ms, int64_t *counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
