@@ -354,6 +354,8 @@ extern "C" int elfel_hip_set_mesh(elfel_hip_ctx *c, int field, int32_t nen, int6
     if ((rc = take_flags(c, &flags))) return rc;
     if (flags & DERR_RANGE) return set_err(c, ELFEL_ERANGE, "connectivity refers to a node outside 1..Nn");
     f.nen = nen; f.E = E; f.Nn = Nn;
+    f.h_conn.resize((size_t)(E * nen));
+    for (int64_t i = 0; i < E * nen; i++) f.h_conn[i] = (int32_t)(conn[i] - 1);
     f.has_mesh = true;
     f.mesh_version++;
     c->pat.valid = false;
@@ -380,6 +382,8 @@ extern "C" int elfel_hip_set_dofs(elfel_hip_ctx *c, int field, int32_t ncomp, in
         return set_err(c, ELFEL_ERANGE, "dof number outside 1..ndofs (unnumbered dof 0?)");
     f.ncomp = ncomp; f.ndofs = ndofs;
     if (f.Nn != Nn && f.has_mesh) return set_err(c, ELFEL_EINVAL, "dof map size does not match the mesh");
+    f.h_dof.resize((size_t)n);
+    for (int64_t i = 0; i < n; i++) f.h_dof[i] = (int32_t)(dofnums[i] - 1);
     f.has_dofs = true;
     f.dof_version++;
     c->pat.valid = false;

@@ -56,6 +56,7 @@ struct Field {
     int64_t E = 0, Nn = 0, ndofs = 0;
     bool has_mesh = false, has_dofs = false;
     Buf conn, xy, dof;
+    std::vector<int32_t> h_conn, h_dof;  // host copies (0-based) for the symbolic phase
     uint64_t mesh_version = 0, dof_version = 0;
 };
 
@@ -68,6 +69,8 @@ struct Pattern {
     int64_t ntiles = 0, ntasks = 0, ngroups = 0;
     int ncolmax = 0;             // bytes of pos per task
     int maxrank = 0;
+    int lds_vals = 0;            // largest tile value count
+    double symbolic_ms = 0;      // host wall time of the last symbolic build
     Buf ptr, ind;                // CSR pattern (int32)
     Buf tile;                    // int4 per tile: task0, ntasks, v0, nvals | (row0 in .w)
     Buf tile_rows;               // int2 per tile: row0, nrows

@@ -1,14 +1,356 @@
-// reuse.hip -- cached-pattern fused row-owner kernels (placeholder).
-#include "elem.h"
+// reuse.hip -- the REUSE method: symbolic phase (pattern, tiles, task tables)
+// built once per mesh + dof map, then one gather + one fused owner-computes
+// kernel per assembly (reuse_kernel.h).
+//
+// Symbolic phase:
+//   1. the pattern (ptr/ind) comes from the plain method on the device: the
+//      union of the element blocks including structural zeros, exactly the
+//      pattern sparse() gives the reference's COO stream;
+//   2. node -> element adjacency, in element order (a stable counting sort);
+//   3. row groups (consecutive dofs of one node) in dof order, packed into
+//      tiles of <= REUSE_BLOCK tasks and <= REUSE_VCAP values;
+//   4. per task, the position of every element column in the owner's sorted
+//      column list (binary search).
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+
+#include "reuse_kernel.h"
+
+namespace {
+
+struct Adj {
+    std::vector<int32_t> ptr, elem;
+    std::vector<uint8_t> loc;
+};
+
+void adjacency(const Field &f, Adj &A)
+{
+    const int64_t Nn = f.Nn, E = f.E;
+    const int nen = f.nen;
+    A.ptr.assign(Nn + 1, 0);
+    for (int64_t i = 0; i < E * nen; i++) A.ptr[f.h_conn[i] + 1]++;
+    for (int64_t n = 0; n < Nn; n++) A.ptr[n + 1] += A.ptr[n];
+    std::vector<int32_t> fill(A.ptr.begin(), A.ptr.end() - 1);
+    A.elem.resize(E * nen);
+    A.loc.resize(E * nen);
+    for (int64_t e = 0; e < E; e++)
+        for (int k = 0; k < nen; k++) {
+            const int32_t n = f.h_conn[e * nen + k];
+            const int32_t p = fill[n]++;
+            A.elem[p] = (int32_t)e;
+            A.loc[p] = (uint8_t)k;
+        }
+}
+
+}  // namespace
+
+static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    Pattern &P = c->pat;
+    P.valid = false;
+    const bool stokes = pb->kind == ELFEL_STOKES_GEN || pb->kind == ELFEL_STOKES_VECLAP;
+    const Field &F0 = c->f[0];
+    const Field *F1 = stokes ? &c->f[1] : nullptr;
+    const int nen = F0.nen, nc = F0.ncomp, nd = nen * nc;
+    const int ncol = nd + (stokes ? 3 : 0);
+    const int ncolp = (ncol + 3) & ~3;
+    const int64_t n = c->nrow;
+
+    // 1. pattern from the plain method
+    elfel_hip_problem p2 = *pb;
+    p2.method = ELFEL_SORT;
+    p2.precision = ELFEL_FP64;
+    int rc = plain_assemble(c, &p2, tab, cf);
+    if (rc) return rc;
+    int32_t hnnz = 0;
+    HIPCHK(c, hipMemcpyAsync(&hnnz, c->dscal.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int64_t nnz = hnnz;
+    std::vector<int32_t> hptr(n + 1), hind(nnz);
+    HIPCHK(c, hipMemcpy(hptr.data(), c->ptr.p, 4 * (n + 1), hipMemcpyDeviceToHost));
+    if (nnz) HIPCHK(c, hipMemcpy(hind.data(), c->ind.p, 4 * nnz, hipMemcpyDeviceToHost));
+    if ((rc = ensure(c, P.ptr, 4 * (n + 1))) || (rc = ensure(c, P.ind, 4 * std::max<int64_t>(nnz, 1)))) return rc;
+    HIPCHK(c, hipMemcpyAsync(P.ptr.p, c->ptr.p, 4 * (n + 1), hipMemcpyDeviceToDevice, c->stream));
+    if (nnz) HIPCHK(c, hipMemcpyAsync(P.ind.p, c->ind.p, 4 * nnz, hipMemcpyDeviceToDevice, c->stream));
+
+    // 2. adjacency
+    Adj A0, A1;
+    adjacency(F0, A0);
+    if (stokes) adjacency(*F1, A1);
+
+    // inverse dof map: dof -> (field, node, comp)
+    std::vector<int32_t> inv_node(n, -1);
+    std::vector<uint8_t> inv_fc(n, 0);  // field << 4 | comp
+    for (int64_t nd0 = 0; nd0 < F0.Nn; nd0++)
+        for (int cc = 0; cc < nc; cc++) {
+            const int32_t d = F0.h_dof[nd0 * nc + cc];
+            inv_node[d] = (int32_t)nd0;
+            inv_fc[d] = (uint8_t)cc;
+        }
+    if (stokes)
+        for (int64_t m = 0; m < F1->Nn; m++) {
+            const int32_t d = F1->h_dof[m];
+            inv_node[d] = (int32_t)m;
+            inv_fc[d] = (uint8_t)(1 << 4);
+        }
+
+    // 3-4. groups, tiles, tasks, positions
+    std::vector<int4> tiles;
+    std::vector<int4> trows;
+    std::vector<int32_t> t_elem;
+    std::vector<uint32_t> t_info;
+    std::vector<uint8_t> t_pos;
+    t_elem.reserve((size_t)F0.E * nen + (stokes ? F0.E * 3 : 0));
+    t_info.reserve(t_elem.capacity());
+    t_pos.reserve(t_elem.capacity() * ncolp);
+    int4 cur = {0, 0, 0, 0}, cr = {0, 0, 0, 0};
+    bool open = false;
+    int maxv = 0, maxrank = 0;
+    auto close = [&]() {
+        if (!open) return;
+        tiles.push_back(cur);
+        trows.push_back(cr);
+        maxv = std::max(maxv, cur.w);
+        open = false;
+    };
+    int64_t r = 0;
+    int32_t colbuf[16];
+    while (r < n) {
+        const int32_t node = inv_node[r];
+        const int field = inv_fc[r] >> 4, comp = inv_fc[r] & 15;
+        int nr = 1;
+        if (node >= 0 && field == 0 && nc == 2 && comp == 0 && r + 1 < n && inv_node[r + 1] == node &&
+            inv_fc[r + 1] == 1)
+            nr = 2;
+        const int L = hptr[r + 1] - hptr[r];
+        if (nr == 2 && hptr[r + 2] - hptr[r + 1] != L) return set_err(c, ELFEL_ENOTSUP, "rows of one node differ in pattern");
+        int deg = 0;
+        const Adj *A = nullptr;
+        if (node >= 0) {
+            A = field == 0 ? &A0 : &A1;
+            deg = A->ptr[node + 1] - A->ptr[node];
+        }
+        if (deg > REUSE_BLOCK || nr * L > REUSE_VCAP || deg > 16 || L > 255)
+            return set_err(c, ELFEL_ENOTSUP, "node too highly connected for the reuse tiles");
+        if (open && (cur.y + deg > REUSE_BLOCK || cur.w + nr * L > REUSE_VCAP || cr.y + nr > REUSE_BLOCK)) close();
+        if (!open) {
+            cur = int4{(int)t_elem.size(), 0, hptr[r], 0};
+            cr = int4{(int)r, 0, 0, 0};
+            open = true;
+        }
+        const int vbase = cur.w;
+        const int rowcode = nr == 2 ? 2 : (comp == 1 ? 1 : 0);
+        const int l8 = nr == 2 ? L : cr.y;
+        const int32_t *list = hind.data() + hptr[r];
+        for (int s = 0; s < deg; s++) {
+            const int32_t e = A->elem[A->ptr[node] + s];
+            const int a = A->loc[A->ptr[node] + s];
+            // element columns: velocity/primary dofs, then pressure dofs (Stokes velocity owner)
+            int nj = 0;
+            for (int k = 0; k < nen; k++)
+                for (int cc = 0; cc < nc; cc++) colbuf[nj++] = F0.h_dof[(int64_t)F0.h_conn[(int64_t)e * nen + k] * nc + cc];
+            if (stokes && field == 0)
+                for (int k = 0; k < 3; k++) colbuf[nj++] = F1->h_dof[F1->h_conn[(int64_t)e * 3 + k]];
+            for (int j = 0; j < ncolp; j++) {
+                uint8_t pv = 0;
+                if (j < nj) {
+                    const int32_t *it = std::lower_bound(list, list + L, colbuf[j]);
+                    if (it == list + L || *it != colbuf[j])
+                        return set_err(c, ELFEL_EINVAL, "internal: element column missing from the pattern");
+                    pv = (uint8_t)(it - list);
+                }
+                t_pos.push_back(pv);
+            }
+            t_elem.push_back(e);
+            t_info.push_back((uint32_t)a | ((uint32_t)s << 3) | ((uint32_t)rowcode << 7) |
+                             ((uint32_t)(field == 1) << 9) | ((uint32_t)vbase << 10) | ((uint32_t)l8 << 24));
+            maxrank = std::max(maxrank, s + 1);
+            cr.z = std::max(cr.z, s + 1);
+        }
+        cur.y += deg;
+        cur.w += nr * L;
+        cr.y += nr;
+        r += nr;
+    }
+    close();
+    const int64_t ntasks = (int64_t)t_elem.size();
+    if ((rc = ensure(c, P.tile, sizeof(int4) * std::max<size_t>(tiles.size(), 1))) ||
+        (rc = ensure(c, P.tile_rows, sizeof(int4) * std::max<size_t>(trows.size(), 1))) ||
+        (rc = ensure(c, P.task_elem, 4 * std::max<int64_t>(ntasks, 1))) ||
+        (rc = ensure(c, P.task_info, 4 * std::max<int64_t>(ntasks, 1))) ||
+        (rc = ensure(c, P.task_pos, std::max<size_t>(t_pos.size(), 1))))
+        return rc;
+    if (!tiles.empty()) {
+        HIPCHK(c, hipMemcpy(P.tile.p, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(P.tile_rows.p, trows.data(), sizeof(int4) * trows.size(), hipMemcpyHostToDevice));
+    }
+    if (ntasks) {
+        HIPCHK(c, hipMemcpy(P.task_elem.p, t_elem.data(), 4 * ntasks, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(P.task_info.p, t_info.data(), 4 * ntasks, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(P.task_pos.p, t_pos.data(), t_pos.size(), hipMemcpyHostToDevice));
+    }
+    P.kind = pb->kind; P.fe = pb->fe; P.layout = pb->layout;
+    P.key_mesh0 = F0.mesh_version; P.key_dof0 = F0.dof_version;
+    P.key_mesh1 = stokes ? F1->mesh_version : 0; P.key_dof1 = stokes ? F1->dof_version : 0;
+    P.nrow = P.ncol = n;
+    P.nnz = nnz;
+    P.ntiles = (int64_t)tiles.size();
+    P.ntasks = ntasks;
+    P.ncolmax = ncolp;
+    P.maxrank = maxrank;
+    P.lds_vals = maxv;
+    P.valid = true;
+    P.symbolic_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
 
 int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf)
 {
-    (void)pb; (void)tab; (void)cf;
-    return set_err(c, ELFEL_ENOTSUP, "reuse method not built yet");
+    const bool stokes = pb->kind == ELFEL_STOKES_GEN || pb->kind == ELFEL_STOKES_VECLAP;
+    const Field &F0 = c->f[0];
+    Pattern &P = c->pat;
+    const bool fresh = P.valid && P.kind == pb->kind && P.fe == pb->fe && P.layout == pb->layout &&
+                       P.key_mesh0 == F0.mesh_version && P.key_dof0 == F0.dof_version &&
+                       (!stokes || (P.key_mesh1 == c->f[1].mesh_version && P.key_dof1 == c->f[1].dof_version)) &&
+                       P.nrow == c->nrow;
+    const bool mixed = pb->precision == ELFEL_MIXED;
+    reuse_launch_fn fn = mixed ? pick_reuse32(pb->kind, pb->fe, tab.nq, pb->layout == ELFEL_CSC)
+                               : pick_reuse64(pb->kind, pb->fe, tab.nq, pb->layout == ELFEL_CSC);
+    if (!fn) return set_err(c, ELFEL_ENOTSUP, "unsupported (kind, element, rule) for the reuse method");
+    int rc;
+    if (!fresh) {
+        hipEvent_t ev = nullptr;
+        tm_begin(c, ELFEL_PH_SYMBOLIC, &ev);
+        rc = build_symbolic(c, pb, tab, cf);
+        tm_end(c, ELFEL_PH_SYMBOLIC, ev);
+        if (rc) return rc;
+    }
+    const int64_t E = F0.E;
+    const int nen = F0.nen;
+    if ((rc = ensure(c, c->geom, (size_t)E * nen * 2 * (mixed ? 4 : 8))) ||
+        (rc = ensure(c, c->val, 8 * std::max<int64_t>(P.nnz, 1))))
+        return rc;
+    const bool heat = pb->kind == ELFEL_HEAT;
+    if (heat && (rc = ensure(c, c->F, 8 * std::max<int64_t>(c->nrow, 1)))) return rc;
+    hipEvent_t ev = nullptr;
+    tm_begin(c, ELFEL_PH_GATHER, &ev);
+    if (mixed)
+        rc = gather_geom32(c, nen, E, (const int32_t *)F0.conn.p, (const double2 *)F0.xy.p, (float *)c->geom.p);
+    else
+        rc = gather_geom64(c, nen, E, (const int32_t *)F0.conn.p, (const double2 *)F0.xy.p, (double *)c->geom.p);
+    if (rc) return rc;
+    tm_end(c, ELFEL_PH_GATHER, ev);
+    ReuseArgs A;
+    A.tiles = (const int4 *)P.tile.p;
+    A.tile_rows = (const int4 *)P.tile_rows.p;
+    A.task_elem = (const int32_t *)P.task_elem.p;
+    A.task_info = (const uint32_t *)P.task_info.p;
+    A.task_pos = (const uint8_t *)P.task_pos.p;
+    A.ncolp = P.ncolmax;
+    A.ntiles = (int)P.ntiles;
+    A.geom = c->geom.p;
+    A.conn = (const int32_t *)F0.conn.p;
+    A.xy = (const double2 *)F0.xy.p;
+    A.vals = (double *)c->val.p;
+    A.F = heat ? (double *)c->F.p : nullptr;
+    A.fallback = mixed ? (int64_t *)c->dscal.p + 2 : nullptr;
+    if (mixed) HIPCHK(c, hipMemsetAsync(A.fallback, 0, 8, c->stream));
+    tm_begin(c, ELFEL_PH_NUMERIC, &ev);
+    rc = fn(c, A, cf, tab, P.lds_vals);
+    if (rc) return rc;
+    tm_end(c, ELFEL_PH_NUMERIC, ev);
+    c->nnz = P.nnz;
+    c->nmaj = c->nmin = c->nrow;
+    c->layout = pb->layout;
+    c->result_is_pattern = true;
+    c->has_F = heat;
+    c->nF = heat ? c->nrow : 0;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Geometry statistics: smallest det J over all elements and quadrature points
+// (FP64) and the number of elements the mixed kernels send to FP64.
+// ---------------------------------------------------------------------------
+template <int NEN, int NQ>
+__global__ __launch_bounds__(256) void k_geom_stats(int64_t E, const int32_t *__restrict__ conn,
+                                                    const double2 *__restrict__ xy, QTable tab,
+                                                    double *__restrict__ block_min,
+                                                    unsigned long long *__restrict__ nbad)
+{
+    __shared__ double smin[256];
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double mn = 1e300;
+    if (e < E) {
+        double X[NEN], Y[NEN];
+        float Xf[NEN], Yf[NEN];
+        double2 p0 = xy[conn[e * NEN]];
+#pragma unroll
+        for (int k = 0; k < NEN; k++) {
+            const double2 p = xy[conn[e * NEN + k]];
+            X[k] = p.x; Y[k] = p.y;
+            Xf[k] = (float)(p.x - p0.x); Yf[k] = (float)(p.y - p0.y);
+        }
+        Geo<double, NEN, NQ> g;
+        geometry<double, NEN, NQ>(tab, X, Y, g);
+        Geo<float, NEN, NQ> gf;
+        geometry<float, NEN, NQ>(tab, Xf, Yf, gf);
+        bool good = true;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            mn = fmin(mn, g.det[q]);
+            float j2 = 0.f;
+#pragma unroll
+            for (int k = 0; k < NEN; k++) j2 += gf.gx[q][k] * gf.gx[q][k] + gf.gy[q][k] * gf.gy[q][k];
+            good = good && (gf.det[q] > 0.f) && (j2 * gf.det[q] < REUSE_COND_MAX);
+        }
+        if (!good) atomicAdd(nbad, 1ull);
+    }
+    smin[threadIdx.x] = mn;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) block_min[blockIdx.x] = smin[0];
 }
 
 int geometry_stats_run(elfel_hip_ctx *c, double *min_det, int64_t *nfallback)
 {
-    (void)min_det; (void)nfallback;
-    return set_err(c, ELFEL_ENOTSUP, "geometry statistics not built yet");
+    const Field &F = c->f[0];
+    if (!F.has_mesh) return set_err(c, ELFEL_ESTATE, "no mesh");
+    if (!c->has_result && c->pat.fe < 0) return set_err(c, ELFEL_ESTATE, "assemble first");
+    const int fe = F.nen == 3 ? ELFEL_T3 : (F.nen == 6 ? ELFEL_T6 : ELFEL_Q4);
+    // Use the rule of the last pattern if any, else the element's common rule.
+    int rule = fe == ELFEL_T3 ? 1 : (fe == ELFEL_T6 ? 3 : 2);
+    QTable tab;
+    if (make_qtable(fe, rule, &tab)) return set_err(c, ELFEL_EINVAL, "no quadrature table");
+    const int64_t E = F.E;
+    const int nb = ceil_div(std::max<int64_t>(E, 1), 256);
+    int rc = ensure(c, c->scal, 8 * (size_t)(nb + 1));
+    if (rc) return rc;
+    double *bm = (double *)c->scal.p;
+    unsigned long long *cnt = (unsigned long long *)((int64_t *)c->dscal.p + 3);
+    HIPCHK(c, hipMemsetAsync(cnt, 0, 8, c->stream));
+#define GS(N, Q) hipLaunchKernelGGL((k_geom_stats<N, Q>), dim3(nb), dim3(256), 0, c->stream, E, \
+                                    (const int32_t *)F.conn.p, (const double2 *)F.xy.p, tab, bm, cnt)
+    if (fe == ELFEL_T3) GS(3, 1);
+    else if (fe == ELFEL_T6) GS(6, 3);
+    else GS(4, 4);
+#undef GS
+    HIPCHK(c, hipGetLastError());
+    std::vector<double> h(nb);
+    unsigned long long hc = 0;
+    HIPCHK(c, hipMemcpyAsync(h.data(), bm, 8 * nb, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&hc, cnt, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double m = 1e300;
+    for (double v : h) m = std::min(m, v);
+    if (min_det) *min_det = m;
+    if (nfallback) *nfallback = (int64_t)hc;
+    return 0;
 }

@@ -1,0 +1,18 @@
+// reuse64.hip -- FP64 instantiations of the reuse kernels.  Compiled with
+// -ffp-contract=off: every entry is computed operation by operation as the
+// reference computes it and summed in element order, so the FP64 reuse result
+// equals the plain method's bit for bit.
+#include "reuse_kernel.h"
+
+reuse_launch_fn pick_reuse64(int kind, int fe, int nq, int csc) { REUSE_PICK_BODY(double) }
+
+int gather_geom64(elfel_hip_ctx *c, int nen, int64_t E, const int32_t *conn, const double2 *xy, double *geom)
+{
+    if (E <= 0) return 0;
+    const dim3 g(ceil_div(E, 256)), b(256);
+    if (nen == 3) hipLaunchKernelGGL((k_gather_geom<double, 3>), g, b, 0, c->stream, E, conn, xy, geom);
+    else if (nen == 4) hipLaunchKernelGGL((k_gather_geom<double, 4>), g, b, 0, c->stream, E, conn, xy, geom);
+    else hipLaunchKernelGGL((k_gather_geom<double, 6>), g, b, 0, c->stream, E, conn, xy, geom);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}

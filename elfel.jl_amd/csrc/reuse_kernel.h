@@ -1,0 +1,331 @@
+// reuse_kernel.h -- fused owner-computes assembly with a cached pattern.
+//
+// Work unit ("task") = (owner row group, element): a row group is the run of
+// consecutive dofs of one node (its components), all of which share one
+// sorted column list.  The task computes the owner dofs' entries of that
+// element's block -- rows of ke for CSR, columns of ke for CSC (Elfel's
+// finish! returns CSC, src/Assemblers.jl:121-123) -- and adds them into an LDS
+// copy of the tile's CSR/CSC value range.  Tasks of a group are ordered by
+// element, and the adds run rank by rank with a barrier in between, so every
+// entry is summed in element order from 0.0: the order in which
+// SparseArrays.sparse combines the duplicates of the COO stream.  The tile's
+// values are then written to HBM once, coalesced; no atomics, no sort.
+//
+// Task descriptor (uint32 info):
+//   bits 0-2   local node a of the owner in the element
+//   bits 3-6   rank s of the task in its group (element order)
+//   bits 7-8   rows: 0 -> comp 0 only, 1 -> comp 1 only, 2 -> comps 0 and 1
+//   bit  9     owner is a pressure dof (Taylor-Hood)
+//   bits 10-23 LDS offset of the group's first row in the tile
+//   bits 24-31 row length L (two-row groups) or tile-local row (one-row groups)
+// task_pos[task][j]: position of element column j in the owner's sorted
+// column list (uint8).
+#pragma once
+#include "elem.h"
+
+#define REUSE_BLOCK 256
+#define REUSE_VCAP 6144      // LDS doubles for values of one tile (48 KiB)
+#define REUSE_COND_MAX 1024.0f // sum_k |grad N_k|^2 * det J above which the mixed kernel uses FP64
+
+struct ReuseArgs {
+    const int4 *tiles;        // task0, ntasks, v0, nvals
+    const int4 *tile_rows;    // row0, nrows, nranks, 0
+    const int32_t *task_elem;
+    const uint32_t *task_info;
+    const uint8_t *task_pos;  // [ntasks][ncolp]
+    int ncolp;
+    int ntiles;
+    const void *geom;         // per element [NEN][2] of R
+    const int32_t *conn;      // for the FP64 branch of the mixed kernel
+    const double2 *xy;
+    double *vals;
+    double *F;
+    int64_t *fallback;        // count of FP64-branch tasks (mixed)
+};
+
+// The owner's entries of one element block.  The owner node a and its
+// components are run-time values, so the owner's gradients are first pulled
+// out of the register arrays with unrolled selects (no dynamic register
+// indexing, hence no scratch); every entry is then formed with the same
+// operations, in the same order, as the reference loop that produces it.
+template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC>
+__device__ __forceinline__ void owner_values(const QTable &tab, const Geo<R, NEN, NQ> &g, const R *D, R kap, R mu,
+                                             R Qe, int a, int comp0, int nr, bool powner, R (*out)[NEN * NC + 3],
+                                             R *fo)
+{
+    constexpr int ND = NEN * NC;
+    constexpr bool STOKES = (KIND == ELFEL_STOKES_GEN || KIND == ELFEL_STOKES_VECLAP);
+    constexpr bool BDB = (KIND == ELFEL_ELAST || KIND == ELFEL_STOKES_GEN);
+    R ox[NQ], oy[NQ], on[NQ], onp[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        ox[q] = g.gx[q][0]; oy[q] = g.gy[q][0]; on[q] = tab_n<R>(tab, q, 0); onp[q] = tab_np<R>(tab, q, 0);
+#pragma unroll
+        for (int k = 1; k < NEN; k++) {
+            const bool m = (k == a);
+            ox[q] = m ? g.gx[q][k] : ox[q];
+            oy[q] = m ? g.gy[q][k] : oy[q];
+            on[q] = m ? tab_n<R>(tab, q, k) : on[q];
+            if (k < 3) onp[q] = m ? tab_np<R>(tab, q, k) : onp[q];
+        }
+    }
+    if constexpr (STOKES) {
+        if (powner) {
+            // kup[:, a]: row a of transpose(kup) (CSR) or column a of kup (CSC)
+#pragma unroll
+            for (int i = 0; i < ND; i++) {
+                R acc = R(0);
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    const R gi = (i % NC) == 0 ? g.gx[q][i / NC] : g.gy[q][i / NC];
+                    acc += (-g.jxw[q] * onp[q]) * gi;
+                }
+                out[0][i] = acc;
+            }
+            return;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+        if (c >= nr) continue;
+        const int oc = comp0 + c;  // owner component
+        if constexpr (KIND == ELFEL_HEAT) {
+#pragma unroll
+            for (int j = 0; j < ND; j++) {
+                R acc = R(0);
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    if constexpr (CSC) acc += (g.gx[q][j] * ox[q] + g.gy[q][j] * oy[q]) * (kap * g.jxw[q]);
+                    else acc += (ox[q] * g.gx[q][j] + oy[q] * g.gy[q][j]) * (kap * g.jxw[q]);
+                }
+                out[c][j] = acc;
+            }
+        } else if constexpr (BDB) {
+            if constexpr (CSC) {
+                R dbo[NQ][3];
+#pragma unroll
+                for (int q = 0; q < NQ; q++) dmul_b<R>(D, ox[q], oy[q], oc, dbo[q]);
+#pragma unroll
+                for (int i = 0; i < ND; i++) {
+                    R acc = R(0);
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) acc += dot_b<R>(dbo[q], g.gx[q][i / NC], g.gy[q][i / NC], i % NC) * g.jxw[q];
+                    out[c][i] = acc;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < ND; j++) {
+                    R acc = R(0);
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) {
+                        R db[3];
+                        dmul_b<R>(D, g.gx[q][j / NC], g.gy[q][j / NC], j % NC, db);
+                        acc += dot_b<R>(db, ox[q], oy[q], oc) * g.jxw[q];
+                    }
+                    out[c][j] = acc;
+                }
+            }
+        } else {  // vector Laplacian: only equal components couple
+#pragma unroll
+            for (int j = 0; j < ND; j++) {
+                R acc = R(0);
+                if ((j % NC) == oc) {
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) {
+                        if constexpr (CSC) acc += (mu * g.jxw[q]) * (g.gx[q][j / NC] * ox[q] + g.gy[q][j / NC] * oy[q]);
+                        else acc += (mu * g.jxw[q]) * (ox[q] * g.gx[q][j / NC] + oy[q] * g.gy[q][j / NC]);
+                    }
+                }
+                out[c][j] = acc;
+            }
+        }
+        if constexpr (STOKES) {
+#pragma unroll
+            for (int jp = 0; jp < 3; jp++) {
+                R acc = R(0);
+#pragma unroll
+                for (int q = 0; q < NQ; q++) acc += (-g.jxw[q] * tab_np<R>(tab, q, jp)) * (oc == 0 ? ox[q] : oy[q]);
+                out[c][ND + jp] = acc;
+            }
+        }
+    }
+    if constexpr (KIND == ELFEL_HEAT) {
+        R acc = R(0);
+#pragma unroll
+        for (int q = 0; q < NQ; q++) acc += on[q] * Qe * g.jxw[q];
+        *fo = acc;
+    }
+}
+
+template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC>
+__global__ __launch_bounds__(REUSE_BLOCK) void k_reuse(ReuseArgs A, Coef cf, QTable tab, int lds_vals)
+{
+    extern __shared__ double lds[];
+    constexpr int ND = NEN * NC;
+    constexpr bool STOKES = (KIND == ELFEL_STOKES_GEN || KIND == ELFEL_STOKES_VECLAP);
+    constexpr int NCOL = ND + (STOKES ? 3 : 0);
+    constexpr bool MIXED = sizeof(R) == 4;
+    double *acc = lds;
+    double *accF = lds + lds_vals;
+    const int tid = threadIdx.x;
+    const int4 td = A.tiles[blockIdx.x];
+    const int4 tr = A.tile_rows[blockIdx.x];
+    for (int v = tid; v < td.w; v += REUSE_BLOCK) acc[v] = 0.0;
+    if constexpr (KIND == ELFEL_HEAT)
+        for (int v = tid; v < tr.y; v += REUSE_BLOCK) accF[v] = 0.0;
+
+    const bool active = tid < td.y;
+    int rank = -1, rowcode = 0, vbase = 0, L8 = 0;
+    R out[2][ND + 3];
+    R fo = R(0);
+    int64_t task = 0;
+    if (active) {
+        task = (int64_t)td.x + tid;
+        const int32_t e = A.task_elem[task];
+        const uint32_t info = A.task_info[task];
+        const int a = info & 7;
+        rank = (info >> 3) & 15;
+        rowcode = (info >> 7) & 3;
+        const bool powner = STOKES && ((info >> 9) & 1);
+        vbase = (info >> 10) & 0x3FFF;
+        L8 = info >> 24;
+        const int comp0 = rowcode == 1 ? 1 : 0;
+        const int nr = rowcode == 2 ? 2 : 1;
+        const double sc = cf.scale_e ? cf.scale_e[e] : 1.0;
+        const double kapd = cf.kappa_e ? cf.kappa_e[e] : cf.kappa;
+        const double Qd = cf.Q_e ? cf.Q_e[e] : cf.Q;
+        const R *gp = (const R *)A.geom + (int64_t)e * (2 * NEN);
+        R X[NEN], Y[NEN];
+#pragma unroll
+        for (int k = 0; k < NEN; k++) { X[k] = gp[2 * k]; Y[k] = gp[2 * k + 1]; }
+        Geo<R, NEN, NQ> g;
+        geometry<R, NEN, NQ>(tab, X, Y, g);
+        if constexpr (MIXED) {
+            bool good = true;
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                // Shape-quality proxy: sum_k |grad N_k|^2 * det J is dimensionless, O(1-10)
+                // for well-shaped elements and grows like the condition number of J for
+                // distorted ones; a non-positive det also fails.
+                float j2 = 0.f;
+#pragma unroll
+                for (int k = 0; k < NEN; k++) j2 += g.gx[q][k] * g.gx[q][k] + g.gy[q][k] * g.gy[q][k];
+                good = good && (g.det[q] > 0.f) && (j2 * g.det[q] < REUSE_COND_MAX);
+            }
+            if (!good) {
+                // FP64 Jacobian, determinant and inverse from the absolute coordinates;
+                // the element products then proceed in FP32 as for every element.
+                if (A.fallback) atomicAdd((unsigned long long *)A.fallback, 1ull);
+                double Xd[NEN], Yd[NEN];
+#pragma unroll
+                for (int k = 0; k < NEN; k++) {
+                    const double2 p = A.xy[A.conn[(int64_t)e * NEN + k]];
+                    Xd[k] = p.x; Yd[k] = p.y;
+                }
+                Geo<double, NEN, NQ> gd;
+                geometry<double, NEN, NQ>(tab, Xd, Yd, gd);
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    g.jxw[q] = (float)gd.jxw[q];
+                    g.det[q] = (float)gd.det[q];
+#pragma unroll
+                    for (int k = 0; k < NEN; k++) { g.gx[q][k] = (float)gd.gx[q][k]; g.gy[q][k] = (float)gd.gy[q][k]; }
+                }
+            }
+        }
+        R Dr[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) Dr[i] = (R)(cf.scale_e ? sc * cf.D[i] : cf.D[i]);
+        owner_values<KIND, R, NEN, NC, NQ, CSC>(tab, g, Dr, (R)kapd, (R)cf.mu, (R)Qd, a, comp0, nr, powner, out, &fo);
+    }
+    __syncthreads();
+    const int nranks = tr.z;
+    for (int s = 0; s < nranks; s++) {
+        if (rank == s) {
+            const uint8_t *pp = A.task_pos + task * A.ncolp;
+            uint8_t pos[NCOL];
+#pragma unroll
+            for (int j = 0; j < NCOL; j++) pos[j] = pp[j];
+            const bool powner = STOKES && ((A.task_info[task] >> 9) & 1);
+            const int ncols = (STOKES && powner) ? ND : NCOL;
+            const int nr = rowcode == 2 ? 2 : 1;
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                if (c < nr) {
+                    double *row = acc + vbase + c * L8;
+#pragma unroll
+                    for (int j = 0; j < NCOL; j++)
+                        if (j < ncols) row[pos[j]] += (double)out[c][j];
+                }
+            }
+            if constexpr (KIND == ELFEL_HEAT) accF[L8] += (double)fo;
+        }
+        __syncthreads();
+    }
+    double *dst = A.vals + td.z;
+    for (int v = tid; v < td.w; v += REUSE_BLOCK) dst[v] = acc[v];
+    if constexpr (KIND == ELFEL_HEAT)
+        for (int v = tid; v < tr.y; v += REUSE_BLOCK) A.F[tr.x + v] = accF[v];
+}
+
+// Geometry gather for the reuse kernels: FP64 absolute coordinates, or FP32
+// coordinates relative to the element's first node (differences in FP64, so
+// the FP32 Jacobian does not lose the digits the absolute position holds).
+template <typename R, int NEN>
+__global__ __launch_bounds__(256) void k_gather_geom(int64_t E, const int32_t *__restrict__ conn,
+                                                     const double2 *__restrict__ xy, R *__restrict__ geom)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double2 p[NEN];
+#pragma unroll
+    for (int k = 0; k < NEN; k++) p[k] = xy[conn[e * NEN + k]];
+    R *g = geom + e * (2 * NEN);
+#pragma unroll
+    for (int k = 0; k < NEN; k++) {
+        if constexpr (sizeof(R) == 8) {
+            g[2 * k] = p[k].x;
+            g[2 * k + 1] = p[k].y;
+        } else {
+            g[2 * k] = (R)(p[k].x - p[0].x);
+            g[2 * k + 1] = (R)(p[k].y - p[0].y);
+        }
+    }
+}
+
+typedef int (*reuse_launch_fn)(elfel_hip_ctx *, const ReuseArgs &, const Coef &, const QTable &, int);
+reuse_launch_fn pick_reuse64(int kind, int fe, int nq, int csc);
+reuse_launch_fn pick_reuse32(int kind, int fe, int nq, int csc);
+int gather_geom64(elfel_hip_ctx *c, int nen, int64_t E, const int32_t *conn, const double2 *xy, double *geom);
+int gather_geom32(elfel_hip_ctx *c, int nen, int64_t E, const int32_t *conn, const double2 *xy, float *geom);
+
+// Instantiation helpers for reuse64.hip / reuse32.hip.
+template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC>
+int launch_reuse(elfel_hip_ctx *c, const ReuseArgs &A, const Coef &cf, const QTable &tab, int lds_vals)
+{
+    if (A.ntiles <= 0) return 0;
+    const size_t lds = sizeof(double) * ((size_t)lds_vals + (KIND == ELFEL_HEAT ? REUSE_BLOCK : 0));
+    hipLaunchKernelGGL((k_reuse<KIND, R, NEN, NC, NQ, CSC>), dim3(A.ntiles), dim3(REUSE_BLOCK), lds, c->stream, A, cf,
+                       tab, lds_vals);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+#define REUSE_PICK_BODY(R)                                                                           \
+    if (kind == ELFEL_HEAT) {                                                                        \
+        if (fe == ELFEL_T3 && nq == 1) return csc ? launch_reuse<ELFEL_HEAT, R, 3, 1, 1, true> : launch_reuse<ELFEL_HEAT, R, 3, 1, 1, false>; \
+        if (fe == ELFEL_T3 && nq == 3) return csc ? launch_reuse<ELFEL_HEAT, R, 3, 1, 3, true> : launch_reuse<ELFEL_HEAT, R, 3, 1, 3, false>; \
+        if (fe == ELFEL_T6 && nq == 3) return csc ? launch_reuse<ELFEL_HEAT, R, 6, 1, 3, true> : launch_reuse<ELFEL_HEAT, R, 6, 1, 3, false>; \
+        if (fe == ELFEL_Q4 && nq == 1) return csc ? launch_reuse<ELFEL_HEAT, R, 4, 1, 1, true> : launch_reuse<ELFEL_HEAT, R, 4, 1, 1, false>; \
+        if (fe == ELFEL_Q4 && nq == 4) return csc ? launch_reuse<ELFEL_HEAT, R, 4, 1, 4, true> : launch_reuse<ELFEL_HEAT, R, 4, 1, 4, false>; \
+    } else if (kind == ELFEL_ELAST) {                                                                \
+        if (fe == ELFEL_T3 && nq == 1) return csc ? launch_reuse<ELFEL_ELAST, R, 3, 2, 1, true> : launch_reuse<ELFEL_ELAST, R, 3, 2, 1, false>; \
+        if (fe == ELFEL_T3 && nq == 3) return csc ? launch_reuse<ELFEL_ELAST, R, 3, 2, 3, true> : launch_reuse<ELFEL_ELAST, R, 3, 2, 3, false>; \
+        if (fe == ELFEL_T6 && nq == 3) return csc ? launch_reuse<ELFEL_ELAST, R, 6, 2, 3, true> : launch_reuse<ELFEL_ELAST, R, 6, 2, 3, false>; \
+        if (fe == ELFEL_Q4 && nq == 4) return csc ? launch_reuse<ELFEL_ELAST, R, 4, 2, 4, true> : launch_reuse<ELFEL_ELAST, R, 4, 2, 4, false>; \
+    } else if (kind == ELFEL_STOKES_GEN) {                                                           \
+        if (fe == ELFEL_T6 && nq == 3) return csc ? launch_reuse<ELFEL_STOKES_GEN, R, 6, 2, 3, true> : launch_reuse<ELFEL_STOKES_GEN, R, 6, 2, 3, false>; \
+    } else if (kind == ELFEL_STOKES_VECLAP) {                                                        \
+        if (fe == ELFEL_T6 && nq == 3) return csc ? launch_reuse<ELFEL_STOKES_VECLAP, R, 6, 2, 3, true> : launch_reuse<ELFEL_STOKES_VECLAP, R, 6, 2, 3, false>; \
+    }                                                                                                \
+    return nullptr;

@@ -40,7 +40,8 @@ class _Chunks:
         if self.scalars:
             self.parts.append(np.asarray(self.scalars, self.dtype))
             self.scalars = []
-        self.parts.append(np.asarray(a, self.dtype).ravel())
+        # copy: the caller re-initialises its local assembler for the next element
+        self.parts.append(np.array(a, dtype=self.dtype, copy=True).ravel())
 
     def add1(self, v):
         self.scalars.append(v)
