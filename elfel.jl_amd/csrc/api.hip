@@ -163,6 +163,7 @@ extern "C" int elfel_hip_free(elfel_hip_ctx *c)
                    &c->pat.ptr, &c->pat.ind, &c->pat.tile, &c->pat.tile_rows, &c->pat.task_elem,
                    &c->pat.task_info, &c->pat.task_pos};
     for (Buf *b : bufs) release(*b);
+    batch_free(c);
     for (auto &p : c->tm.pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
     for (auto e : c->tm.pool) hipEventDestroy(e);
     hipStreamDestroy(c->stream);

@@ -1,0 +1,480 @@
+// batch.hip -- many independent problems of uneven size assembled by one
+// persistent fused launch (heat-T3 1-point and elasticity-T6 3-point, mixed
+// precision), replayed from a hipGraph.
+//
+// build: per problem, the pattern comes from the plain method and the task
+// tables from the reuse tiler; the tables of all problems of one kind are
+// concatenated (element, node, value and row offsets folded in), so a tile
+// needs no problem id.  The launch shape -- tasks per workgroup (block of 128
+// or 256 threads, the tile capacity) and the persistent grid size -- is chosen
+// by timing every variant on the batch itself (autotune) and cached on disk,
+// keyed by device and batch signature.
+// assemble: one graph launch = gather T3 + gather T6 + k_reuse_batch.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <string>
+
+#include "reuse_kernel.h"
+
+struct BatchProb {
+    int kind = 0;
+    int64_t E = 0, Nn = 0, ndofs = 0, nnz = 0;
+    int64_t elem_base = 0, node_base = 0, val_base = 0, ptr_base = 0, row_base = 0;
+    std::vector<int64_t> conn, dofs;
+    std::vector<double> xy, coef;
+    bool has_coef = false;
+};
+
+struct KindData {
+    int64_t E = 0, Nn = 0, ntasks = 0, ntiles = 0;
+    Buf conn, xy, coef, geom;
+    Pattern tab[2];  // tables for block 128 and 256
+};
+
+struct Batch {
+    std::vector<BatchProb> probs;
+    elfel_hip_problem pb_heat{}, pb_el{};
+    bool have_heat = false, have_el = false, built = false;
+    int layout = ELFEL_CSC;
+    KindData kd[2];  // 0 heat T3, 1 elasticity T6
+    Buf ptr, ind, val, F;
+    int64_t total_nnz = 0, total_ptr = 0, total_rows = 0;
+    int block = 256, grid = 0, lds_vals = 0;
+    double tuned_ms = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static Batch *get_batch(elfel_hip_ctx *c)
+{
+    if (!c->batch) c->batch = new Batch();
+    return (Batch *)c->batch;
+}
+
+static void batch_destroy_graph(Batch *b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+void batch_free(elfel_hip_ctx *c)
+{
+    Batch *b = (Batch *)c->batch;
+    if (!b) return;
+    batch_destroy_graph(b);
+    for (auto &k : b->kd) {
+        release(k.conn); release(k.xy); release(k.coef); release(k.geom);
+        for (auto &t : k.tab) {
+            release(t.tile); release(t.tile_rows); release(t.task_elem); release(t.task_info); release(t.task_pos);
+        }
+    }
+    release(b->ptr); release(b->ind); release(b->val); release(b->F);
+    delete b;
+    c->batch = nullptr;
+}
+
+extern "C" int elfel_hip_batch_begin(elfel_hip_ctx *c)
+{
+    if (!c) return ELFEL_EINVAL;
+    c->err.clear();
+    (void)hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    batch_free(c);
+    get_batch(c);
+    return ELFEL_OK;
+}
+
+extern "C" int elfel_hip_batch_add(elfel_hip_ctx *c, const elfel_hip_problem *pb, int32_t nen, int64_t E,
+                                   const int64_t *conn, int64_t Nn, const double *xy, int32_t ncomp,
+                                   const int64_t *dofnums, int64_t ndofs, const double *coef_e)
+{
+    if (!c) return ELFEL_EINVAL;
+    c->err.clear();
+    Batch *b = get_batch(c);
+    if (b->built) return set_err(c, ELFEL_ESTATE, "batch already built; call elfel_hip_batch_begin");
+    if (!pb || E <= 0 || Nn <= 0 || !conn || !xy || !dofnums || ndofs <= 0)
+        return set_err(c, ELFEL_EINVAL, "elfel_hip_batch_add: bad arguments");
+    const bool heat = pb->kind == ELFEL_HEAT && pb->fe == ELFEL_T3 && pb->rule == 1 && nen == 3 && ncomp == 1;
+    const bool el = pb->kind == ELFEL_ELAST && pb->fe == ELFEL_T6 && pb->rule == 3 && nen == 6 && ncomp == 2;
+    if (!heat && !el) return set_err(c, ELFEL_ENOTSUP, "batch problems: heat T3 1-point or elasticity T6 3-point");
+    if (pb->precision != ELFEL_MIXED) return set_err(c, ELFEL_ENOTSUP, "the batched launch is the mixed method");
+    elfel_hip_problem &ref = heat ? b->pb_heat : b->pb_el;
+    bool &have = heat ? b->have_heat : b->have_el;
+    if (!have) { ref = *pb; have = true; }
+    else if (memcmp(&ref, pb, sizeof(*pb)) != 0)
+        return set_err(c, ELFEL_ENOTSUP, "problems of one kind must share their uniform coefficients");
+    if (!b->probs.empty() && pb->layout != b->layout) return set_err(c, ELFEL_EINVAL, "mixed layouts in one batch");
+    b->layout = pb->layout;
+    for (int64_t i = 0; i < E * nen; i++)
+        if (conn[i] < 1 || conn[i] > Nn) return set_err(c, ELFEL_ERANGE, "connectivity outside 1..Nn");
+    for (int64_t i = 0; i < Nn * ncomp; i++)
+        if (dofnums[i] < 1 || dofnums[i] > ndofs) return set_err(c, ELFEL_ERANGE, "dof number outside 1..ndofs");
+    BatchProb p;
+    p.kind = heat ? 0 : 1;
+    p.E = E; p.Nn = Nn; p.ndofs = ndofs;
+    p.conn.assign(conn, conn + E * nen);
+    p.dofs.assign(dofnums, dofnums + Nn * ncomp);
+    p.xy.assign(xy, xy + 2 * Nn);
+    if (coef_e) { p.coef.assign(coef_e, coef_e + E); p.has_coef = true; }
+    b->probs.push_back(std::move(p));
+    return ELFEL_OK;
+}
+
+static std::string tune_cache_path()
+{
+    const char *e = getenv("ELFEL_HIP_TUNE_CACHE");
+    if (e && *e) return e;
+    const char *h = getenv("HOME");
+    return std::string(h ? h : "/tmp") + "/.elfel_hip_tune.txt";
+}
+
+static bool tune_lookup(const std::string &key, int *block, int *gmult)
+{
+    FILE *f = fopen(tune_cache_path().c_str(), "r");
+    if (!f) return false;
+    char k[512];
+    int bl, gm;
+    bool found = false;
+    while (fscanf(f, "%511s %d %d", k, &bl, &gm) == 3)
+        if (key == k) { *block = bl; *gmult = gm; found = true; }
+    fclose(f);
+    return found;
+}
+
+static void tune_store(const std::string &key, int block, int gmult)
+{
+    FILE *f = fopen(tune_cache_path().c_str(), "a");
+    if (!f) return;
+    fprintf(f, "%s %d %d\n", key.c_str(), block, gmult);
+    fclose(f);
+}
+
+static ReuseArgs kind_args(KindData &k, int bi, double *vals, double *F)
+{
+    Pattern &P = k.tab[bi];
+    ReuseArgs A;
+    A.tiles = (const int4 *)P.tile.p;
+    A.tile_rows = (const int4 *)P.tile_rows.p;
+    A.task_elem = (const int32_t *)P.task_elem.p;
+    A.task_info = (const uint32_t *)P.task_info.p;
+    A.task_pos = (const uint8_t *)P.task_pos.p;
+    A.ncolp = P.ncolmax;
+    A.ntiles = (int)P.ntiles;
+    A.geom = k.geom.p;
+    A.conn = (const int32_t *)k.conn.p;
+    A.xy = (const double2 *)k.xy.p;
+    A.vals = vals;
+    A.F = F;
+    A.fallback = nullptr;
+    return A;
+}
+
+// One batched assembly on the stream (no synchronisation).
+static int batch_launch(elfel_hip_ctx *c, Batch *b, int block, int grid)
+{
+    const int bi = block == 128 ? 0 : 1;
+    KindData &H = b->kd[0], &Ev = b->kd[1];
+    if (H.E) {
+        int rc = gather_geom32(c, 3, H.E, (const int32_t *)H.conn.p, (const double2 *)H.xy.p, (float *)H.geom.p);
+        if (rc) return rc;
+    }
+    if (Ev.E) {
+        int rc = gather_geom32(c, 6, Ev.E, (const int32_t *)Ev.conn.p, (const double2 *)Ev.xy.p, (float *)Ev.geom.p);
+        if (rc) return rc;
+    }
+    QTable th{}, te{};
+    make_qtable(ELFEL_T3, 1, &th);
+    make_qtable(ELFEL_T6, 3, &te);
+    Coef ch{}, ce{};
+    ch.kappa = b->pb_heat.kappa; ch.Q = b->pb_heat.Q; ch.mu = b->pb_heat.mu;
+    ch.kappa_e = H.coef.p ? (const double *)H.coef.p : nullptr;
+    ce.mu = b->pb_el.mu;
+    for (int i = 0; i < 9; i++) { ch.D[i] = b->pb_heat.D[i]; ce.D[i] = b->pb_el.D[i]; }
+    ce.scale_e = Ev.coef.p ? (const double *)Ev.coef.p : nullptr;
+    ReuseArgs AH = kind_args(H, bi, (double *)b->val.p, (double *)b->F.p);
+    ReuseArgs AE = kind_args(Ev, bi, (double *)b->val.p, nullptr);
+    if (!H.E) AH.ntiles = 0;
+    if (!Ev.E) AE.ntiles = 0;
+    const int nt = AH.ntiles + AE.ntiles;
+    if (nt == 0) return 0;
+    const int lds_vals = std::max(H.E ? (int)H.tab[bi].lds_vals : 0, Ev.E ? (int)Ev.tab[bi].lds_vals : 0);
+    const size_t lds = sizeof(double) * ((size_t)lds_vals + block);
+    const int g = grid > 0 ? std::min(grid, nt) : nt;
+    const bool csc = b->layout == ELFEL_CSC;
+    if (block == 128) {
+        if (csc) hipLaunchKernelGGL((k_reuse_batch<true, 128>), dim3(g), dim3(128), lds, c->stream, AH, AE, ch, ce, th, te, lds_vals);
+        else hipLaunchKernelGGL((k_reuse_batch<false, 128>), dim3(g), dim3(128), lds, c->stream, AH, AE, ch, ce, th, te, lds_vals);
+    } else {
+        if (csc) hipLaunchKernelGGL((k_reuse_batch<true, 256>), dim3(g), dim3(256), lds, c->stream, AH, AE, ch, ce, th, te, lds_vals);
+        else hipLaunchKernelGGL((k_reuse_batch<false, 256>), dim3(g), dim3(256), lds, c->stream, AH, AE, ch, ce, th, te, lds_vals);
+    }
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+static int time_variant(elfel_hip_ctx *c, Batch *b, int block, int grid, double *ms)
+{
+    int rc = batch_launch(c, b, block, grid);  // warm-up
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    const int reps = 5;
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps && !rc; r++) rc = batch_launch(c, b, block, grid);
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float t = 0;
+    hipEventElapsedTime(&t, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    *ms = t / reps;
+    return rc;
+}
+
+extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
+{
+    if (!c) return ELFEL_EINVAL;
+    c->err.clear();
+    (void)hipSetDevice(c->device);
+    Batch *b = get_batch(c);
+    if (b->probs.empty()) return set_err(c, ELFEL_ESTATE, "empty batch");
+    batch_destroy_graph(b);
+    // pass 1: patterns and tables per problem
+    std::vector<std::vector<int32_t>> hptrs(b->probs.size()), hinds(b->probs.size());
+    std::vector<ReuseTables> tabs[2];
+    tabs[0].resize(b->probs.size());
+    tabs[1].resize(b->probs.size());
+    int64_t koff_E[2] = {0, 0}, koff_N[2] = {0, 0};
+    int64_t val_base = 0, ptr_base = 0, row_base = 0;
+    for (size_t k = 0; k < b->probs.size(); k++) {
+        BatchProb &p = b->probs[k];
+        const elfel_hip_problem &pb = p.kind == 0 ? b->pb_heat : b->pb_el;
+        const int nen = p.kind == 0 ? 3 : 6, nc = p.kind == 0 ? 1 : 2;
+        int rc = elfel_hip_set_mesh(c, 0, nen, p.E, p.conn.data(), p.Nn, p.xy.data());
+        if (!rc) rc = elfel_hip_set_dofs(c, 0, nc, p.Nn, p.dofs.data(), p.ndofs);
+        if (rc) return rc;
+        c->nrow = c->ncol = p.ndofs;
+        QTable tab;
+        make_qtable(pb.fe, pb.rule, &tab);
+        Coef cf{};
+        for (int i = 0; i < 9; i++) cf.D[i] = pb.D[i];
+        if ((rc = pattern_to_host(c, &pb, tab, cf, hptrs[k], hinds[k]))) return rc;
+        for (int bi = 0; bi < 2; bi++)
+            if ((rc = tile_tables(c, false, c->f[0], nullptr, p.ndofs, hptrs[k], hinds[k], bi ? 256 : 128, tabs[bi][k])))
+                return rc;
+        p.nnz = (int64_t)hinds[k].size();
+        p.elem_base = koff_E[p.kind];
+        p.node_base = koff_N[p.kind];
+        koff_E[p.kind] += p.E;
+        koff_N[p.kind] += p.Nn;
+        p.val_base = val_base;
+        p.ptr_base = ptr_base;
+        p.row_base = row_base;
+        val_base += p.nnz;
+        ptr_base += p.ndofs + 1;
+        if (p.kind == 0) row_base += p.ndofs;
+    }
+    b->total_nnz = val_base;
+    b->total_ptr = ptr_base;
+    b->total_rows = row_base;
+    // pass 2: concatenate per kind and upload
+    int rc;
+    if ((rc = ensure(c, b->ptr, 4 * ptr_base)) || (rc = ensure(c, b->ind, 4 * std::max<int64_t>(val_base, 1))) ||
+        (rc = ensure(c, b->val, 8 * std::max<int64_t>(val_base, 1))) ||
+        (rc = ensure(c, b->F, 8 * std::max<int64_t>(row_base, 1))))
+        return rc;
+    for (size_t k = 0; k < b->probs.size(); k++) {
+        const BatchProb &p = b->probs[k];
+        HIPCHK(c, hipMemcpy((int32_t *)b->ptr.p + p.ptr_base, hptrs[k].data(), 4 * (p.ndofs + 1), hipMemcpyHostToDevice));
+        if (p.nnz)
+            HIPCHK(c, hipMemcpy((int32_t *)b->ind.p + p.val_base, hinds[k].data(), 4 * p.nnz, hipMemcpyHostToDevice));
+    }
+    for (int kind = 0; kind < 2; kind++) {
+        KindData &K = b->kd[kind];
+        K.E = koff_E[kind];
+        K.Nn = koff_N[kind];
+        if (!K.E) continue;
+        const int nen = kind == 0 ? 3 : 6;
+        std::vector<int32_t> conn(K.E * nen);
+        std::vector<double> xy(2 * K.Nn), coef;
+        bool any_coef = false;
+        for (const BatchProb &p : b->probs) if (p.kind == kind && p.has_coef) any_coef = true;
+        if (any_coef) coef.assign(K.E, 1.0);
+        for (const BatchProb &p : b->probs) {
+            if (p.kind != kind) continue;
+            for (int64_t i = 0; i < p.E * nen; i++) conn[p.elem_base * nen + i] = (int32_t)(p.conn[i] - 1 + p.node_base);
+            std::copy(p.xy.begin(), p.xy.end(), xy.begin() + 2 * p.node_base);
+            if (any_coef) {
+                const double u = kind == 0 ? b->pb_heat.kappa : 1.0;
+                for (int64_t e = 0; e < p.E; e++) coef[p.elem_base + e] = p.has_coef ? p.coef[e] : u;
+            }
+        }
+        if ((rc = ensure(c, K.conn, 4 * conn.size())) || (rc = ensure(c, K.xy, 8 * xy.size())) ||
+            (rc = ensure(c, K.geom, 4 * (size_t)K.E * nen * 2)))
+            return rc;
+        HIPCHK(c, hipMemcpy(K.conn.p, conn.data(), 4 * conn.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(K.xy.p, xy.data(), 8 * xy.size(), hipMemcpyHostToDevice));
+        if (any_coef) {
+            if ((rc = ensure(c, K.coef, 8 * coef.size()))) return rc;
+            HIPCHK(c, hipMemcpy(K.coef.p, coef.data(), 8 * coef.size(), hipMemcpyHostToDevice));
+        } else {
+            release(K.coef);
+        }
+        for (int bi = 0; bi < 2; bi++) {
+            ReuseTables cat;
+            for (size_t k = 0; k < b->probs.size(); k++) {
+                const BatchProb &p = b->probs[k];
+                if (p.kind != kind) continue;
+                const ReuseTables &t = tabs[bi][k];
+                const int task0 = (int)cat.elem.size();
+                cat.ncolp = t.ncolp;
+                cat.maxv = std::max(cat.maxv, t.maxv);
+                cat.maxrank = std::max(cat.maxrank, t.maxrank);
+                for (size_t i = 0; i < t.tiles.size(); i++) {
+                    int4 td = t.tiles[i], tr = t.trows[i];
+                    td.x += task0;
+                    td.z += (int)p.val_base;
+                    tr.x += (int)p.row_base;
+                    cat.tiles.push_back(td);
+                    cat.trows.push_back(tr);
+                }
+                for (int32_t e : t.elem) cat.elem.push_back(e + (int32_t)p.elem_base);
+                cat.info.insert(cat.info.end(), t.info.begin(), t.info.end());
+                cat.pos.insert(cat.pos.end(), t.pos.begin(), t.pos.end());
+            }
+            if ((rc = upload_tables(c, cat, K.tab[bi]))) return rc;
+        }
+        K.ntiles = K.tab[1].ntiles;
+        K.ntasks = K.tab[1].ntasks;
+    }
+    if (b->total_nnz >= INT32_MAX || b->total_ptr >= INT32_MAX)
+        return set_err(c, ELFEL_ENOTSUP, "batch beyond int32 value offsets");
+    b->built = true;
+    // launch shape: cache or autotune
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    const int ncu = prop.multiProcessorCount;
+    char key[512];
+    snprintf(key, sizeof key, "%s:batch:heatT3r1x%d+elastT6r3x%d:%s:tiles%lld", prop.gcnArchName,
+             b->have_heat ? 1 : 0, b->have_el ? 1 : 0, b->layout == ELFEL_CSC ? "csc" : "csr",
+             (long long)((b->kd[0].ntiles + b->kd[1].ntiles) / 1024));
+    int block = 256, gmult = 0;
+    bool cached = tune_lookup(key, &block, &gmult);
+    if (autotune || !cached) {
+        const int blocks[2] = {128, 256};
+        const int gms[5] = {2, 4, 8, 16, 0};
+        double best = 1e30;
+        for (int bl : blocks)
+            for (int gm : gms) {
+                double ms = 0;
+                if ((rc = time_variant(c, b, bl, gm ? gm * ncu : 0, &ms))) return rc;
+                if (ms < best) { best = ms; block = bl; gmult = gm; }
+            }
+        b->tuned_ms = best;
+        tune_store(key, block, gmult);
+    }
+    b->block = block;
+    b->grid = gmult ? gmult * ncu : 0;
+    // capture one assembly in a graph
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = batch_launch(c, b, b->block, b->grid);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) { if (g) hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return set_err(c, ELFEL_EHIP, "hipStreamEndCapture failed");
+    b->graph = g;
+    HIPCHK(c, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    return ELFEL_OK;
+}
+
+extern "C" int elfel_hip_batch_assemble(elfel_hip_ctx *c, int wait)
+{
+    if (!c) return ELFEL_EINVAL;
+    c->err.clear();
+    (void)hipSetDevice(c->device);
+    Batch *b = (Batch *)c->batch;
+    if (!b || !b->built || !b->exec) return set_err(c, ELFEL_ESTATE, "build the batch first");
+    hipEvent_t ev = nullptr;
+    tm_begin(c, ELFEL_PH_TOTAL, &ev);
+    HIPCHK(c, hipGraphLaunch(b->exec, c->stream));
+    tm_end(c, ELFEL_PH_TOTAL, ev);
+    if (wait) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ELFEL_OK;
+}
+
+extern "C" int elfel_hip_batch_info(elfel_hip_ctx *c, int32_t k, int64_t *ndofs, int64_t *nnz, int64_t *nelem,
+                                    int *block, int *grid)
+{
+    if (!c) return ELFEL_EINVAL;
+    c->err.clear();
+    Batch *b = (Batch *)c->batch;
+    if (!b || !b->built) return set_err(c, ELFEL_ESTATE, "build the batch first");
+    if (k < 0 || k >= (int32_t)b->probs.size()) return set_err(c, ELFEL_EINVAL, "problem index out of range");
+    if (ndofs) *ndofs = b->probs[k].ndofs;
+    if (nnz) *nnz = b->probs[k].nnz;
+    if (nelem) *nelem = b->probs[k].E;
+    if (block) *block = b->block;
+    if (grid) *grid = b->grid;
+    return ELFEL_OK;
+}
+
+__global__ void k_ptr_i64(const int32_t *__restrict__ s, int64_t *__restrict__ d, int64_t n, int64_t base)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (int64_t)s[i] + base;
+}
+
+extern "C" int elfel_hip_batch_copy(elfel_hip_ctx *c, int32_t k, int base, int64_t *ptr, int64_t *ind, double *val,
+                                    double *F)
+{
+    if (!c) return ELFEL_EINVAL;
+    c->err.clear();
+    (void)hipSetDevice(c->device);
+    Batch *b = (Batch *)c->batch;
+    if (!b || !b->built) return set_err(c, ELFEL_ESTATE, "build the batch first");
+    if (k < 0 || k >= (int32_t)b->probs.size()) return set_err(c, ELFEL_EINVAL, "problem index out of range");
+    const BatchProb &p = b->probs[k];
+    int rc = ensure(c, c->stage64, 8 * std::max<int64_t>(p.nnz, p.ndofs + 1));
+    if (rc) return rc;
+    int64_t *st = (int64_t *)c->stage64.p;
+    if (ptr) {
+        hipLaunchKernelGGL(k_ptr_i64, dim3(ceil_div(p.ndofs + 1, 256)), dim3(256), 0, c->stream,
+                           (const int32_t *)b->ptr.p + p.ptr_base, st, p.ndofs + 1, (int64_t)base);
+        HIPCHK(c, hipMemcpyAsync(ptr, st, 8 * (p.ndofs + 1), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (ind && p.nnz) {
+        hipLaunchKernelGGL(k_ptr_i64, dim3(ceil_div(p.nnz, 256)), dim3(256), 0, c->stream,
+                           (const int32_t *)b->ind.p + p.val_base, st, p.nnz, (int64_t)base);
+        HIPCHK(c, hipMemcpyAsync(ind, st, 8 * p.nnz, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (val && p.nnz)
+        HIPCHK(c, hipMemcpyAsync(val, (double *)b->val.p + p.val_base, 8 * p.nnz, hipMemcpyDeviceToHost, c->stream));
+    if (F && p.kind == 0)
+        HIPCHK(c, hipMemcpyAsync(F, (double *)b->F.p + p.row_base, 8 * p.ndofs, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ELFEL_OK;
+}
+
+extern "C" int elfel_hip_batch_stats(elfel_hip_ctx *c, int64_t *nprob, int64_t *nelem, int64_t *nnz,
+                                     int64_t *ntiles, double *tuned_ms)
+{
+    if (!c) return ELFEL_EINVAL;
+    Batch *b = (Batch *)c->batch;
+    if (!b || !b->built) return set_err(c, ELFEL_ESTATE, "build the batch first");
+    if (nprob) *nprob = (int64_t)b->probs.size();
+    if (nelem) *nelem = b->kd[0].E + b->kd[1].E;
+    if (nnz) *nnz = b->total_nnz;
+    if (ntiles) *ntiles = b->kd[0].ntiles + b->kd[1].ntiles;
+    if (tuned_ms) *tuned_ms = b->tuned_ms;
+    return ELFEL_OK;
+}

@@ -114,6 +114,8 @@ struct elfel_hip_ctx {
     Timing tm;
     // solver scratch
     Buf pcg;
+    // batched problems (batch.hip)
+    void *batch = nullptr;
 };
 
 // ---- error handling -------------------------------------------------------
@@ -160,6 +162,9 @@ int convert_indices(elfel_hip_ctx *c, const int64_t *src_dev, int32_t *dst, int6
 
 // ---- reuse.hip ----------------------------------------------------------------
 int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf);
+
+// ---- batch.hip ----
+void batch_free(elfel_hip_ctx *c);
 
 // ---- solve.hip ----------------------------------------------------------------
 int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int precision, double tol,

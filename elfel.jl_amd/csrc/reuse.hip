@@ -47,20 +47,11 @@ void adjacency(const Field &f, Adj &A)
 
 }  // namespace
 
-static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf)
+// Pattern of the problem described by the context's fields (plain method, FP64
+// values unused), copied to the host; the device copy stays in c->ptr/c->ind.
+int pattern_to_host(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf,
+                    std::vector<int32_t> &hptr, std::vector<int32_t> &hind)
 {
-    auto t0 = std::chrono::steady_clock::now();
-    Pattern &P = c->pat;
-    P.valid = false;
-    const bool stokes = pb->kind == ELFEL_STOKES_GEN || pb->kind == ELFEL_STOKES_VECLAP;
-    const Field &F0 = c->f[0];
-    const Field *F1 = stokes ? &c->f[1] : nullptr;
-    const int nen = F0.nen, nc = F0.ncomp, nd = nen * nc;
-    const int ncol = nd + (stokes ? 3 : 0);
-    const int ncolp = (ncol + 3) & ~3;
-    const int64_t n = c->nrow;
-
-    // 1. pattern from the plain method
     elfel_hip_problem p2 = *pb;
     p2.method = ELFEL_SORT;
     p2.precision = ELFEL_FP64;
@@ -69,20 +60,25 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
     int32_t hnnz = 0;
     HIPCHK(c, hipMemcpyAsync(&hnnz, c->dscal.p, 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const int64_t nnz = hnnz;
-    std::vector<int32_t> hptr(n + 1), hind(nnz);
+    const int64_t n = c->nrow;
+    hptr.resize(n + 1);
+    hind.resize(hnnz);
     HIPCHK(c, hipMemcpy(hptr.data(), c->ptr.p, 4 * (n + 1), hipMemcpyDeviceToHost));
-    if (nnz) HIPCHK(c, hipMemcpy(hind.data(), c->ind.p, 4 * nnz, hipMemcpyDeviceToHost));
-    if ((rc = ensure(c, P.ptr, 4 * (n + 1))) || (rc = ensure(c, P.ind, 4 * std::max<int64_t>(nnz, 1)))) return rc;
-    HIPCHK(c, hipMemcpyAsync(P.ptr.p, c->ptr.p, 4 * (n + 1), hipMemcpyDeviceToDevice, c->stream));
-    if (nnz) HIPCHK(c, hipMemcpyAsync(P.ind.p, c->ind.p, 4 * nnz, hipMemcpyDeviceToDevice, c->stream));
+    if (hnnz) HIPCHK(c, hipMemcpy(hind.data(), c->ind.p, 4 * (size_t)hnnz, hipMemcpyDeviceToHost));
+    return 0;
+}
 
-    // 2. adjacency
+// Row groups in dof order packed into tiles of <= cap tasks and <= 24 cap values.
+int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1, int64_t n,
+                const std::vector<int32_t> &hptr, const std::vector<int32_t> &hind, int cap, ReuseTables &T)
+{
+    const int nen = F0.nen, nc = F0.ncomp, nd = nen * nc;
+    const int ncol = nd + (stokes ? 3 : 0);
+    const int ncolp = (ncol + 3) & ~3;
+    const int vcap = REUSE_VCAP_PER_THREAD * cap;
     Adj A0, A1;
     adjacency(F0, A0);
     if (stokes) adjacency(*F1, A1);
-
-    // inverse dof map: dof -> (field, node, comp)
     std::vector<int32_t> inv_node(n, -1);
     std::vector<uint8_t> inv_fc(n, 0);  // field << 4 | comp
     for (int64_t nd0 = 0; nd0 < F0.Nn; nd0++)
@@ -97,24 +93,18 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
             inv_node[d] = (int32_t)m;
             inv_fc[d] = (uint8_t)(1 << 4);
         }
-
-    // 3-4. groups, tiles, tasks, positions
-    std::vector<int4> tiles;
-    std::vector<int4> trows;
-    std::vector<int32_t> t_elem;
-    std::vector<uint32_t> t_info;
-    std::vector<uint8_t> t_pos;
-    t_elem.reserve((size_t)F0.E * nen + (stokes ? F0.E * 3 : 0));
-    t_info.reserve(t_elem.capacity());
-    t_pos.reserve(t_elem.capacity() * ncolp);
+    T = ReuseTables{};
+    T.ncolp = ncolp;
+    T.elem.reserve((size_t)F0.E * nen + (stokes ? F0.E * 3 : 0));
+    T.info.reserve(T.elem.capacity());
+    T.pos.reserve(T.elem.capacity() * ncolp);
     int4 cur = {0, 0, 0, 0}, cr = {0, 0, 0, 0};
     bool open = false;
-    int maxv = 0, maxrank = 0;
     auto close = [&]() {
         if (!open) return;
-        tiles.push_back(cur);
-        trows.push_back(cr);
-        maxv = std::max(maxv, cur.w);
+        T.tiles.push_back(cur);
+        T.trows.push_back(cr);
+        T.maxv = std::max(T.maxv, cur.w);
         open = false;
     };
     int64_t r = 0;
@@ -127,18 +117,19 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
             inv_fc[r + 1] == 1)
             nr = 2;
         const int L = hptr[r + 1] - hptr[r];
-        if (nr == 2 && hptr[r + 2] - hptr[r + 1] != L) return set_err(c, ELFEL_ENOTSUP, "rows of one node differ in pattern");
+        if (nr == 2 && hptr[r + 2] - hptr[r + 1] != L)
+            return set_err(c, ELFEL_ENOTSUP, "rows of one node differ in pattern");
         int deg = 0;
         const Adj *A = nullptr;
         if (node >= 0) {
             A = field == 0 ? &A0 : &A1;
             deg = A->ptr[node + 1] - A->ptr[node];
         }
-        if (deg > REUSE_BLOCK || nr * L > REUSE_VCAP || deg > 16 || L > 255)
+        if (deg > cap || nr * L > vcap || deg > 16 || L > 255)
             return set_err(c, ELFEL_ENOTSUP, "node too highly connected for the reuse tiles");
-        if (open && (cur.y + deg > REUSE_BLOCK || cur.w + nr * L > REUSE_VCAP || cr.y + nr > REUSE_BLOCK)) close();
+        if (open && (cur.y + deg > cap || cur.w + nr * L > vcap || cr.y + nr > cap)) close();
         if (!open) {
-            cur = int4{(int)t_elem.size(), 0, hptr[r], 0};
+            cur = int4{(int)T.elem.size(), 0, hptr[r], 0};
             cr = int4{(int)r, 0, 0, 0};
             open = true;
         }
@@ -149,10 +140,10 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
         for (int s = 0; s < deg; s++) {
             const int32_t e = A->elem[A->ptr[node] + s];
             const int a = A->loc[A->ptr[node] + s];
-            // element columns: velocity/primary dofs, then pressure dofs (Stokes velocity owner)
             int nj = 0;
             for (int k = 0; k < nen; k++)
-                for (int cc = 0; cc < nc; cc++) colbuf[nj++] = F0.h_dof[(int64_t)F0.h_conn[(int64_t)e * nen + k] * nc + cc];
+                for (int cc = 0; cc < nc; cc++)
+                    colbuf[nj++] = F0.h_dof[(int64_t)F0.h_conn[(int64_t)e * nen + k] * nc + cc];
             if (stokes && field == 0)
                 for (int k = 0; k < 3; k++) colbuf[nj++] = F1->h_dof[F1->h_conn[(int64_t)e * 3 + k]];
             for (int j = 0; j < ncolp; j++) {
@@ -163,12 +154,12 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
                         return set_err(c, ELFEL_EINVAL, "internal: element column missing from the pattern");
                     pv = (uint8_t)(it - list);
                 }
-                t_pos.push_back(pv);
+                T.pos.push_back(pv);
             }
-            t_elem.push_back(e);
-            t_info.push_back((uint32_t)a | ((uint32_t)s << 3) | ((uint32_t)rowcode << 7) |
+            T.elem.push_back(e);
+            T.info.push_back((uint32_t)a | ((uint32_t)s << 3) | ((uint32_t)rowcode << 7) |
                              ((uint32_t)(field == 1) << 9) | ((uint32_t)vbase << 10) | ((uint32_t)l8 << 24));
-            maxrank = std::max(maxrank, s + 1);
+            T.maxrank = std::max(T.maxrank, s + 1);
             cr.z = std::max(cr.z, s + 1);
         }
         cur.y += deg;
@@ -177,32 +168,60 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
         r += nr;
     }
     close();
-    const int64_t ntasks = (int64_t)t_elem.size();
-    if ((rc = ensure(c, P.tile, sizeof(int4) * std::max<size_t>(tiles.size(), 1))) ||
-        (rc = ensure(c, P.tile_rows, sizeof(int4) * std::max<size_t>(trows.size(), 1))) ||
+    return 0;
+}
+
+int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P)
+{
+    const int64_t ntasks = (int64_t)T.elem.size();
+    int rc;
+    if ((rc = ensure(c, P.tile, sizeof(int4) * std::max<size_t>(T.tiles.size(), 1))) ||
+        (rc = ensure(c, P.tile_rows, sizeof(int4) * std::max<size_t>(T.trows.size(), 1))) ||
         (rc = ensure(c, P.task_elem, 4 * std::max<int64_t>(ntasks, 1))) ||
         (rc = ensure(c, P.task_info, 4 * std::max<int64_t>(ntasks, 1))) ||
-        (rc = ensure(c, P.task_pos, std::max<size_t>(t_pos.size(), 1))))
+        (rc = ensure(c, P.task_pos, std::max<size_t>(T.pos.size(), 1))))
         return rc;
-    if (!tiles.empty()) {
-        HIPCHK(c, hipMemcpy(P.tile.p, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(P.tile_rows.p, trows.data(), sizeof(int4) * trows.size(), hipMemcpyHostToDevice));
+    if (!T.tiles.empty()) {
+        HIPCHK(c, hipMemcpy(P.tile.p, T.tiles.data(), sizeof(int4) * T.tiles.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(P.tile_rows.p, T.trows.data(), sizeof(int4) * T.trows.size(), hipMemcpyHostToDevice));
     }
     if (ntasks) {
-        HIPCHK(c, hipMemcpy(P.task_elem.p, t_elem.data(), 4 * ntasks, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(P.task_info.p, t_info.data(), 4 * ntasks, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(P.task_pos.p, t_pos.data(), t_pos.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(P.task_elem.p, T.elem.data(), 4 * ntasks, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(P.task_info.p, T.info.data(), 4 * ntasks, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(P.task_pos.p, T.pos.data(), T.pos.size(), hipMemcpyHostToDevice));
     }
+    P.ntiles = (int64_t)T.tiles.size();
+    P.ntasks = ntasks;
+    P.ncolmax = T.ncolp;
+    P.maxrank = T.maxrank;
+    P.lds_vals = T.maxv;
+    return 0;
+}
+
+static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    Pattern &P = c->pat;
+    P.valid = false;
+    const bool stokes = pb->kind == ELFEL_STOKES_GEN || pb->kind == ELFEL_STOKES_VECLAP;
+    const Field &F0 = c->f[0];
+    const Field *F1 = stokes ? &c->f[1] : nullptr;
+    const int64_t n = c->nrow;
+    std::vector<int32_t> hptr, hind;
+    int rc = pattern_to_host(c, pb, tab, cf, hptr, hind);
+    if (rc) return rc;
+    const int64_t nnz = (int64_t)hind.size();
+    if ((rc = ensure(c, P.ptr, 4 * (n + 1))) || (rc = ensure(c, P.ind, 4 * std::max<int64_t>(nnz, 1)))) return rc;
+    HIPCHK(c, hipMemcpyAsync(P.ptr.p, c->ptr.p, 4 * (n + 1), hipMemcpyDeviceToDevice, c->stream));
+    if (nnz) HIPCHK(c, hipMemcpyAsync(P.ind.p, c->ind.p, 4 * nnz, hipMemcpyDeviceToDevice, c->stream));
+    ReuseTables T;
+    if ((rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, REUSE_BLOCK, T))) return rc;
+    if ((rc = upload_tables(c, T, P))) return rc;
     P.kind = pb->kind; P.fe = pb->fe; P.layout = pb->layout;
     P.key_mesh0 = F0.mesh_version; P.key_dof0 = F0.dof_version;
     P.key_mesh1 = stokes ? F1->mesh_version : 0; P.key_dof1 = stokes ? F1->dof_version : 0;
     P.nrow = P.ncol = n;
     P.nnz = nnz;
-    P.ntiles = (int64_t)tiles.size();
-    P.ntasks = ntasks;
-    P.ncolmax = ncolp;
-    P.maxrank = maxrank;
-    P.lds_vals = maxv;
     P.valid = true;
     P.symbolic_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;

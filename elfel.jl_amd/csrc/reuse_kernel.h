@@ -24,7 +24,8 @@
 #include "elem.h"
 
 #define REUSE_BLOCK 256
-#define REUSE_VCAP 6144      // LDS doubles for values of one tile (48 KiB)
+#define REUSE_VCAP_PER_THREAD 24   // LDS doubles for tile values per thread of the block (48 KiB at 256)
+#define REUSE_VCAP (REUSE_VCAP_PER_THREAD * REUSE_BLOCK)
 #define REUSE_COND_MAX 1024.0f // sum_k |grad N_k|^2 * det J above which the mixed kernel uses FP64
 
 struct ReuseArgs {
@@ -157,10 +158,11 @@ __device__ __forceinline__ void owner_values(const QTable &tab, const Geo<R, NEN
     }
 }
 
-template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC>
-__global__ __launch_bounds__(REUSE_BLOCK) void k_reuse(ReuseArgs A, Coef cf, QTable tab, int lds_vals)
+// One tile: a workgroup of BLOCK threads, one task per thread.
+template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC, int BLOCK>
+__device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, const QTable &tab, double *lds,
+                                           int lds_vals, int tile)
 {
-    extern __shared__ double lds[];
     constexpr int ND = NEN * NC;
     constexpr bool STOKES = (KIND == ELFEL_STOKES_GEN || KIND == ELFEL_STOKES_VECLAP);
     constexpr int NCOL = ND + (STOKES ? 3 : 0);
@@ -168,11 +170,11 @@ __global__ __launch_bounds__(REUSE_BLOCK) void k_reuse(ReuseArgs A, Coef cf, QTa
     double *acc = lds;
     double *accF = lds + lds_vals;
     const int tid = threadIdx.x;
-    const int4 td = A.tiles[blockIdx.x];
-    const int4 tr = A.tile_rows[blockIdx.x];
-    for (int v = tid; v < td.w; v += REUSE_BLOCK) acc[v] = 0.0;
+    const int4 td = A.tiles[tile];
+    const int4 tr = A.tile_rows[tile];
+    for (int v = tid; v < td.w; v += BLOCK) acc[v] = 0.0;
     if constexpr (KIND == ELFEL_HEAT)
-        for (int v = tid; v < tr.y; v += REUSE_BLOCK) accF[v] = 0.0;
+        for (int v = tid; v < tr.y; v += BLOCK) accF[v] = 0.0;
 
     const bool active = tid < td.y;
     int rank = -1, rowcode = 0, vbase = 0, L8 = 0;
@@ -263,9 +265,32 @@ __global__ __launch_bounds__(REUSE_BLOCK) void k_reuse(ReuseArgs A, Coef cf, QTa
         __syncthreads();
     }
     double *dst = A.vals + td.z;
-    for (int v = tid; v < td.w; v += REUSE_BLOCK) dst[v] = acc[v];
+    for (int v = tid; v < td.w; v += BLOCK) dst[v] = acc[v];
     if constexpr (KIND == ELFEL_HEAT)
-        for (int v = tid; v < tr.y; v += REUSE_BLOCK) A.F[tr.x + v] = accF[v];
+        for (int v = tid; v < tr.y; v += BLOCK) A.F[tr.x + v] = accF[v];
+}
+
+template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC>
+__global__ __launch_bounds__(REUSE_BLOCK) void k_reuse(ReuseArgs A, Coef cf, QTable tab, int lds_vals)
+{
+    extern __shared__ double lds[];
+    reuse_tile<KIND, R, NEN, NC, NQ, CSC, REUSE_BLOCK>(A, cf, tab, lds, lds_vals, blockIdx.x);
+}
+
+// Batched problems of uneven size in one persistent launch: heat-T3 (1-point)
+// tiles first, then elasticity-T6 (3-point) tiles; each workgroup walks the
+// tile list with stride gridDim.x.  The tile kind is uniform per workgroup.
+template <bool CSC, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_reuse_batch(ReuseArgs H, ReuseArgs Ev, Coef cfh, Coef cfe, QTable th,
+                                                       QTable te, int lds_vals)
+{
+    extern __shared__ double lds[];
+    const int nt = H.ntiles + Ev.ntiles;
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        if (t < H.ntiles) reuse_tile<ELFEL_HEAT, float, 3, 1, 1, CSC, BLOCK>(H, cfh, th, lds, lds_vals, t);
+        else reuse_tile<ELFEL_ELAST, float, 6, 2, 3, CSC, BLOCK>(Ev, cfe, te, lds, lds_vals, t - H.ntiles);
+        __syncthreads();
+    }
 }
 
 // Geometry gather for the reuse kernels: FP64 absolute coordinates, or FP32
@@ -292,6 +317,20 @@ __global__ __launch_bounds__(256) void k_gather_geom(int64_t E, const int32_t *_
         }
     }
 }
+
+// Host-side task tables of the reuse method (reuse.hip).
+struct ReuseTables {
+    std::vector<int4> tiles, trows;
+    std::vector<int32_t> elem;
+    std::vector<uint32_t> info;
+    std::vector<uint8_t> pos;
+    int ncolp = 0, maxv = 0, maxrank = 0;
+};
+int pattern_to_host(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf,
+                    std::vector<int32_t> &hptr, std::vector<int32_t> &hind);
+int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1, int64_t n,
+                const std::vector<int32_t> &hptr, const std::vector<int32_t> &hind, int cap, ReuseTables &T);
+int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P);
 
 typedef int (*reuse_launch_fn)(elfel_hip_ctx *, const ReuseArgs &, const Coef &, const QTable &, int);
 reuse_launch_fn pick_reuse64(int kind, int fe, int nq, int csc);

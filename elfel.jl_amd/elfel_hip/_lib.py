@@ -29,6 +29,8 @@ EXPORTED = [
     "elfel_hip_set_mesh", "elfel_hip_set_dofs", "elfel_hip_set_coefficients", "elfel_hip_assemble",
     "elfel_hip_copy_vector", "elfel_hip_synchronize", "elfel_hip_geometry_stats", "elfel_hip_pcg",
     "elfel_hip_timing", "elfel_hip_timing_read",
+    "elfel_hip_batch_begin", "elfel_hip_batch_add", "elfel_hip_batch_build", "elfel_hip_batch_assemble",
+    "elfel_hip_batch_info", "elfel_hip_batch_copy", "elfel_hip_batch_stats",
 ]
 
 
@@ -84,6 +86,15 @@ def load():
     lib.elfel_hip_geometry_stats.argtypes = [P, f64p, i64p]
     lib.elfel_hip_pcg.argtypes = [P, ctypes.c_int64, f64p, f64p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), f64p]
+    lib.elfel_hip_batch_begin.argtypes = [P]
+    lib.elfel_hip_batch_add.argtypes = [P, ctypes.POINTER(Problem), ctypes.c_int32, ctypes.c_int64, i64p,
+                                        ctypes.c_int64, f64p, ctypes.c_int32, i64p, ctypes.c_int64, f64p]
+    lib.elfel_hip_batch_build.argtypes = [P, ctypes.c_int]
+    lib.elfel_hip_batch_assemble.argtypes = [P, ctypes.c_int]
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib.elfel_hip_batch_info.argtypes = [P, ctypes.c_int32, i64p, i64p, i64p, ip, ip]
+    lib.elfel_hip_batch_copy.argtypes = [P, ctypes.c_int32, ctypes.c_int, i64p, i64p, f64p, f64p]
+    lib.elfel_hip_batch_stats.argtypes = [P, i64p, i64p, i64p, i64p, f64p]
     lib.elfel_hip_timing.argtypes = [P, ctypes.c_int]
     lib.elfel_hip_timing_read.argtypes = [P, f64p, i64p]
     for name in EXPORTED:
@@ -234,6 +245,56 @@ class Context:
         self._chk(self.lib.elfel_hip_pcg(self.h, nfree, _ptr(F, ctypes.c_double), _ptr(x, ctypes.c_double), precision,
                                          tol, maxit, ctypes.byref(it), ctypes.byref(outer), ctypes.byref(rr)), "pcg")
         return x, it.value, outer.value, rr.value
+
+    # ---- batched problems ----------------------------------------------------
+    def batch_begin(self):
+        self._chk(self.lib.elfel_hip_batch_begin(self.h), "batch_begin")
+
+    def batch_add(self, kind, fe, rule, conn, xy, dofnums, ndofs, coef_e=None, kappa=1.0, Q=0.0, D=None,
+                  layout=CSC):
+        pb = Problem()
+        pb.kind, pb.fe, pb.rule, pb.precision, pb.method, pb.layout = kind, fe, rule, MIXED, REUSE, layout
+        pb.kappa, pb.Q, pb.mu = kappa, Q, 1.0
+        if D is not None:
+            Dc = np.asarray(D, np.float64).reshape(3, 3).ravel(order="F")
+            for i in range(9):
+                pb.D[i] = Dc[i]
+        conn, xy, d = _i64(conn), _f64(xy), _i64(dofnums)
+        if d.ndim == 1:
+            d = d.reshape(-1, 1)
+        ce = None if coef_e is None else _f64(coef_e)
+        self._chk(self.lib.elfel_hip_batch_add(self.h, ctypes.byref(pb), conn.shape[1], conn.shape[0],
+                                               _ptr(conn, ctypes.c_int64), xy.shape[0], _ptr(xy, ctypes.c_double),
+                                               d.shape[1], _ptr(d, ctypes.c_int64), ndofs,
+                                               _ptr(ce, ctypes.c_double)), "batch_add")
+
+    def batch_build(self, autotune=False):
+        self._chk(self.lib.elfel_hip_batch_build(self.h, 1 if autotune else 0), "batch_build")
+
+    def batch_assemble(self, wait=True):
+        self._chk(self.lib.elfel_hip_batch_assemble(self.h, 1 if wait else 0), "batch_assemble")
+
+    def batch_info(self, k):
+        nd, nnz, ne = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        bl, gr = ctypes.c_int(), ctypes.c_int()
+        self._chk(self.lib.elfel_hip_batch_info(self.h, k, ctypes.byref(nd), ctypes.byref(nnz), ctypes.byref(ne),
+                                                ctypes.byref(bl), ctypes.byref(gr)), "batch_info")
+        return dict(ndofs=nd.value, nnz=nnz.value, nelem=ne.value, block=bl.value, grid=gr.value)
+
+    def batch_copy(self, k, base=0):
+        info = self.batch_info(k)
+        n, nnz = info["ndofs"], info["nnz"]
+        ptr, ind, val, F = np.empty(n + 1, np.int64), np.empty(nnz, np.int64), np.empty(nnz), np.zeros(n)
+        self._chk(self.lib.elfel_hip_batch_copy(self.h, k, base, _ptr(ptr, ctypes.c_int64), _ptr(ind, ctypes.c_int64),
+                                                _ptr(val, ctypes.c_double), _ptr(F, ctypes.c_double)), "batch_copy")
+        return ptr, ind, val, F
+
+    def batch_stats(self):
+        a = [ctypes.c_int64() for _ in range(4)]
+        t = ctypes.c_double()
+        self._chk(self.lib.elfel_hip_batch_stats(self.h, *[ctypes.byref(x) for x in a], ctypes.byref(t)),
+                  "batch_stats")
+        return dict(nprob=a[0].value, nelem=a[1].value, nnz=a[2].value, ntiles=a[3].value, tuned_ms=t.value)
 
     def timing(self, enable=True):
         self._chk(self.lib.elfel_hip_timing(self.h, 1 if enable else 0), "timing")

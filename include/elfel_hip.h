@@ -136,6 +136,29 @@ int elfel_hip_geometry_stats(elfel_hip_ctx *ctx, double *min_det, int64_t *n_fp6
 int elfel_hip_pcg(elfel_hip_ctx *ctx, int64_t nfree, const double *F, double *x, int precision,
                   double tol, int maxit, int *iters, int *outer, double *relres);
 
+/* ---- batched problems ------------------------------------------------------
+ * Many independent problems of uneven size (heat T3 1-point and plane
+ * elasticity T6 3-point, mixed precision) assembled by one persistent fused
+ * launch replayed from a hipGraph.  Each problem is one reference driver's
+ * integrate! + finish! (test/test_heat.jl:35-71, test/test_elasticity.jl:
+ * 149-181); problems of one kind share their uniform coefficients (kappa, Q,
+ * D); coef_e is the per-element conductivity (heat) or scale of D
+ * (elasticity), or NULL.  batch_build runs the symbolic phase of every
+ * problem, picks the launch shape (autotune = 1 times every variant on the
+ * batch; 0 uses the on-disk cache when it has an entry) and captures the
+ * graph.  No counterpart in the reference beyond the per-problem loops. */
+int elfel_hip_batch_begin(elfel_hip_ctx *ctx);
+int elfel_hip_batch_add(elfel_hip_ctx *ctx, const elfel_hip_problem *pb, int32_t nen, int64_t E, const int64_t *conn,
+                        int64_t Nn, const double *xy, int32_t ncomp, const int64_t *dofnums, int64_t ndofs,
+                        const double *coef_e);
+int elfel_hip_batch_build(elfel_hip_ctx *ctx, int autotune);
+int elfel_hip_batch_assemble(elfel_hip_ctx *ctx, int wait);
+int elfel_hip_batch_info(elfel_hip_ctx *ctx, int32_t k, int64_t *ndofs, int64_t *nnz, int64_t *nelem, int *block,
+                         int *grid);
+int elfel_hip_batch_copy(elfel_hip_ctx *ctx, int32_t k, int base, int64_t *ptr, int64_t *ind, double *val, double *F);
+int elfel_hip_batch_stats(elfel_hip_ctx *ctx, int64_t *nprob, int64_t *nelem, int64_t *nnz, int64_t *ntiles,
+                          double *tuned_ms);
+
 /* ---- timing ---------------------------------------------------------------
  * hipEvent timing of each phase on the context's stream, accumulated while
  * enabled.  ms[ELFEL_NPHASES], counts[ELFEL_NPHASES] (launch counts). */
