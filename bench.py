@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-N", type=int, default=400)
     ap.add_argument("--only", choices=["all", "mixed", "fp64", "fp64-reuse"], default="all")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
+                    help="only run a device copy of this size (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -51,6 +53,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU per box; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     import elfel_hip as eh
     from elfel_hip import _lib as L
     from elfel_hip import workloads as W
@@ -107,8 +112,8 @@ def main():
     nnz = next(iter(res.values()))["nnz"]
     # algorithmic bytes
     survey_bytes = E * 6 * 4 + Nn * 16 + n * 4 + nnz * (8 + 4) + (n + 1) * 4  # full assembly, mesh -> CSC
-    geom_bytes_f32 = E * 6 * 2 * 4
-    kernel_bytes_mixed = geom_bytes_f32 + nnz * 8          # k_reuse: element records in, values out
+    geom_bytes = E * 6 * 2 * 8                              # FP64 element coordinate records
+    kernel_bytes_mixed = geom_bytes + nnz * 8               # k_reuse: element records in, values out
     step_bytes_reuse = E * 6 * 4 + Nn * 16 + nnz * 8       # gather + k_reuse: conn + coords in, values out
 
     out = {}
@@ -173,6 +178,19 @@ def main():
         "device": {"name": devname, "cus": ncu},
     }
     print(json.dumps(line))
+
+
+def copy_probe(mib):
+    """A device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+    counters give the scale factor between counter units and bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": n * 4, "launches": 3}))
 
 
 def cpu_baseline(N):

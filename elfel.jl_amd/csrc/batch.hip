@@ -181,11 +181,11 @@ static int batch_launch(elfel_hip_ctx *c, Batch *b, int block, int grid)
     const int bi = block == 128 ? 0 : 1;
     KindData &H = b->kd[0], &Ev = b->kd[1];
     if (H.E) {
-        int rc = gather_geom32(c, 3, H.E, (const int32_t *)H.conn.p, (const double2 *)H.xy.p, (float *)H.geom.p);
+        int rc = gather_geom_rel(c, 3, H.E, (const int32_t *)H.conn.p, (const double2 *)H.xy.p, (double *)H.geom.p);
         if (rc) return rc;
     }
     if (Ev.E) {
-        int rc = gather_geom32(c, 6, Ev.E, (const int32_t *)Ev.conn.p, (const double2 *)Ev.xy.p, (float *)Ev.geom.p);
+        int rc = gather_geom_rel(c, 6, Ev.E, (const int32_t *)Ev.conn.p, (const double2 *)Ev.xy.p, (double *)Ev.geom.p);
         if (rc) return rc;
     }
     QTable th{}, te{};
@@ -317,7 +317,7 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
             }
         }
         if ((rc = ensure(c, K.conn, 4 * conn.size())) || (rc = ensure(c, K.xy, 8 * xy.size())) ||
-            (rc = ensure(c, K.geom, 4 * (size_t)K.E * nen * 2)))
+            (rc = ensure(c, K.geom, 8 * (size_t)K.E * nen * 2)))
             return rc;
         HIPCHK(c, hipMemcpy(K.conn.p, conn.data(), 4 * conn.size(), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(K.xy.p, xy.data(), 8 * xy.size(), hipMemcpyHostToDevice));

@@ -250,7 +250,7 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
     }
     const int64_t E = F0.E;
     const int nen = F0.nen;
-    if ((rc = ensure(c, c->geom, (size_t)E * nen * 2 * (mixed ? 4 : 8))) ||
+    if ((rc = ensure(c, c->geom, (size_t)E * nen * 2 * 8)) ||
         (rc = ensure(c, c->val, 8 * std::max<int64_t>(P.nnz, 1))))
         return rc;
     const bool heat = pb->kind == ELFEL_HEAT;
@@ -258,7 +258,7 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
     hipEvent_t ev = nullptr;
     tm_begin(c, ELFEL_PH_GATHER, &ev);
     if (mixed)
-        rc = gather_geom32(c, nen, E, (const int32_t *)F0.conn.p, (const double2 *)F0.xy.p, (float *)c->geom.p);
+        rc = gather_geom_rel(c, nen, E, (const int32_t *)F0.conn.p, (const double2 *)F0.xy.p, (double *)c->geom.p);
     else
         rc = gather_geom64(c, nen, E, (const int32_t *)F0.conn.p, (const double2 *)F0.xy.p, (double *)c->geom.p);
     if (rc) return rc;
@@ -276,8 +276,7 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
     A.xy = (const double2 *)F0.xy.p;
     A.vals = (double *)c->val.p;
     A.F = heat ? (double *)c->F.p : nullptr;
-    A.fallback = mixed ? (int64_t *)c->dscal.p + 2 : nullptr;
-    if (mixed) HIPCHK(c, hipMemsetAsync(A.fallback, 0, 8, c->stream));
+    A.fallback = nullptr;
     tm_begin(c, ELFEL_PH_NUMERIC, &ev);
     rc = fn(c, A, cf, tab, P.lds_vals);
     if (rc) return rc;

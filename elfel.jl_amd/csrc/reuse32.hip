@@ -1,17 +1,17 @@
-// reuse32.hip -- mixed-precision instantiations of the reuse kernels: FP32
-// shape gradients, Jacobian and element products (contracted into FMAs), FP64
-// sums into the matrix, FP64 branch for badly shaped elements.
+// reuse32.hip -- mixed-precision instantiations of the reuse kernels: FP64
+// Jacobian, determinant and inverse, FP32 shape gradients and element
+// products (contracted into FMAs), FP64 sums into the matrix.
 #include "reuse_kernel.h"
 
 reuse_launch_fn pick_reuse32(int kind, int fe, int nq, int csc) { REUSE_PICK_BODY(float) }
 
-int gather_geom32(elfel_hip_ctx *c, int nen, int64_t E, const int32_t *conn, const double2 *xy, float *geom)
+int gather_geom_rel(elfel_hip_ctx *c, int nen, int64_t E, const int32_t *conn, const double2 *xy, double *geom)
 {
     if (E <= 0) return 0;
     const dim3 g(ceil_div(E, 256)), b(256);
-    if (nen == 3) hipLaunchKernelGGL((k_gather_geom<float, 3>), g, b, 0, c->stream, E, conn, xy, geom);
-    else if (nen == 4) hipLaunchKernelGGL((k_gather_geom<float, 4>), g, b, 0, c->stream, E, conn, xy, geom);
-    else hipLaunchKernelGGL((k_gather_geom<float, 6>), g, b, 0, c->stream, E, conn, xy, geom);
+    if (nen == 3) hipLaunchKernelGGL((k_gather_geom<3, true>), g, b, 0, c->stream, E, conn, xy, geom);
+    else if (nen == 4) hipLaunchKernelGGL((k_gather_geom<4, true>), g, b, 0, c->stream, E, conn, xy, geom);
+    else hipLaunchKernelGGL((k_gather_geom<6, true>), g, b, 0, c->stream, E, conn, xy, geom);
     HIPCHK(c, hipGetLastError());
     return 0;
 }

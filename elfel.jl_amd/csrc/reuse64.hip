@@ -10,9 +10,9 @@ int gather_geom64(elfel_hip_ctx *c, int nen, int64_t E, const int32_t *conn, con
 {
     if (E <= 0) return 0;
     const dim3 g(ceil_div(E, 256)), b(256);
-    if (nen == 3) hipLaunchKernelGGL((k_gather_geom<double, 3>), g, b, 0, c->stream, E, conn, xy, geom);
-    else if (nen == 4) hipLaunchKernelGGL((k_gather_geom<double, 4>), g, b, 0, c->stream, E, conn, xy, geom);
-    else hipLaunchKernelGGL((k_gather_geom<double, 6>), g, b, 0, c->stream, E, conn, xy, geom);
+    if (nen == 3) hipLaunchKernelGGL((k_gather_geom<3, false>), g, b, 0, c->stream, E, conn, xy, geom);
+    else if (nen == 4) hipLaunchKernelGGL((k_gather_geom<4, false>), g, b, 0, c->stream, E, conn, xy, geom);
+    else hipLaunchKernelGGL((k_gather_geom<6, false>), g, b, 0, c->stream, E, conn, xy, geom);
     HIPCHK(c, hipGetLastError());
     return 0;
 }

@@ -26,7 +26,7 @@
 #define REUSE_BLOCK 256
 #define REUSE_VCAP_PER_THREAD 24   // LDS doubles for tile values per thread of the block (48 KiB at 256)
 #define REUSE_VCAP (REUSE_VCAP_PER_THREAD * REUSE_BLOCK)
-#define REUSE_COND_MAX 1024.0f // sum_k |grad N_k|^2 * det J above which the mixed kernel uses FP64
+#define REUSE_COND_MAX 1024.0f // sum_k |grad N_k|^2 * det J: above it an element counts as badly shaped
 
 struct ReuseArgs {
     const int4 *tiles;        // task0, ntasks, v0, nvals
@@ -178,17 +178,24 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
 
     const bool active = tid < td.y;
     int rank = -1, rowcode = 0, vbase = 0, L8 = 0;
+    bool pown = false;
     R out[2][ND + 3];
     R fo = R(0);
-    int64_t task = 0;
+    // column positions, prefetched as 32-bit words before the barriered adds
+    constexpr int NPW = (NCOL + 3) / 4;
+    uint32_t posw[NPW];
     if (active) {
-        task = (int64_t)td.x + tid;
+        const int64_t task = (int64_t)td.x + tid;
         const int32_t e = A.task_elem[task];
         const uint32_t info = A.task_info[task];
+        const uint32_t *pw = (const uint32_t *)(A.task_pos + task * A.ncolp);
+#pragma unroll
+        for (int w = 0; w < NPW; w++) posw[w] = pw[w];
         const int a = info & 7;
         rank = (info >> 3) & 15;
         rowcode = (info >> 7) & 3;
         const bool powner = STOKES && ((info >> 9) & 1);
+        pown = powner;
         vbase = (info >> 10) & 0x3FFF;
         L8 = info >> 24;
         const int comp0 = rowcode == 1 ? 1 : 0;
@@ -196,44 +203,28 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
         const double sc = cf.scale_e ? cf.scale_e[e] : 1.0;
         const double kapd = cf.kappa_e ? cf.kappa_e[e] : cf.kappa;
         const double Qd = cf.Q_e ? cf.Q_e[e] : cf.Q;
-        const R *gp = (const R *)A.geom + (int64_t)e * (2 * NEN);
-        R X[NEN], Y[NEN];
+        // Geometry in FP64 from the gathered records (absolute coordinates for the
+        // FP64 kernel, coordinates relative to the element's first node for the
+        // mixed kernel).  The mixed kernel rounds the spatial gradients and JxW to
+        // FP32 for the element products: an FP32 Jacobian loses too much to the
+        // cancellation in sum_k x_k grad N_k on curved T6 elements.
+        const double *gp = (const double *)A.geom + (int64_t)e * (2 * NEN);
+        double X[NEN], Y[NEN];
 #pragma unroll
         for (int k = 0; k < NEN; k++) { X[k] = gp[2 * k]; Y[k] = gp[2 * k + 1]; }
         Geo<R, NEN, NQ> g;
-        geometry<R, NEN, NQ>(tab, X, Y, g);
         if constexpr (MIXED) {
-            bool good = true;
+            Geo<double, NEN, NQ> gd;
+            geometry<double, NEN, NQ>(tab, X, Y, gd);
 #pragma unroll
             for (int q = 0; q < NQ; q++) {
-                // Shape-quality proxy: sum_k |grad N_k|^2 * det J is dimensionless, O(1-10)
-                // for well-shaped elements and grows like the condition number of J for
-                // distorted ones; a non-positive det also fails.
-                float j2 = 0.f;
+                g.jxw[q] = (float)gd.jxw[q];
+                g.det[q] = (float)gd.det[q];
 #pragma unroll
-                for (int k = 0; k < NEN; k++) j2 += g.gx[q][k] * g.gx[q][k] + g.gy[q][k] * g.gy[q][k];
-                good = good && (g.det[q] > 0.f) && (j2 * g.det[q] < REUSE_COND_MAX);
+                for (int k = 0; k < NEN; k++) { g.gx[q][k] = (float)gd.gx[q][k]; g.gy[q][k] = (float)gd.gy[q][k]; }
             }
-            if (!good) {
-                // FP64 Jacobian, determinant and inverse from the absolute coordinates;
-                // the element products then proceed in FP32 as for every element.
-                if (A.fallback) atomicAdd((unsigned long long *)A.fallback, 1ull);
-                double Xd[NEN], Yd[NEN];
-#pragma unroll
-                for (int k = 0; k < NEN; k++) {
-                    const double2 p = A.xy[A.conn[(int64_t)e * NEN + k]];
-                    Xd[k] = p.x; Yd[k] = p.y;
-                }
-                Geo<double, NEN, NQ> gd;
-                geometry<double, NEN, NQ>(tab, Xd, Yd, gd);
-#pragma unroll
-                for (int q = 0; q < NQ; q++) {
-                    g.jxw[q] = (float)gd.jxw[q];
-                    g.det[q] = (float)gd.det[q];
-#pragma unroll
-                    for (int k = 0; k < NEN; k++) { g.gx[q][k] = (float)gd.gx[q][k]; g.gy[q][k] = (float)gd.gy[q][k]; }
-                }
-            }
+        } else {
+            geometry<double, NEN, NQ>(tab, X, Y, g);
         }
         R Dr[9];
 #pragma unroll
@@ -244,12 +235,7 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
     const int nranks = tr.z;
     for (int s = 0; s < nranks; s++) {
         if (rank == s) {
-            const uint8_t *pp = A.task_pos + task * A.ncolp;
-            uint8_t pos[NCOL];
-#pragma unroll
-            for (int j = 0; j < NCOL; j++) pos[j] = pp[j];
-            const bool powner = STOKES && ((A.task_info[task] >> 9) & 1);
-            const int ncols = (STOKES && powner) ? ND : NCOL;
+            const int ncols = (STOKES && pown) ? ND : NCOL;
             const int nr = rowcode == 2 ? 2 : 1;
 #pragma unroll
             for (int c = 0; c < 2; c++) {
@@ -257,7 +243,7 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
                     double *row = acc + vbase + c * L8;
 #pragma unroll
                     for (int j = 0; j < NCOL; j++)
-                        if (j < ncols) row[pos[j]] += (double)out[c][j];
+                        if (j < ncols) row[(posw[j >> 2] >> (8 * (j & 3))) & 0xFF] += (double)out[c][j];
                 }
             }
             if constexpr (KIND == ELFEL_HEAT) accF[L8] += (double)fo;
@@ -293,28 +279,23 @@ __global__ __launch_bounds__(BLOCK) void k_reuse_batch(ReuseArgs H, ReuseArgs Ev
     }
 }
 
-// Geometry gather for the reuse kernels: FP64 absolute coordinates, or FP32
-// coordinates relative to the element's first node (differences in FP64, so
-// the FP32 Jacobian does not lose the digits the absolute position holds).
-template <typename R, int NEN>
+// Geometry gather for the reuse kernels: per element, FP64 coordinates of its
+// nodes -- absolute (FP64 kernel: the reference's operands exactly) or
+// relative to the element's first node (mixed kernel).
+template <int NEN, bool RELATIVE>
 __global__ __launch_bounds__(256) void k_gather_geom(int64_t E, const int32_t *__restrict__ conn,
-                                                     const double2 *__restrict__ xy, R *__restrict__ geom)
+                                                     const double2 *__restrict__ xy, double *__restrict__ geom)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     double2 p[NEN];
 #pragma unroll
     for (int k = 0; k < NEN; k++) p[k] = xy[conn[e * NEN + k]];
-    R *g = geom + e * (2 * NEN);
+    double2 *g = (double2 *)(geom + e * (2 * NEN));
 #pragma unroll
     for (int k = 0; k < NEN; k++) {
-        if constexpr (sizeof(R) == 8) {
-            g[2 * k] = p[k].x;
-            g[2 * k + 1] = p[k].y;
-        } else {
-            g[2 * k] = (R)(p[k].x - p[0].x);
-            g[2 * k + 1] = (R)(p[k].y - p[0].y);
-        }
+        if constexpr (RELATIVE) g[k] = make_double2(p[k].x - p[0].x, p[k].y - p[0].y);
+        else g[k] = p[k];
     }
 }
 
@@ -336,7 +317,7 @@ typedef int (*reuse_launch_fn)(elfel_hip_ctx *, const ReuseArgs &, const Coef &,
 reuse_launch_fn pick_reuse64(int kind, int fe, int nq, int csc);
 reuse_launch_fn pick_reuse32(int kind, int fe, int nq, int csc);
 int gather_geom64(elfel_hip_ctx *c, int nen, int64_t E, const int32_t *conn, const double2 *xy, double *geom);
-int gather_geom32(elfel_hip_ctx *c, int nen, int64_t E, const int32_t *conn, const double2 *xy, float *geom);
+int gather_geom_rel(elfel_hip_ctx *c, int nen, int64_t E, const int32_t *conn, const double2 *xy, double *geom);
 
 // Instantiation helpers for reuse64.hip / reuse32.hip.
 template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC>
