@@ -1,17 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark: element integration + global assembly on one MI355X.
 
-Workload (default): plane-strain elasticity, 400 x 400 grid of T6 triangles
-(320,000 elements, 1,283,202 dofs), 3-point rule, nodes jittered by
-U(-0.15, 0.15) h (seeded SplitMix64), inputs resident in HBM.
+Default workload (--config c2): plane-strain elasticity, 400 x 400 grid of T6
+triangles (320,000 elements, 1,283,202 dofs), 3-point rule, nodes jittered by
+U(-0.15, 0.15) h (seeded SplitMix64), inputs resident in HBM, CSC output.
 
-A step is one assembly of the whole mesh into the global CSC matrix:
-  * mixed  : REUSE method, FP32 element arithmetic, FP64 sums (the value);
-  * fp64   : the plain FP64 method (gather, element kernel, COO in stream
-             order, stable radix sort, in-order segmented reduce), every step;
+A step is one assembly of the whole mesh into the global sparse matrix:
+  * mixed      : REUSE method, FP64 geometry, FP32 element products, FP64 sums
+                 (the value);
+  * fp64       : the plain FP64 method (element kernel, COO in stream order,
+                 stable radix sort, in-order segmented reduce), every step;
   * fp64-reuse : REUSE method in FP64 (bit-identical to the plain method).
 The pattern of the REUSE method is built once before the timed region and its
 build time is reported separately.
+
+Other workloads, each printing its own line:
+  c1  heat T3 100x100, 1-point rule, CSR + load vector;
+  c3  Stokes Taylor-Hood T6/T3 500x500, K and B blocks in one launch, CSC;
+  c4  256 heat-T3 / elasticity-T6 problems of uneven size in one persistent
+      launch replayed from a hipGraph (mixed), against the FP64 plain method
+      problem by problem;
+  c5  heat Q4 4096x4096, 2x2 Gauss, lognormal conductivity, then Jacobi-PCG
+      (FP32 inner iterations + FP64 refinement, and FP64) to 1e-10.
 
 Prints ONE JSON line.  --gpus N > 1 is not supported (one GPU per box).
 """
@@ -27,6 +37,7 @@ sys.path.insert(0, os.path.join(ROOT, "elfel.jl_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "elements assembled/s + HBM GB/s: FP32-mixed vs FP64 GPU path at <=1e-6 rel err"
 
 
 def parse():
@@ -34,10 +45,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--N", type=int, default=400, help="grid size of the T6 elasticity workload")
+    ap.add_argument("--config", choices=["c1", "c2", "c3", "c4", "c5"], default="c2")
+    ap.add_argument("--N", type=int, default=0, help="grid size (default: the configuration's)")
     ap.add_argument("--fp64-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-N", type=int, default=400)
+    ap.add_argument("--no-pcg", action="store_true", help="c5: assembly only")
     ap.add_argument("--only", choices=["all", "mixed", "fp64", "fp64-reuse"], default="all")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
                     help="only run a device copy of this size (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -45,113 +57,158 @@ def parse():
 
 
 def rel_frob(a, b):
-    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU per box; --gpus must be 1"}))
-        sys.exit(2)
-    if args.copy_probe:
-        copy_probe(args.copy_probe)
-        return
-    import elfel_hip as eh
-    from elfel_hip import _lib as L
-    from elfel_hip import workloads as W
+# ---------------------------------------------------------------------------
+# single-problem workloads
+# ---------------------------------------------------------------------------
+def setup_case(cfgname, N, ctx, L, W):
+    """Uploads the mesh(es) and dof maps; returns the assemble() arguments and
+    the sizes that enter the byte counts."""
+    if cfgname == "c1":
+        N = N or 100
+        cfg = W.heat_t3(N)
+        f = cfg["fesp"]
+        ctx.set_mesh(0, f.mesh.conn, f.mesh.xy)
+        ctx.set_dofs(0, f.dofnums, cfg["ndofs"])
+        return dict(cfg=cfg, N=N, kind=L.HEAT, fe=L.T3, rule=1, layout=L.CSR, kw=dict(kappa=1.0, Q=-6.0),
+                    workload=f"heat T3 {N}x{N}, 1-pt rule, kappa 1, Q -6, CSR + load vector",
+                    E=f.mesh.nelements, Nn=f.mesh.nnodes, n=cfg["ndofs"], nen=3, nen_p=0, ncoef=0, heat=True,
+                    kernel="k_reuse<HEAT, float, T3, CSR>", oracle=dict(kind="heat", fesp=f))
+    if cfgname == "c2":
+        N = N or 400
+        cfg = W.elast_t6(N, frac=0.15)
+        f = cfg["fesp"]
+        ctx.set_mesh(0, f.mesh.conn, f.mesh.xy)
+        ctx.set_dofs(0, f.dofnums, cfg["ndofs"])
+        return dict(cfg=cfg, N=N, kind=L.ELAST, fe=L.T6, rule=3, layout=L.CSC, kw=dict(D=cfg["D"]),
+                    workload=f"plane-strain elasticity T6 {N}x{N}, 3-pt rule, +-15% h node jitter",
+                    E=f.mesh.nelements, Nn=f.mesh.nnodes, n=cfg["ndofs"], nen=6, nen_p=0, ncoef=0, heat=False,
+                    kernel="k_reuse<ELAST, float, T6, CSC>", oracle=dict(kind="elast", fesp=f, D=cfg["D"]))
+    if cfgname == "c3":
+        N = N or 500
+        cfg = W.stokes_t6t3(N)
+        u, p = cfg["ufesp"], cfg["pfesp"]
+        n = cfg["ndofs"]
+        ctx.set_mesh(0, u.mesh.conn, u.mesh.xy)
+        ctx.set_dofs(0, u.dofnums, n)
+        ctx.set_mesh(1, p.mesh.conn, p.mesh.xy)
+        ctx.set_dofs(1, p.dofnums, n)
+        return dict(cfg=cfg, N=N, kind=L.STOKES_GEN, fe=L.T6, rule=3, layout=L.CSC, kw=dict(D=cfg["D"], mu=1.0),
+                    workload=f"Stokes Taylor-Hood T6/T3 {N}x{N}, 3-pt rule, K and B in one launch",
+                    E=u.mesh.nelements, Nn=u.mesh.nnodes, n=n, nen=6, nen_p=3, ncoef=0, heat=False,
+                    kernel="k_reuse<STOKES_GEN, float, T6, CSC>",
+                    oracle=dict(kind="stokes", fesp=u, pfesp=p, D=cfg["D"]))
+    if cfgname == "c5":
+        N = N or 4096
+        cfg = W.heat_q4(N, sigma=1.0)
+        f = cfg["fesp"]
+        ctx.set_mesh(0, f.mesh.conn, f.mesh.xy)
+        ctx.set_dofs(0, f.dofnums, cfg["ndofs"])
+        ctx.set_coefficients(f.mesh.nelements, cfg["kappa_e"], cfg["Q_e"], None)
+        return dict(cfg=cfg, N=N, kind=L.HEAT, fe=L.Q4, rule=2, layout=L.CSR, kw=dict(),
+                    workload=f"heat Q4 {N}x{N}, 2x2 Gauss, kappa_e ~ lognormal(0,1), f_e ~ U[-1,1], CSR + load vector",
+                    E=f.mesh.nelements, Nn=f.mesh.nnodes, n=cfg["ndofs"], nen=4, nen_p=0, ncoef=2, heat=True,
+                    kernel="k_reuse<HEAT, float, Q4, CSR>",
+                    oracle=dict(kind="heat", fesp=f, kappa_e=cfg["kappa_e"], Q_e=cfg["Q_e"]))
+    raise ValueError(cfgname)
 
-    t0 = time.time()
-    cfg = W.elast_t6(args.N, frac=0.15)
-    t_host = time.time() - t0
-    fesp, n = cfg["fesp"], cfg["ndofs"]
-    mesh = fesp.mesh
-    E, Nn = mesh.nelements, mesh.nnodes
+
+def bench_single(args, L, W):
     ctx = L.Context(0)
     devname, ncu = ctx.device_name()
     t0 = time.time()
-    ctx.set_mesh(0, mesh.conn, mesh.xy)
-    ctx.set_dofs(0, fesp.dofnums, n)
-    t_upload = time.time() - t0
-    D = cfg["D"]
-    rule = 3
+    cs = setup_case(args.config, args.N, ctx, L, W)
+    t_setup = time.time() - t0
+    E, Nn, n = cs["E"], cs["Nn"], cs["n"]
+
+    def assemble(precision, method, wait=True):
+        return ctx.assemble(cs["kind"], cs["fe"], cs["rule"], precision=precision, method=method,
+                            layout=cs["layout"], wait=wait, **cs["kw"])
 
     def run(precision, method, steps, warmup):
         ctx.timing(False)
-        nnz = ctx.assemble(L.ELAST, L.T6, rule, precision=precision, method=method, layout=L.CSC, D=D)
+        nnz = assemble(precision, method)
         ctx.synchronize()
         for _ in range(warmup):
-            ctx.assemble(L.ELAST, L.T6, rule, precision=precision, method=method, layout=L.CSC, D=D, wait=False)
+            assemble(precision, method, wait=False)
         ctx.synchronize()
         ctx.timing(True)
         w0 = time.perf_counter()
         for _ in range(steps):
-            ctx.assemble(L.ELAST, L.T6, rule, precision=precision, method=method, layout=L.CSC, D=D, wait=False)
+            assemble(precision, method, wait=False)
         ctx.synchronize()
         wall = (time.perf_counter() - w0) * 1e3 / steps
         ph = ctx.timing_read()
         ctx.timing(False)
         ptr, ind, val, shape, lay = ctx.copy_sparse(base=0)
-        return dict(nnz=nnz, wall_ms=wall, phases=ph, val=val, ptr=ptr, ind=ind)
+        F = ctx.copy_vector(n) if cs["heat"] else None
+        return dict(nnz=nnz, wall_ms=wall, phases=ph, val=val, ptr=ptr, ind=ind, F=F)
 
     res = {}
     # Symbolic phase of the REUSE method (pattern + tiles), timed once.
-    ctx.timing(True)
     t0 = time.perf_counter()
-    ctx.assemble(L.ELAST, L.T6, rule, precision=L.MIXED, method=L.REUSE, layout=L.CSC, D=D)
+    assemble(L.MIXED, L.REUSE)
     ctx.synchronize()
     t_sym = (time.perf_counter() - t0) * 1e3
-    ctx.timing(False)
     if args.only in ("all", "mixed"):
         res["mixed"] = run(L.MIXED, L.REUSE, args.steps, args.warmup)
     if args.only in ("all", "fp64-reuse"):
         res["fp64_reuse"] = run(L.FP64, L.REUSE, args.steps, args.warmup)
     if args.only in ("all", "fp64"):
         res["fp64"] = run(L.FP64, L.SORT, max(1, min(args.fp64_steps, args.steps)), min(2, args.warmup))
-    mindet, nfallback = ctx.geometry_stats()
+    mindet, nbad = ctx.geometry_stats()
 
     nnz = next(iter(res.values()))["nnz"]
-    # algorithmic bytes
-    survey_bytes = E * 6 * 4 + Nn * 16 + n * 4 + nnz * (8 + 4) + (n + 1) * 4  # full assembly, mesh -> CSC
-    geom_bytes = E * 6 * 2 * 8                              # FP64 element coordinate records
-    kernel_bytes_mixed = geom_bytes + nnz * 8               # k_reuse: element records in, values out
-    step_bytes_reuse = E * 6 * 4 + Nn * 16 + nnz * 8       # gather + k_reuse: conn + coords in, values out
+    ncoef = cs["ncoef"]
+    # compulsory bytes of a full assembly (mesh -> CSR/CSC + load vector)
+    full_bytes = (E * (cs["nen"] + cs["nen_p"]) * 4 + Nn * 16 + E * 8 * ncoef + n * 4 + nnz * 12 + (n + 1) * 4
+                  + (n * 8 if cs["heat"] else 0))
+    # k_reuse with the gather fused in: conn + coordinates + coefficients in, values (+ F) out
+    kernel_bytes = (E * (cs["nen"] + cs["nen_p"]) * 4 + Nn * 16 + E * 8 * ncoef + nnz * 8
+                    + (n * 8 if cs["heat"] else 0))
 
     out = {}
     for key, r in res.items():
         ph = r["phases"]
         steps = max(1, ph["total"][1])
         ms = ph["total"][0] / steps
+
+        def per(name):
+            return (ph[name][0] / ph[name][1]) if ph[name][1] else None
         out[key] = dict(ms_per_step=ms, wall_ms_per_step=r["wall_ms"], elements_per_s=E / (ms * 1e-3),
-                        numeric_ms=(ph["numeric"][0] / ph["numeric"][1]) if ph["numeric"][1] else None,
-                        gather_ms=(ph["gather"][0] / ph["gather"][1]) if ph["gather"][1] else None,
-                        element_ms=(ph["element"][0] / ph["element"][1]) if ph["element"][1] else None,
-                        sort_ms=(ph["sort"][0] / ph["sort"][1]) if ph["sort"][1] else None,
-                        reduce_ms=(ph["reduce"][0] / ph["reduce"][1]) if ph["reduce"][1] else None,
-                        survey_gbs=survey_bytes / (ms * 1e-3) / 1e9)
+                        numeric_ms=per("numeric"), gather_ms=per("gather"), element_ms=per("element"),
+                        sort_ms=per("sort"), reduce_ms=per("reduce"),
+                        full_assembly_gbs=full_bytes / (ms * 1e-3) / 1e9)
     errs = {}
     if "fp64" in res and "mixed" in res:
-        assert np.array_equal(res["fp64"]["ptr"], res["mixed"]["ptr"]) and np.array_equal(res["fp64"]["ind"],
-                                                                                           res["mixed"]["ind"])
-        errs["mixed_vs_fp64_frobenius"] = rel_frob(res["mixed"]["val"], res["fp64"]["val"])
-        errs["mixed_vs_fp64_max_entry_over_max"] = float(np.abs(res["mixed"]["val"] - res["fp64"]["val"]).max() /
-                                                         np.abs(res["fp64"]["val"]).max())
+        a, b = res["mixed"], res["fp64"]
+        assert np.array_equal(a["ptr"], b["ptr"]) and np.array_equal(a["ind"], b["ind"])
+        errs["mixed_vs_fp64_frobenius"] = rel_frob(a["val"], b["val"])
+        errs["mixed_vs_fp64_max_entry_over_max"] = float(np.abs(a["val"] - b["val"]).max() / np.abs(b["val"]).max())
+        if cs["heat"]:
+            errs["mixed_vs_fp64_load_vector"] = rel_frob(a["F"], b["F"])
     if "fp64" in res and "fp64_reuse" in res:
         errs["fp64_reuse_equals_fp64_plain"] = bool(np.array_equal(res["fp64"]["val"], res["fp64_reuse"]["val"]))
 
     m = out.get("mixed") or next(iter(out.values()))
     kern_ms = m["numeric_ms"] or m["ms_per_step"]
-    achieved = kernel_bytes_mixed / (kern_ms * 1e-3) / 1e9
+    achieved = kernel_bytes / (kern_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_reuse<ELAST, float, T6, CSC>", "kernel_ms": kern_ms,
-                "algorithmic_bytes": kernel_bytes_mixed}
+                "kernel": cs["kernel"], "kernel_ms": kern_ms, "algorithmic_bytes": kernel_bytes}
+
+    extra = {}
+    if args.config == "c5" and not args.no_pcg:
+        extra["pcg"] = bench_pcg(ctx, L, cs, res)
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.cpu_sample_N)
+        cpu = cpu_baseline(args.config, cs)
 
     line = {
-        "metric": "elements assembled/s + HBM GB/s: FP32-mixed vs FP64 GPU path at <=1e-6 rel err",
+        "metric": METRIC,
         "value": m["elements_per_s"],
         "unit": "elements/s",
         "n_gpus": 1,
@@ -161,22 +218,152 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 element arithmetic, f64 sums",
-        "data": "synthetic (seeded SplitMix64 jitter), inputs resident in HBM",
-        "config": {"workload": f"plane-strain elasticity T6 {args.N}x{args.N}, 3-pt rule, +-15% h node jitter",
-                   "elements": E, "nodes": Nn, "dofs": n, "nnz": nnz, "layout": "CSC (finish! layout)",
+        "dtype": "f64 geometry, f32 element products, f64 sums",
+        "data": "synthetic (seeded SplitMix64), inputs resident in HBM",
+        "config": {"workload": cs["workload"], "elements": E, "nodes": Nn, "dofs": n, "nnz": nnz,
+                   "layout": "CSR" if cs["layout"] == L.CSR else "CSC",
                    "method": "REUSE (pattern cached, fused owner-computes kernel)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "paths": out,
         "speedup_mixed_vs_fp64_plain": (out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"])
         if ("fp64" in out and "mixed" in out) else None,
-        "step_gbs_mixed": step_bytes_reuse / (m["ms_per_step"] * 1e-3) / 1e9,
+        "full_assembly_bytes": full_bytes,
         "errors": errs,
-        "geometry": {"min_det": mindet, "fp64_branch_elements": nfallback},
-        "setup": {"host_mesh_and_numbering_s": t_host, "upload_s": t_upload, "symbolic_ms": t_sym},
+        "geometry": {"min_det": mindet, "elements_over_quality_threshold": nbad},
+        "setup": {"host_mesh_and_upload_s": t_setup, "symbolic_ms": t_sym},
         "device": {"name": devname, "cus": ncu},
     }
+    line.update(extra)
+    print(json.dumps(line))
+
+
+def bench_pcg(ctx, L, cs, res):
+    """C5: Jacobi-PCG on the free block, FP32 inner iterations with FP64
+    refinement against FP64 PCG, both to ||b - A x|| / ||b|| <= 1e-10."""
+    cfg = cs["cfg"]
+    nu, n = cfg["nfree"], cfg["ndofs"]
+    # the matrix in the context must be the FP64 one
+    ctx.assemble(cs["kind"], cs["fe"], cs["rule"], precision=L.FP64, method=L.REUSE, layout=cs["layout"],
+                 **cs["kw"])
+    F = ctx.copy_vector(n)
+    x0 = np.zeros(n)
+    outp = {}
+    nnz_free = None
+    for name, prec in (("mixed", L.MIXED), ("fp64", L.FP64)):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        x, it, outer, rr = ctx.pcg(nu, F, x0, precision=prec, tol=1e-10, maxit=200000)
+        dt = time.perf_counter() - t0
+        outp[name] = dict(seconds=dt, iterations=it, refinements=outer, relres=rr, x=x)
+    ptr = res["fp64"]["ptr"] if "fp64" in res else None
+    if ptr is not None:
+        nnz_free = int(ptr[nu])  # upper bound: rows < nu, all columns
+    d = outp["mixed"]["x"][:nu] - outp["fp64"]["x"][:nu]
+    sol_diff = float(np.linalg.norm(d) / max(np.linalg.norm(outp["fp64"]["x"][:nu]), 1e-300))
+    for v in outp.values():
+        v.pop("x")
+    return {"unknowns": nu, "tol": 1e-10, "paths": outp, "mixed_vs_fp64_solution": sol_diff,
+            "speedup_mixed_vs_fp64": outp["fp64"]["seconds"] / outp["mixed"]["seconds"],
+            "nnz_rows_free": nnz_free}
+
+
+# ---------------------------------------------------------------------------
+# C4: batched uneven problems
+# ---------------------------------------------------------------------------
+def bench_batch(args, L, W):
+    ctx = L.Context(0)
+    devname, ncu = ctx.device_name()
+    specs = W.batch_specs()
+    t0 = time.time()
+    probs = [W.batch_problem(k, kind, nn) for k, (kind, nn) in enumerate(specs)]
+    t_host = time.time() - t0
+    ctx.batch_begin()
+    E = Nn = conn_b = coef_b = F_b = 0
+    for p in probs:
+        f = p["fesp"]
+        if p["kind"] == "heat":
+            ctx.batch_add(L.HEAT, L.T3, 1, f.mesh.conn, f.mesh.xy, f.dofnums, f.ndofs(),
+                          coef_e=p["kappa_e"], kappa=1.0, Q=p["Q"])
+            F_b += f.ndofs() * 8
+        else:
+            ctx.batch_add(L.ELAST, L.T6, 3, f.mesh.conn, f.mesh.xy, f.dofnums, f.ndofs(),
+                          coef_e=p["scale_e"], D=p["D"])
+        E += f.mesh.nelements
+        Nn += f.mesh.nnodes
+        conn_b += f.mesh.conn.size * 4
+        coef_b += f.mesh.nelements * 8
+    t0 = time.perf_counter()
+    ctx.batch_build(autotune=True)
+    t_build = (time.perf_counter() - t0) * 1e3
+    st = ctx.batch_stats()
+    ctx.batch_assemble()
+    for _ in range(args.warmup):
+        ctx.batch_assemble(wait=False)
+    ctx.synchronize()
+    ctx.timing(True)
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.batch_assemble(wait=False)
+    ctx.synchronize()
+    wall = (time.perf_counter() - w0) * 1e3 / args.steps
+    ph = ctx.timing_read()
+    ctx.timing(False)
+    ms = ph["total"][0] / max(1, ph["total"][1])
+    info0 = ctx.batch_info(0)
+    kernel_bytes = conn_b + Nn * 16 + coef_b + st["nnz"] * 8 + F_b
+    # errors of a sample of problems against the FP64 plain method
+    errs = []
+    fp64_ms = 0.0
+    sample = list(range(0, len(probs), max(1, len(probs) // 16)))
+    ctx2 = L.Context(0)
+    for k in sample:
+        p = probs[k]
+        f = p["fesp"]
+        ctx2.set_mesh(0, f.mesh.conn, f.mesh.xy)
+        ctx2.set_dofs(0, f.dofnums, f.ndofs())
+        if p["kind"] == "heat":
+            ctx2.set_coefficients(f.mesh.nelements, p["kappa_e"], None, None)
+            a = dict(kind=L.HEAT, fe=L.T3, rule=1, kw=dict(kappa=1.0, Q=p["Q"]))
+        else:
+            ctx2.set_coefficients(f.mesh.nelements, None, None, p["scale_e"])
+            a = dict(kind=L.ELAST, fe=L.T6, rule=3, kw=dict(D=p["D"]))
+        ctx2.assemble(a["kind"], a["fe"], a["rule"], precision=L.FP64, method=L.SORT, layout=L.CSC, **a["kw"])
+        ctx2.synchronize()
+        ctx2.timing(True)
+        ctx2.assemble(a["kind"], a["fe"], a["rule"], precision=L.FP64, method=L.SORT, layout=L.CSC, **a["kw"])
+        ctx2.synchronize()
+        fp64_ms += ctx2.timing_read()["total"][0]
+        ctx2.timing(False)
+        ref = ctx2.copy_sparse(base=0)
+        got = ctx.batch_copy(k)
+        assert np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1])
+        errs.append(rel_frob(got[2], ref[2]))
+    E_sample = sum(probs[k]["fesp"].mesh.nelements for k in sample)
+    fp64_eps = E_sample / (fp64_ms * 1e-3)
+    line = {
+        "metric": METRIC, "value": E / (ms * 1e-3), "unit": "elements/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 geometry, f32 element products, f64 sums",
+        "data": "synthetic (seeded SplitMix64), inputs resident in HBM",
+        "config": {"workload": "256 uneven heat-T3 (1-pt) / elasticity-T6 (3-pt) problems, E_k log-uniform "
+                               "[1e3, 2e5], lognormal(0, 0.5) coefficients, +-20% h jitter, one persistent "
+                               "launch replayed from a hipGraph",
+                   "problems": len(probs), "elements": E, "nnz": st["nnz"], "tiles": st["ntiles"],
+                   "block": info0["block"], "grid": info0["grid"]},
+        "roofline": {"bound": "hbm", "achieved": round(kernel_bytes / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(kernel_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "graph: gathers + k_reuse_batch", "algorithmic_bytes": kernel_bytes},
+        "cpu_baseline": None,
+        "wall_ms_per_step": wall,
+        "fp64_plain_elements_per_s_sample": fp64_eps,
+        "speedup_mixed_vs_fp64_plain": (E / (ms * 1e-3)) / fp64_eps,
+        "errors": {"max_frobenius_vs_fp64_sample": max(errs), "problems_checked": len(sample)},
+        "setup": {"host_meshes_s": t_host, "build_and_autotune_ms": t_build, "tuned_ms": st["tuned_ms"]},
+        "device": {"name": devname, "cus": ncu},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline_batch(probs, sample)
     print(json.dumps(line))
 
 
@@ -193,23 +380,90 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes_each_way": n * 4, "launches": 3}))
 
 
-def cpu_baseline(N):
-    """The oracle (C, one thread) on the same workload: element loop + sparse()."""
+# ---------------------------------------------------------------------------
+# CPU baseline: the oracle (C, one thread), element loop + sparse()
+# ---------------------------------------------------------------------------
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    return O
+
+
+def _oracle_problem(O, o):
+    f = o["fesp"]
+    fe = {"T3": O.T3, "T6": O.T6, "Q4": O.Q4}[f.fe.name]
+    if o["kind"] == "heat":
+        rule = 2 if f.fe.name == "Q4" else 1
+        return O.make_problem(O.HEAT, fe, rule, f.mesh.conn, f.mesh.xy, f.dofnums, kappa=1.0,
+                              Q=o.get("Q", -6.0), kappa_e=o.get("kappa_e"), Q_e=o.get("Q_e"))
+    if o["kind"] == "elast":
+        return O.make_problem(O.ELAST, fe, 3, f.mesh.conn, f.mesh.xy, f.dofnums, D=o["D"],
+                              scale_e=o.get("scale_e"))
+    p = o["pfesp"]
+    return O.make_problem(O.STOKES_GEN, fe, 3, f.mesh.conn, f.mesh.xy, f.dofnums, conn_p=p.mesh.conn,
+                          dofnums_p=p.dofnums, D=o["D"], mu=1.0)
+
+
+def cpu_baseline(cfgname, cs):
     try:
-        import oracle as O
+        O = _oracle()
     except Exception as e:  # the checker is optional for the GPU numbers
         return {"error": f"oracle unavailable: {e}"}
     from elfel_hip import workloads as W
-    cfg = W.elast_t6(N, frac=0.15)
-    fesp = cfg["fesp"]
-    p = O.make_problem(O.ELAST, O.T6, 3, fesp.mesh.conn, fesp.mesh.xy, fesp.dofnums, D=cfg["D"])
+    sample_cs = cs
+    sample = None
+    if cfgname == "c5" and cs["N"] > 1024:
+        cfg = W.heat_q4(1024, sigma=1.0)
+        f = cfg["fesp"]
+        sample_cs = dict(oracle=dict(kind="heat", fesp=f, kappa_e=cfg["kappa_e"], Q_e=cfg["Q_e"]),
+                         E=f.mesh.nelements)
+        sample = "heat Q4 1024x1024 (1,048,576 elements) of the same construction"
+    if cfgname == "c3" and cs["N"] > 250:
+        cfg = W.stokes_t6t3(250)
+        sample_cs = dict(oracle=dict(kind="stokes", fesp=cfg["ufesp"], pfesp=cfg["pfesp"], D=cfg["D"]),
+                         E=cfg["ufesp"].mesh.nelements)
+        sample = "Stokes T6/T3 250x250 (125,000 elements) of the same construction"
+    p = _oracle_problem(O, sample_cs["oracle"])
     te, ts, nnz = O.time_assembly(p, reps=1, nthreads=1)
-    E = fesp.mesh.nelements
+    E = sample_cs["E"]
     return {"value": E / (te + ts), "unit": "elements/s", "cores": 1, "kind": "port",
-            "sample": f"full {N}x{N} T6 mesh ({E} elements), 1 repetition: element loop {te:.3f} s + "
+            "sample": (sample or f"the full mesh ({E} elements)") + f", 1 repetition: element loop {te:.3f} s + "
                       f"sparse() {ts:.3f} s, single thread (the reference's path is serial)",
             "element_loop_s": te, "sparse_s": ts, "host_cores_visible": os.cpu_count()}
+
+
+def cpu_baseline_batch(probs, sample):
+    try:
+        O = _oracle()
+    except Exception as e:
+        return {"error": f"oracle unavailable: {e}"}
+    te = ts = 0.0
+    E = 0
+    for k in sample:
+        p = probs[k]
+        o = dict(kind=p["kind"], fesp=p["fesp"], kappa_e=p.get("kappa_e"), Q=p.get("Q", 0.0),
+                 D=p.get("D"), scale_e=p.get("scale_e"))
+        a, b, _ = O.time_assembly(_oracle_problem(O, o), reps=1, nthreads=1)
+        te, ts, E = te + a, ts + b, E + p["fesp"].mesh.nelements
+    return {"value": E / (te + ts), "unit": "elements/s", "cores": 1, "kind": "port",
+            "sample": f"{len(sample)} of the 256 problems ({E} elements), element loop {te:.3f} s + sparse() "
+                      f"{ts:.3f} s, single thread"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU per box; --gpus must be 1"}))
+        sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
+    from elfel_hip import _lib as L
+    from elfel_hip import workloads as W
+    if args.config == "c4":
+        bench_batch(args, L, W)
+    else:
+        bench_single(args, L, W)
 
 
 if __name__ == "__main__":

@@ -10,7 +10,7 @@
 int plain_coo_keys(elfel_hip_ctx *c, const int64_t *I, const int64_t *J, int64_t n, int64_t nrow, int64_t ncol,
                    int minor_bits, int csc, uint64_t *keys);
 int plain_vec_keys(elfel_hip_ctx *c, const int64_t *I, int64_t n, int64_t ndofs, uint64_t *keys);
-int geometry_stats_run(elfel_hip_ctx *c, double *min_det, int64_t *nfallback);
+int geometry_stats_run(elfel_hip_ctx *c, double *min_det, int64_t *nbad);
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -483,10 +483,10 @@ extern "C" int elfel_hip_copy_vector(elfel_hip_ctx *c, double *F, int64_t n)
     return ELFEL_OK;
 }
 
-extern "C" int elfel_hip_geometry_stats(elfel_hip_ctx *c, double *min_det, int64_t *nfallback)
+extern "C" int elfel_hip_geometry_stats(elfel_hip_ctx *c, double *min_det, int64_t *nbad)
 {
     CHECK_CTX(c);
-    return geometry_stats_run(c, min_det, nfallback);
+    return geometry_stats_run(c, min_det, nbad);
 }
 
 extern "C" int elfel_hip_pcg(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int precision, double tol,

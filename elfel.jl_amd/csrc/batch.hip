@@ -171,7 +171,6 @@ static ReuseArgs kind_args(KindData &k, int bi, double *vals, double *F)
     A.xy = (const double2 *)k.xy.p;
     A.vals = vals;
     A.F = F;
-    A.fallback = nullptr;
     return A;
 }
 

@@ -70,6 +70,7 @@ struct Pattern {
     int ncolmax = 0;             // bytes of pos per task
     int maxrank = 0;
     int lds_vals = 0;            // largest tile value count
+    int block = 0;               // threads per workgroup = task capacity of a tile
     double symbolic_ms = 0;      // host wall time of the last symbolic build
     Buf ptr, ind;                // CSR pattern (int32)
     Buf tile;                    // int4 per tile: task0, ntasks, v0, nvals | (row0 in .w)
@@ -109,7 +110,7 @@ struct elfel_hip_ctx {
     bool has_result = false, has_F = false;
     bool result_is_pattern = false;   // ptr/ind live in the cached pattern
     Pattern pat;
-    // device scalars: [0] nnz (int32), [1] error flags, [2..3] min det (double bits), [4..5] fallback count
+    // device scalars: [0] nnz (int32), [1] error flags, [3] bad-shape count (int64, at byte 24)
     Buf dscal;
     Timing tm;
     // solver scratch

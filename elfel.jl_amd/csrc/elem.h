@@ -10,8 +10,10 @@
 //            test/test_stokes.jl:645-652 (vector Laplacian)
 // Each matrix entry is accumulated over quadrature points in point order
 // starting from 0.0, which is the order of `ke[i, j] += ...` in the loops.
-// R = float is the mixed form: coordinates relative to the element's first
-// node (differences taken in FP64 by the gather), one reciprocal of det.
+// The mixed kernels take their geometry from geometry_mixed(): Jacobian,
+// determinant and one reciprocal of det per point in FP64 from coordinates
+// relative to the element's first node, spatial gradients and JxW rounded to
+// FP32 for the element products.
 #pragma once
 #include "common.h"
 
@@ -67,6 +69,38 @@ __device__ __forceinline__ void geometry(const QTable &t, const R *X, const R *Y
             }
         }
         g.jxw[q] = det * tab_w<R>(t, q);
+    }
+}
+
+// Mixed-precision geometry.  An FP32 Jacobian loses up to ~1e-5 of the matrix
+// to the cancellation in sum_k x_k grad N_k on curved T6 elements, so J, det
+// and 1/det stay in FP64; only the results are rounded.
+template <int NEN, int NQ>
+__device__ __forceinline__ void geometry_mixed(const QTable &t, const double *X, const double *Y,
+                                               Geo<float, NEN, NQ> &g)
+{
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        double j11 = X[0] * t.G[q][0][0], j21 = Y[0] * t.G[q][0][0];
+        double j12 = X[0] * t.G[q][0][1], j22 = Y[0] * t.G[q][0][1];
+#pragma unroll
+        for (int k = 1; k < NEN; k++) {
+            j11 = fma(X[k], t.G[q][k][0], j11);
+            j21 = fma(Y[k], t.G[q][k][0], j21);
+            j12 = fma(X[k], t.G[q][k][1], j12);
+            j22 = fma(Y[k], t.G[q][k][1], j22);
+        }
+        const double det = j11 * j22 - j21 * j12;
+        const double inv = 1.0 / det;
+        const double i11 = j22 * inv, i12 = -j21 * inv, i21 = -j12 * inv, i22 = j11 * inv;  // J^-T
+        g.det[q] = (float)det;
+        g.jxw[q] = (float)(det * t.w[q]);
+#pragma unroll
+        for (int k = 0; k < NEN; k++) {
+            const double a = t.G[q][k][0], b = t.G[q][k][1];
+            g.gx[q][k] = (float)fma(a, i11, b * i12);
+            g.gy[q][k] = (float)fma(b, i22, a * i21);
+        }
     }
 }
 

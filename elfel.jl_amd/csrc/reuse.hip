@@ -12,6 +12,7 @@
 //   4. per task, the position of every element column in the owner's sorted
 //      column list (binary search).
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -44,6 +45,14 @@ void adjacency(const Field &f, Adj &A)
             A.loc[p] = (uint8_t)k;
         }
 }
+
+int env_int(const char *name, int dflt)
+{
+    const char *v = getenv(name);
+    return (v && *v) ? atoi(v) : dflt;
+}
+
+bool rank_order_enabled() { return env_int("ELFEL_HIP_RANK_ORDER", 1) != 0; }
 
 }  // namespace
 
@@ -100,8 +109,30 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     T.pos.reserve(T.elem.capacity() * ncolp);
     int4 cur = {0, 0, 0, 0}, cr = {0, 0, 0, 0};
     bool open = false;
+    // Within a tile the tasks are laid out rank by rank (stable), so the
+    // threads that add in one rank step sit in as few waves as possible and
+    // every other wave skips that step.
+    const bool by_rank = rank_order_enabled();
+    std::vector<int32_t> te;
+    std::vector<uint32_t> ti;
+    std::vector<uint8_t> tp;
     auto close = [&]() {
         if (!open) return;
+        if (by_rank && cur.y > 1) {
+            const size_t t0 = (size_t)cur.x, nt = (size_t)cur.y;
+            te.assign(T.elem.begin() + t0, T.elem.end());
+            ti.assign(T.info.begin() + t0, T.info.end());
+            tp.assign(T.pos.begin() + t0 * ncolp, T.pos.end());
+            size_t o = t0;
+            for (int s = 0; s < cr.z; s++)
+                for (size_t k = 0; k < nt; k++)
+                    if ((int)((ti[k] >> 3) & 15) == s) {
+                        T.elem[o] = te[k];
+                        T.info[o] = ti[k];
+                        std::copy(tp.begin() + k * ncolp, tp.begin() + (k + 1) * ncolp, T.pos.begin() + o * ncolp);
+                        o++;
+                    }
+        }
         T.tiles.push_back(cur);
         T.trows.push_back(cr);
         T.maxv = std::max(T.maxv, cur.w);
@@ -171,6 +202,34 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     return 0;
 }
 
+// Tile order: tiles are independent (each owns its value range), so their
+// launch order is free.  Sorting by the first element a tile touches puts the
+// tiles that read the same element records close together; laying the sorted
+// list out so that workgroup b (dispatched to XCD b mod 8) walks a contiguous
+// chunk keeps those re-reads inside one XCD's L2.
+static void order_tiles(ReuseTables &T, int nxcd)
+{
+    const size_t nt = T.tiles.size();
+    if (nt < 2) return;
+    std::vector<std::pair<int32_t, int32_t>> key(nt);
+    for (size_t t = 0; t < nt; t++) {
+        int32_t m = INT32_MAX;
+        for (int k = 0; k < T.tiles[t].y; k++) m = std::min(m, T.elem[T.tiles[t].x + k]);
+        key[t] = {m, (int32_t)t};
+    }
+    std::stable_sort(key.begin(), key.end());
+    std::vector<int64_t> off(nxcd + 1, 0);
+    for (int x = 0; x < nxcd; x++) off[x + 1] = off[x] + (int64_t)(nt / nxcd + ((size_t)x < nt % nxcd ? 1 : 0));
+    std::vector<int4> tiles(nt), trows(nt);
+    for (size_t b = 0; b < nt; b++) {
+        const size_t pos = off[b % nxcd] + b / nxcd;
+        tiles[b] = T.tiles[key[pos].second];
+        trows[b] = T.trows[key[pos].second];
+    }
+    T.tiles.swap(tiles);
+    T.trows.swap(trows);
+}
+
 int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P)
 {
     const int64_t ntasks = (int64_t)T.elem.size();
@@ -198,7 +257,8 @@ int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P)
     return 0;
 }
 
-static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf)
+static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf,
+                          int block)
 {
     auto t0 = std::chrono::steady_clock::now();
     Pattern &P = c->pat;
@@ -215,8 +275,10 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
     HIPCHK(c, hipMemcpyAsync(P.ptr.p, c->ptr.p, 4 * (n + 1), hipMemcpyDeviceToDevice, c->stream));
     if (nnz) HIPCHK(c, hipMemcpyAsync(P.ind.p, c->ind.p, 4 * nnz, hipMemcpyDeviceToDevice, c->stream));
     ReuseTables T;
-    if ((rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, REUSE_BLOCK, T))) return rc;
+    if ((rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, block, T))) return rc;
+    if (env_int("ELFEL_HIP_TILE_ORDER", 1)) order_tiles(T, 8);
     if ((rc = upload_tables(c, T, P))) return rc;
+    P.block = block;
     P.kind = pb->kind; P.fe = pb->fe; P.layout = pb->layout;
     P.key_mesh0 = F0.mesh_version; P.key_dof0 = F0.dof_version;
     P.key_mesh1 = stokes ? F1->mesh_version : 0; P.key_dof1 = stokes ? F1->dof_version : 0;
@@ -236,15 +298,16 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
                        P.key_mesh0 == F0.mesh_version && P.key_dof0 == F0.dof_version &&
                        (!stokes || (P.key_mesh1 == c->f[1].mesh_version && P.key_dof1 == c->f[1].dof_version)) &&
                        P.nrow == c->nrow;
+    const int block = env_int("ELFEL_HIP_REUSE_BLOCK", 256) == 128 ? 128 : 256;
     const bool mixed = pb->precision == ELFEL_MIXED;
-    reuse_launch_fn fn = mixed ? pick_reuse32(pb->kind, pb->fe, tab.nq, pb->layout == ELFEL_CSC)
-                               : pick_reuse64(pb->kind, pb->fe, tab.nq, pb->layout == ELFEL_CSC);
+    reuse_launch_fn fn = mixed ? pick_reuse32(pb->kind, pb->fe, tab.nq, pb->layout == ELFEL_CSC, block)
+                               : pick_reuse64(pb->kind, pb->fe, tab.nq, pb->layout == ELFEL_CSC, block);
     if (!fn) return set_err(c, ELFEL_ENOTSUP, "unsupported (kind, element, rule) for the reuse method");
     int rc;
-    if (!fresh) {
+    if (!fresh || P.block != block) {
         hipEvent_t ev = nullptr;
         tm_begin(c, ELFEL_PH_SYMBOLIC, &ev);
-        rc = build_symbolic(c, pb, tab, cf);
+        rc = build_symbolic(c, pb, tab, cf, block);
         tm_end(c, ELFEL_PH_SYMBOLIC, ev);
         if (rc) return rc;
     }
@@ -256,13 +319,16 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
     const bool heat = pb->kind == ELFEL_HEAT;
     if (heat && (rc = ensure(c, c->F, 8 * std::max<int64_t>(c->nrow, 1)))) return rc;
     hipEvent_t ev = nullptr;
-    tm_begin(c, ELFEL_PH_GATHER, &ev);
-    if (mixed)
-        rc = gather_geom_rel(c, nen, E, (const int32_t *)F0.conn.p, (const double2 *)F0.xy.p, (double *)c->geom.p);
-    else
-        rc = gather_geom64(c, nen, E, (const int32_t *)F0.conn.p, (const double2 *)F0.xy.p, (double *)c->geom.p);
-    if (rc) return rc;
-    tm_end(c, ELFEL_PH_GATHER, ev);
+    const bool fused = env_int("ELFEL_HIP_FUSED_GATHER", 1) != 0;
+    if (!fused) {
+        tm_begin(c, ELFEL_PH_GATHER, &ev);
+        if (mixed)
+            rc = gather_geom_rel(c, nen, E, (const int32_t *)F0.conn.p, (const double2 *)F0.xy.p, (double *)c->geom.p);
+        else
+            rc = gather_geom64(c, nen, E, (const int32_t *)F0.conn.p, (const double2 *)F0.xy.p, (double *)c->geom.p);
+        if (rc) return rc;
+        tm_end(c, ELFEL_PH_GATHER, ev);
+    }
     ReuseArgs A;
     A.tiles = (const int4 *)P.tile.p;
     A.tile_rows = (const int4 *)P.tile_rows.p;
@@ -271,12 +337,11 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
     A.task_pos = (const uint8_t *)P.task_pos.p;
     A.ncolp = P.ncolmax;
     A.ntiles = (int)P.ntiles;
-    A.geom = c->geom.p;
+    A.geom = fused ? nullptr : c->geom.p;
     A.conn = (const int32_t *)F0.conn.p;
     A.xy = (const double2 *)F0.xy.p;
     A.vals = (double *)c->val.p;
     A.F = heat ? (double *)c->F.p : nullptr;
-    A.fallback = nullptr;
     tm_begin(c, ELFEL_PH_NUMERIC, &ev);
     rc = fn(c, A, cf, tab, P.lds_vals);
     if (rc) return rc;
@@ -292,7 +357,8 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
 
 // ---------------------------------------------------------------------------
 // Geometry statistics: smallest det J over all elements and quadrature points
-// (FP64) and the number of elements the mixed kernels send to FP64.
+// and the number of badly shaped elements (det J <= 0, or
+// sum_k |grad N_k|^2 det J >= REUSE_COND_MAX at some point), all in FP64.
 // ---------------------------------------------------------------------------
 template <int NEN, int NQ>
 __global__ __launch_bounds__(256) void k_geom_stats(int64_t E, const int32_t *__restrict__ conn,
@@ -305,26 +371,21 @@ __global__ __launch_bounds__(256) void k_geom_stats(int64_t E, const int32_t *__
     double mn = 1e300;
     if (e < E) {
         double X[NEN], Y[NEN];
-        float Xf[NEN], Yf[NEN];
-        double2 p0 = xy[conn[e * NEN]];
 #pragma unroll
         for (int k = 0; k < NEN; k++) {
             const double2 p = xy[conn[e * NEN + k]];
             X[k] = p.x; Y[k] = p.y;
-            Xf[k] = (float)(p.x - p0.x); Yf[k] = (float)(p.y - p0.y);
         }
         Geo<double, NEN, NQ> g;
         geometry<double, NEN, NQ>(tab, X, Y, g);
-        Geo<float, NEN, NQ> gf;
-        geometry<float, NEN, NQ>(tab, Xf, Yf, gf);
         bool good = true;
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             mn = fmin(mn, g.det[q]);
-            float j2 = 0.f;
+            double j2 = 0.0;
 #pragma unroll
-            for (int k = 0; k < NEN; k++) j2 += gf.gx[q][k] * gf.gx[q][k] + gf.gy[q][k] * gf.gy[q][k];
-            good = good && (gf.det[q] > 0.f) && (j2 * gf.det[q] < REUSE_COND_MAX);
+            for (int k = 0; k < NEN; k++) j2 += g.gx[q][k] * g.gx[q][k] + g.gy[q][k] * g.gy[q][k];
+            good = good && (g.det[q] > 0.0) && (j2 * g.det[q] < REUSE_COND_MAX);
         }
         if (!good) atomicAdd(nbad, 1ull);
     }
@@ -337,7 +398,7 @@ __global__ __launch_bounds__(256) void k_geom_stats(int64_t E, const int32_t *__
     if (threadIdx.x == 0) block_min[blockIdx.x] = smin[0];
 }
 
-int geometry_stats_run(elfel_hip_ctx *c, double *min_det, int64_t *nfallback)
+int geometry_stats_run(elfel_hip_ctx *c, double *min_det, int64_t *nbad)
 {
     const Field &F = c->f[0];
     if (!F.has_mesh) return set_err(c, ELFEL_ESTATE, "no mesh");
@@ -369,6 +430,6 @@ int geometry_stats_run(elfel_hip_ctx *c, double *min_det, int64_t *nfallback)
     double m = 1e300;
     for (double v : h) m = std::min(m, v);
     if (min_det) *min_det = m;
-    if (nfallback) *nfallback = (int64_t)hc;
+    if (nbad) *nbad = (int64_t)hc;
     return 0;
 }

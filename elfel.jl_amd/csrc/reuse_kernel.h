@@ -26,7 +26,7 @@
 #define REUSE_BLOCK 256
 #define REUSE_VCAP_PER_THREAD 24   // LDS doubles for tile values per thread of the block (48 KiB at 256)
 #define REUSE_VCAP (REUSE_VCAP_PER_THREAD * REUSE_BLOCK)
-#define REUSE_COND_MAX 1024.0f // sum_k |grad N_k|^2 * det J: above it an element counts as badly shaped
+#define REUSE_COND_MAX 1024.0  // sum_k |grad N_k|^2 * det J: above it an element counts as badly shaped
 
 struct ReuseArgs {
     const int4 *tiles;        // task0, ntasks, v0, nvals
@@ -36,12 +36,11 @@ struct ReuseArgs {
     const uint8_t *task_pos;  // [ntasks][ncolp]
     int ncolp;
     int ntiles;
-    const void *geom;         // per element [NEN][2] of R
-    const int32_t *conn;      // for the FP64 branch of the mixed kernel
+    const void *geom;         // per element [NEN][2] double from a gather kernel, or null:
+    const int32_t *conn;      //   then the kernel reads conn and xy itself
     const double2 *xy;
     double *vals;
     double *F;
-    int64_t *fallback;        // count of FP64-branch tasks (mixed)
 };
 
 // The owner's entries of one element block.  The owner node a and its
@@ -82,6 +81,83 @@ __device__ __forceinline__ void owner_values(const QTable &tab, const Geo<R, NEN
                     acc += (-g.jxw[q] * onp[q]) * gi;
                 }
                 out[0][i] = acc;
+            }
+            return;
+        }
+    }
+    if constexpr (sizeof(R) == 4) {
+        // Mixed form: the owner's factor (D B_owner for a column owner, D^T
+        // B_owner for a row owner, or the gradient) is scaled by the
+        // coefficient and JxW once per point, and the structural zeros of B
+        // are skipped: two products per entry and point.
+        if constexpr (KIND == ELFEL_HEAT) {
+            R wx[NQ], wy[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; q++) { wx[q] = ox[q] * (kap * g.jxw[q]); wy[q] = oy[q] * (kap * g.jxw[q]); }
+#pragma unroll
+            for (int j = 0; j < ND; j++) {
+                R acc = R(0);
+#pragma unroll
+                for (int q = 0; q < NQ; q++) acc = fmaf(g.gx[q][j], wx[q], fmaf(g.gy[q][j], wy[q], acc));
+                out[0][j] = acc;
+            }
+            R f = R(0);
+#pragma unroll
+            for (int q = 0; q < NQ; q++) f += on[q] * Qe * g.jxw[q];
+            *fo = f;
+            return;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                if (c >= nr) continue;
+                const int oc = comp0 + c;
+                if constexpr (BDB) {
+                    R t[NQ][3];
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) {
+                        const R b0 = oc == 0 ? ox[q] : R(0), b1 = oc == 0 ? R(0) : oy[q], b2 = oc == 0 ? oy[q] : ox[q];
+#pragma unroll
+                        for (int r = 0; r < 3; r++) {
+                            const R v = CSC ? D[r] * b0 + D[r + 3] * b1 + D[r + 6] * b2
+                                            : D[3 * r] * b0 + D[3 * r + 1] * b1 + D[3 * r + 2] * b2;
+                            t[q][r] = v * g.jxw[q];
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < ND; i++) {
+                        R acc = R(0);
+#pragma unroll
+                        for (int q = 0; q < NQ; q++) {
+                            const R gx = g.gx[q][i / NC], gy = g.gy[q][i / NC];
+                            acc = (i % NC) == 0 ? fmaf(t[q][0], gx, fmaf(t[q][2], gy, acc))
+                                                : fmaf(t[q][1], gy, fmaf(t[q][2], gx, acc));
+                        }
+                        out[c][i] = acc;
+                    }
+                } else {  // vector Laplacian
+                    R wx[NQ], wy[NQ];
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) { wx[q] = ox[q] * (mu * g.jxw[q]); wy[q] = oy[q] * (mu * g.jxw[q]); }
+#pragma unroll
+                    for (int j = 0; j < ND; j++) {
+                        R acc = R(0);
+                        if ((j % NC) == oc) {
+#pragma unroll
+                            for (int q = 0; q < NQ; q++)
+                                acc = fmaf(g.gx[q][j / NC], wx[q], fmaf(g.gy[q][j / NC], wy[q], acc));
+                        }
+                        out[c][j] = acc;
+                    }
+                }
+                if constexpr (STOKES) {
+#pragma unroll
+                    for (int jp = 0; jp < 3; jp++) {
+                        R acc = R(0);
+#pragma unroll
+                        for (int q = 0; q < NQ; q++) acc += (-g.jxw[q] * tab_np<R>(tab, q, jp)) * (oc == 0 ? ox[q] : oy[q]);
+                        out[c][ND + jp] = acc;
+                    }
+                }
             }
             return;
         }
@@ -205,27 +281,28 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
         const double Qd = cf.Q_e ? cf.Q_e[e] : cf.Q;
         // Geometry in FP64 from the gathered records (absolute coordinates for the
         // FP64 kernel, coordinates relative to the element's first node for the
-        // mixed kernel).  The mixed kernel rounds the spatial gradients and JxW to
-        // FP32 for the element products: an FP32 Jacobian loses too much to the
-        // cancellation in sum_k x_k grad N_k on curved T6 elements.
-        const double *gp = (const double *)A.geom + (int64_t)e * (2 * NEN);
+        // mixed kernel).
         double X[NEN], Y[NEN];
+        if (A.geom) {
+            const double *gp = (const double *)A.geom + (int64_t)e * (2 * NEN);
 #pragma unroll
-        for (int k = 0; k < NEN; k++) { X[k] = gp[2 * k]; Y[k] = gp[2 * k + 1]; }
-        Geo<R, NEN, NQ> g;
-        if constexpr (MIXED) {
-            Geo<double, NEN, NQ> gd;
-            geometry<double, NEN, NQ>(tab, X, Y, gd);
-#pragma unroll
-            for (int q = 0; q < NQ; q++) {
-                g.jxw[q] = (float)gd.jxw[q];
-                g.det[q] = (float)gd.det[q];
-#pragma unroll
-                for (int k = 0; k < NEN; k++) { g.gx[q][k] = (float)gd.gx[q][k]; g.gy[q][k] = (float)gd.gy[q][k]; }
-            }
+            for (int k = 0; k < NEN; k++) { X[k] = gp[2 * k]; Y[k] = gp[2 * k + 1]; }
         } else {
-            geometry<double, NEN, NQ>(tab, X, Y, g);
+            // gather fused in: the element's node numbers, then its coordinates
+            int32_t nd[NEN];
+#pragma unroll
+            for (int k = 0; k < NEN; k++) nd[k] = A.conn[(int64_t)e * NEN + k];
+#pragma unroll
+            for (int k = 0; k < NEN; k++) { const double2 p = A.xy[nd[k]]; X[k] = p.x; Y[k] = p.y; }
+            if constexpr (MIXED) {
+                const double x0 = X[0], y0 = Y[0];
+#pragma unroll
+                for (int k = 0; k < NEN; k++) { X[k] -= x0; Y[k] -= y0; }
+            }
         }
+        Geo<R, NEN, NQ> g;
+        if constexpr (MIXED) geometry_mixed<NEN, NQ>(tab, X, Y, g);
+        else geometry<double, NEN, NQ>(tab, X, Y, g);
         R Dr[9];
 #pragma unroll
         for (int i = 0; i < 9; i++) Dr[i] = (R)(cf.scale_e ? sc * cf.D[i] : cf.D[i]);
@@ -235,16 +312,22 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
     const int nranks = tr.z;
     for (int s = 0; s < nranks; s++) {
         if (rank == s) {
+            // A task's positions are distinct (distinct dofs of one element), so
+            // all of its partial sums are read before any is written back: one
+            // LDS round trip per rank instead of one per entry.
             const int ncols = (STOKES && pown) ? ND : NCOL;
             const int nr = rowcode == 2 ? 2 : 1;
 #pragma unroll
             for (int c = 0; c < 2; c++) {
-                if (c < nr) {
-                    double *row = acc + vbase + c * L8;
+                if (c >= nr) continue;
+                double *row = acc + vbase + c * L8;
+                double cur[NCOL];
 #pragma unroll
-                    for (int j = 0; j < NCOL; j++)
-                        if (j < ncols) row[(posw[j >> 2] >> (8 * (j & 3))) & 0xFF] += (double)out[c][j];
-                }
+                for (int j = 0; j < NCOL; j++)
+                    if (j < ncols) cur[j] = row[(posw[j >> 2] >> (8 * (j & 3))) & 0xFF];
+#pragma unroll
+                for (int j = 0; j < NCOL; j++)
+                    if (j < ncols) row[(posw[j >> 2] >> (8 * (j & 3))) & 0xFF] = cur[j] + (double)out[c][j];
             }
             if constexpr (KIND == ELFEL_HEAT) accF[L8] += (double)fo;
         }
@@ -256,11 +339,11 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
         for (int v = tid; v < tr.y; v += BLOCK) A.F[tr.x + v] = accF[v];
 }
 
-template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC>
-__global__ __launch_bounds__(REUSE_BLOCK) void k_reuse(ReuseArgs A, Coef cf, QTable tab, int lds_vals)
+template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_reuse(ReuseArgs A, Coef cf, QTable tab, int lds_vals)
 {
     extern __shared__ double lds[];
-    reuse_tile<KIND, R, NEN, NC, NQ, CSC, REUSE_BLOCK>(A, cf, tab, lds, lds_vals, blockIdx.x);
+    reuse_tile<KIND, R, NEN, NC, NQ, CSC, BLOCK>(A, cf, tab, lds, lds_vals, blockIdx.x);
 }
 
 // Batched problems of uneven size in one persistent launch: heat-T3 (1-point)
@@ -286,16 +369,17 @@ template <int NEN, bool RELATIVE>
 __global__ __launch_bounds__(256) void k_gather_geom(int64_t E, const int32_t *__restrict__ conn,
                                                      const double2 *__restrict__ xy, double *__restrict__ geom)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= E) return;
-    double2 p[NEN];
-#pragma unroll
-    for (int k = 0; k < NEN; k++) p[k] = xy[conn[e * NEN + k]];
-    double2 *g = (double2 *)(geom + e * (2 * NEN));
-#pragma unroll
-    for (int k = 0; k < NEN; k++) {
-        if constexpr (RELATIVE) g[k] = make_double2(p[k].x - p[0].x, p[k].y - p[0].y);
-        else g[k] = p[k];
+    // one work-item per (element, node) slot: conn is read and the records are
+    // written with unit stride across the wavefront
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E * NEN) return;
+    const double2 p = xy[conn[i]];
+    double2 *g = (double2 *)geom;
+    if constexpr (RELATIVE) {
+        const double2 p0 = xy[conn[i - i % NEN]];
+        g[i] = make_double2(p.x - p0.x, p.y - p0.y);
+    } else {
+        g[i] = p;
     }
 }
 
@@ -314,38 +398,38 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
 int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P);
 
 typedef int (*reuse_launch_fn)(elfel_hip_ctx *, const ReuseArgs &, const Coef &, const QTable &, int);
-reuse_launch_fn pick_reuse64(int kind, int fe, int nq, int csc);
-reuse_launch_fn pick_reuse32(int kind, int fe, int nq, int csc);
+reuse_launch_fn pick_reuse64(int kind, int fe, int nq, int csc, int block);
+reuse_launch_fn pick_reuse32(int kind, int fe, int nq, int csc, int block);
 int gather_geom64(elfel_hip_ctx *c, int nen, int64_t E, const int32_t *conn, const double2 *xy, double *geom);
 int gather_geom_rel(elfel_hip_ctx *c, int nen, int64_t E, const int32_t *conn, const double2 *xy, double *geom);
 
 // Instantiation helpers for reuse64.hip / reuse32.hip.
-template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC>
+template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC, int BLOCK>
 int launch_reuse(elfel_hip_ctx *c, const ReuseArgs &A, const Coef &cf, const QTable &tab, int lds_vals)
 {
     if (A.ntiles <= 0) return 0;
-    const size_t lds = sizeof(double) * ((size_t)lds_vals + (KIND == ELFEL_HEAT ? REUSE_BLOCK : 0));
-    hipLaunchKernelGGL((k_reuse<KIND, R, NEN, NC, NQ, CSC>), dim3(A.ntiles), dim3(REUSE_BLOCK), lds, c->stream, A, cf,
-                       tab, lds_vals);
+    const size_t lds = sizeof(double) * ((size_t)lds_vals + (KIND == ELFEL_HEAT ? BLOCK : 0));
+    hipLaunchKernelGGL((k_reuse<KIND, R, NEN, NC, NQ, CSC, BLOCK>), dim3(A.ntiles), dim3(BLOCK), lds, c->stream, A,
+                       cf, tab, lds_vals);
     HIPCHK(c, hipGetLastError());
     return 0;
 }
 
-#define REUSE_PICK_BODY(R)                                                                           \
+#define REUSE_PICK_BODY(R, B)                                                                           \
     if (kind == ELFEL_HEAT) {                                                                        \
-        if (fe == ELFEL_T3 && nq == 1) return csc ? launch_reuse<ELFEL_HEAT, R, 3, 1, 1, true> : launch_reuse<ELFEL_HEAT, R, 3, 1, 1, false>; \
-        if (fe == ELFEL_T3 && nq == 3) return csc ? launch_reuse<ELFEL_HEAT, R, 3, 1, 3, true> : launch_reuse<ELFEL_HEAT, R, 3, 1, 3, false>; \
-        if (fe == ELFEL_T6 && nq == 3) return csc ? launch_reuse<ELFEL_HEAT, R, 6, 1, 3, true> : launch_reuse<ELFEL_HEAT, R, 6, 1, 3, false>; \
-        if (fe == ELFEL_Q4 && nq == 1) return csc ? launch_reuse<ELFEL_HEAT, R, 4, 1, 1, true> : launch_reuse<ELFEL_HEAT, R, 4, 1, 1, false>; \
-        if (fe == ELFEL_Q4 && nq == 4) return csc ? launch_reuse<ELFEL_HEAT, R, 4, 1, 4, true> : launch_reuse<ELFEL_HEAT, R, 4, 1, 4, false>; \
+        if (fe == ELFEL_T3 && nq == 1) return csc ? launch_reuse<ELFEL_HEAT, R, 3, 1, 1, true, B> : launch_reuse<ELFEL_HEAT, R, 3, 1, 1, false, B>; \
+        if (fe == ELFEL_T3 && nq == 3) return csc ? launch_reuse<ELFEL_HEAT, R, 3, 1, 3, true, B> : launch_reuse<ELFEL_HEAT, R, 3, 1, 3, false, B>; \
+        if (fe == ELFEL_T6 && nq == 3) return csc ? launch_reuse<ELFEL_HEAT, R, 6, 1, 3, true, B> : launch_reuse<ELFEL_HEAT, R, 6, 1, 3, false, B>; \
+        if (fe == ELFEL_Q4 && nq == 1) return csc ? launch_reuse<ELFEL_HEAT, R, 4, 1, 1, true, B> : launch_reuse<ELFEL_HEAT, R, 4, 1, 1, false, B>; \
+        if (fe == ELFEL_Q4 && nq == 4) return csc ? launch_reuse<ELFEL_HEAT, R, 4, 1, 4, true, B> : launch_reuse<ELFEL_HEAT, R, 4, 1, 4, false, B>; \
     } else if (kind == ELFEL_ELAST) {                                                                \
-        if (fe == ELFEL_T3 && nq == 1) return csc ? launch_reuse<ELFEL_ELAST, R, 3, 2, 1, true> : launch_reuse<ELFEL_ELAST, R, 3, 2, 1, false>; \
-        if (fe == ELFEL_T3 && nq == 3) return csc ? launch_reuse<ELFEL_ELAST, R, 3, 2, 3, true> : launch_reuse<ELFEL_ELAST, R, 3, 2, 3, false>; \
-        if (fe == ELFEL_T6 && nq == 3) return csc ? launch_reuse<ELFEL_ELAST, R, 6, 2, 3, true> : launch_reuse<ELFEL_ELAST, R, 6, 2, 3, false>; \
-        if (fe == ELFEL_Q4 && nq == 4) return csc ? launch_reuse<ELFEL_ELAST, R, 4, 2, 4, true> : launch_reuse<ELFEL_ELAST, R, 4, 2, 4, false>; \
+        if (fe == ELFEL_T3 && nq == 1) return csc ? launch_reuse<ELFEL_ELAST, R, 3, 2, 1, true, B> : launch_reuse<ELFEL_ELAST, R, 3, 2, 1, false, B>; \
+        if (fe == ELFEL_T3 && nq == 3) return csc ? launch_reuse<ELFEL_ELAST, R, 3, 2, 3, true, B> : launch_reuse<ELFEL_ELAST, R, 3, 2, 3, false, B>; \
+        if (fe == ELFEL_T6 && nq == 3) return csc ? launch_reuse<ELFEL_ELAST, R, 6, 2, 3, true, B> : launch_reuse<ELFEL_ELAST, R, 6, 2, 3, false, B>; \
+        if (fe == ELFEL_Q4 && nq == 4) return csc ? launch_reuse<ELFEL_ELAST, R, 4, 2, 4, true, B> : launch_reuse<ELFEL_ELAST, R, 4, 2, 4, false, B>; \
     } else if (kind == ELFEL_STOKES_GEN) {                                                           \
-        if (fe == ELFEL_T6 && nq == 3) return csc ? launch_reuse<ELFEL_STOKES_GEN, R, 6, 2, 3, true> : launch_reuse<ELFEL_STOKES_GEN, R, 6, 2, 3, false>; \
+        if (fe == ELFEL_T6 && nq == 3) return csc ? launch_reuse<ELFEL_STOKES_GEN, R, 6, 2, 3, true, B> : launch_reuse<ELFEL_STOKES_GEN, R, 6, 2, 3, false, B>; \
     } else if (kind == ELFEL_STOKES_VECLAP) {                                                        \
-        if (fe == ELFEL_T6 && nq == 3) return csc ? launch_reuse<ELFEL_STOKES_VECLAP, R, 6, 2, 3, true> : launch_reuse<ELFEL_STOKES_VECLAP, R, 6, 2, 3, false>; \
+        if (fe == ELFEL_T6 && nq == 3) return csc ? launch_reuse<ELFEL_STOKES_VECLAP, R, 6, 2, 3, true, B> : launch_reuse<ELFEL_STOKES_VECLAP, R, 6, 2, 3, false, B>; \
     }                                                                                                \
     return nullptr;

@@ -121,10 +121,11 @@ int elfel_hip_assemble(elfel_hip_ctx *ctx, const elfel_hip_problem *pb, int64_t 
 int elfel_hip_copy_vector(elfel_hip_ctx *ctx, double *F, int64_t n);
 /* Blocks until the context's stream is idle. */
 int elfel_hip_synchronize(elfel_hip_ctx *ctx);
-/* Smallest Jacobian determinant seen at any quadrature point by the last
- * assembly, and the number of elements routed to the FP64 branch of the
- * mixed kernels. */
-int elfel_hip_geometry_stats(elfel_hip_ctx *ctx, double *min_det, int64_t *n_fp64_fallback);
+/* Mesh quality of field 0: the smallest Jacobian determinant at any
+ * quadrature point (FP64), and the number of elements with det J <= 0 or a
+ * shape-quality proxy sum_k |grad N_k|^2 det J >= 1024 at some point.
+ * Informational: every kernel treats every element alike. */
+int elfel_hip_geometry_stats(elfel_hip_ctx *ctx, double *min_det, int64_t *n_bad_shape);
 
 /* ---- solve --------------------------------------------------------------
  * Replaces the driver's solve! (test/test_heat.jl:73-76: T[1:nu] =

@@ -121,8 +121,10 @@ def test_reuse_equals_plain_bitwise():
         assert np.array_equal(a, b)
 
 
-def test_mixed_fallback_on_distorted_elements():
-    """Badly shaped elements take the FP64 Jacobian branch and stay accurate."""
+def test_mixed_accuracy_on_sliver_elements():
+    """Slivers of aspect ratio 1e4 stay within the mixed tolerance (the mixed
+    kernels form J, det J and 1/det J in FP64) and are counted by the
+    geometry statistics."""
     cfg = W.elast_t6(16, frac=0.0)
     fesp, n = cfg["fesp"], cfg["ndofs"]
     xy = fesp.mesh.xy
@@ -134,7 +136,7 @@ def test_mixed_fallback_on_distorted_elements():
     am = eh.start(eh.SysmatAssemblerHIP(), n, n)
     eh.integrate_elasticity(am, fesp, cfg["qsettings"], cfg["D"], precision="mixed", method="reuse")
     dev = am.finish(as_scipy=False)
-    mindet, nfb = am.context().geometry_stats()
+    mindet, nbad = am.context().geometry_stats()
     compare_sparse(dev, ref, rtol=1e-4)
     assert frob_rel(dev[2], ref[2]) <= MIXED_TOL
-    assert nfb > 0 and mindet > 0
+    assert nbad > 0 and mindet > 0
