@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-pcg", action="store_true", help="c5: assembly only")
+    ap.add_argument("--pcg-maxit", type=int, default=60000)
     ap.add_argument("--only", choices=["all", "mixed", "fp64", "fp64-reuse"], default="all")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
                     help="only run a device copy of this size (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -115,12 +116,17 @@ def setup_case(cfgname, N, ctx, L, W):
     raise ValueError(cfgname)
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def bench_single(args, L, W):
     ctx = L.Context(0)
     devname, ncu = ctx.device_name()
     t0 = time.time()
     cs = setup_case(args.config, args.N, ctx, L, W)
     t_setup = time.time() - t0
+    progress(f"{args.config}: mesh and numbering ready ({t_setup:.1f} s)")
     E, Nn, n = cs["E"], cs["Nn"], cs["n"]
 
     def assemble(precision, method, wait=True):
@@ -133,13 +139,18 @@ def bench_single(args, L, W):
         ctx.synchronize()
         for _ in range(warmup):
             assemble(precision, method, wait=False)
+        # the timed steps: device idle and synchronised on both sides, no events inside
         ctx.synchronize()
-        ctx.timing(True)
         w0 = time.perf_counter()
         for _ in range(steps):
             assemble(precision, method, wait=False)
         ctx.synchronize()
         wall = (time.perf_counter() - w0) * 1e3 / steps
+        # the same steps again with hipEvents around every phase (per-kernel times)
+        ctx.timing(True)
+        for _ in range(steps):
+            assemble(precision, method, wait=False)
+        ctx.synchronize()
         ph = ctx.timing_read()
         ctx.timing(False)
         ptr, ind, val, shape, lay = ctx.copy_sparse(base=0)
@@ -152,12 +163,16 @@ def bench_single(args, L, W):
     assemble(L.MIXED, L.REUSE)
     ctx.synchronize()
     t_sym = (time.perf_counter() - t0) * 1e3
+    progress(f"symbolic phase {t_sym:.0f} ms")
     if args.only in ("all", "mixed"):
         res["mixed"] = run(L.MIXED, L.REUSE, args.steps, args.warmup)
+        progress("mixed reuse timed")
     if args.only in ("all", "fp64-reuse"):
         res["fp64_reuse"] = run(L.FP64, L.REUSE, args.steps, args.warmup)
+        progress("fp64 reuse timed")
     if args.only in ("all", "fp64"):
         res["fp64"] = run(L.FP64, L.SORT, max(1, min(args.fp64_steps, args.steps)), min(2, args.warmup))
+        progress("fp64 plain timed")
     mindet, nbad = ctx.geometry_stats()
 
     nnz = next(iter(res.values()))["nnz"]
@@ -172,12 +187,11 @@ def bench_single(args, L, W):
     out = {}
     for key, r in res.items():
         ph = r["phases"]
-        steps = max(1, ph["total"][1])
-        ms = ph["total"][0] / steps
+        ms = r["wall_ms"]
 
         def per(name):
             return (ph[name][0] / ph[name][1]) if ph[name][1] else None
-        out[key] = dict(ms_per_step=ms, wall_ms_per_step=r["wall_ms"], elements_per_s=E / (ms * 1e-3),
+        out[key] = dict(ms_per_step=ms, event_ms_per_step=per("total"), elements_per_s=E / (ms * 1e-3),
                         numeric_ms=per("numeric"), gather_ms=per("gather"), element_ms=per("element"),
                         sort_ms=per("sort"), reduce_ms=per("reduce"),
                         full_assembly_gbs=full_bytes / (ms * 1e-3) / 1e9)
@@ -201,7 +215,7 @@ def bench_single(args, L, W):
 
     extra = {}
     if args.config == "c5" and not args.no_pcg:
-        extra["pcg"] = bench_pcg(ctx, L, cs, res)
+        extra["pcg"] = bench_pcg(ctx, L, cs, res, args.pcg_maxit)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -238,7 +252,7 @@ def bench_single(args, L, W):
     print(json.dumps(line))
 
 
-def bench_pcg(ctx, L, cs, res):
+def bench_pcg(ctx, L, cs, res, args_maxit=60000):
     """C5: Jacobi-PCG on the free block, FP32 inner iterations with FP64
     refinement against FP64 PCG, both to ||b - A x|| / ||b|| <= 1e-10."""
     cfg = cs["cfg"]
@@ -253,8 +267,10 @@ def bench_pcg(ctx, L, cs, res):
     for name, prec in (("mixed", L.MIXED), ("fp64", L.FP64)):
         ctx.synchronize()
         t0 = time.perf_counter()
-        x, it, outer, rr = ctx.pcg(nu, F, x0, precision=prec, tol=1e-10, maxit=200000)
+        progress(f"pcg {name} ...")
+        x, it, outer, rr = ctx.pcg(nu, F, x0, precision=prec, tol=1e-10, maxit=args_maxit)
         dt = time.perf_counter() - t0
+        progress(f"pcg {name}: {it} iterations, {outer} residual updates, relres {rr:.2e}, {dt:.1f} s")
         outp[name] = dict(seconds=dt, iterations=it, refinements=outer, relres=rr, x=x)
     ptr = res["fp64"]["ptr"] if "fp64" in res else None
     if ptr is not None:

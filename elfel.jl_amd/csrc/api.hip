@@ -88,7 +88,7 @@ void tm_end(elfel_hip_ctx *c, int phase, hipEvent_t a)
     if (!b) return;
     hipEventRecord(b, c->stream);
     c->tm.pending.push_back({phase, a, b});
-    if (c->tm.pending.size() > 4096) tm_collect(c);
+    if (c->tm.pending.size() >= 240) tm_collect(c);
 }
 
 void tm_collect(elfel_hip_ctx *c)
@@ -159,7 +159,7 @@ extern "C" int elfel_hip_free(elfel_hip_ctx *c)
     for (auto &f : c->f) { release(f.conn); release(f.xy); release(f.dof); }
     Buf *bufs[] = {&c->kappa_e, &c->Q_e, &c->scale_e, &c->geom, &c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1],
                    &c->hist, &c->scan_tmp, &c->flags, &c->scal, &c->fkeys[0], &c->fkeys[1], &c->fvals[0],
-                   &c->fvals[1], &c->stage64, &c->ptr, &c->ind, &c->val, &c->F, &c->dscal, &c->pcg,
+                   &c->fvals[1], &c->stage64, &c->ptr, &c->ind, &c->val, &c->F, &c->dscal, &c->pcg, &c->pcg_sell,
                    &c->pat.ptr, &c->pat.ind, &c->pat.tile, &c->pat.tile_rows, &c->pat.task_elem,
                    &c->pat.task_info, &c->pat.task_pos};
     for (Buf *b : bufs) release(*b);
@@ -509,6 +509,12 @@ extern "C" int elfel_hip_timing(elfel_hip_ctx *c, int enable)
     tm_collect(c);
     c->tm.on = enable != 0;
     for (int i = 0; i < ELFEL_NPHASES; i++) { c->tm.ms[i] = 0; c->tm.n[i] = 0; }
+    // events are created here, not inside timed launches
+    while (c->tm.on && c->tm.pool.size() < 512) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) break;
+        c->tm.pool.push_back(e);
+    }
     return ELFEL_OK;
 }
 

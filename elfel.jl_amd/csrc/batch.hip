@@ -9,7 +9,7 @@
 // or 256 threads, the tile capacity) and the persistent grid size -- is chosen
 // by timing every variant on the batch itself (autotune) and cached on disk,
 // keyed by device and batch signature.
-// assemble: one graph launch = gather T3 + gather T6 + k_reuse_batch.
+// assemble: one graph launch of k_reuse_batch (it gathers the coordinates itself).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -31,7 +31,7 @@ struct BatchProb {
 
 struct KindData {
     int64_t E = 0, Nn = 0, ntasks = 0, ntiles = 0;
-    Buf conn, xy, coef, geom;
+    Buf conn, xy, coef;
     Pattern tab[2];  // tables for block 128 and 256
 };
 
@@ -69,7 +69,7 @@ void batch_free(elfel_hip_ctx *c)
     if (!b) return;
     batch_destroy_graph(b);
     for (auto &k : b->kd) {
-        release(k.conn); release(k.xy); release(k.coef); release(k.geom);
+        release(k.conn); release(k.xy); release(k.coef);
         for (auto &t : k.tab) {
             release(t.tile); release(t.tile_rows); release(t.task_elem); release(t.task_info); release(t.task_pos);
         }
@@ -166,7 +166,7 @@ static ReuseArgs kind_args(KindData &k, int bi, double *vals, double *F)
     A.task_pos = (const uint8_t *)P.task_pos.p;
     A.ncolp = P.ncolmax;
     A.ntiles = (int)P.ntiles;
-    A.geom = k.geom.p;
+    A.geom = nullptr;  // the kernel gathers conn and xy itself
     A.conn = (const int32_t *)k.conn.p;
     A.xy = (const double2 *)k.xy.p;
     A.vals = vals;
@@ -179,14 +179,6 @@ static int batch_launch(elfel_hip_ctx *c, Batch *b, int block, int grid)
 {
     const int bi = block == 128 ? 0 : 1;
     KindData &H = b->kd[0], &Ev = b->kd[1];
-    if (H.E) {
-        int rc = gather_geom_rel(c, 3, H.E, (const int32_t *)H.conn.p, (const double2 *)H.xy.p, (double *)H.geom.p);
-        if (rc) return rc;
-    }
-    if (Ev.E) {
-        int rc = gather_geom_rel(c, 6, Ev.E, (const int32_t *)Ev.conn.p, (const double2 *)Ev.xy.p, (double *)Ev.geom.p);
-        if (rc) return rc;
-    }
     QTable th{}, te{};
     make_qtable(ELFEL_T3, 1, &th);
     make_qtable(ELFEL_T6, 3, &te);
@@ -202,7 +194,8 @@ static int batch_launch(elfel_hip_ctx *c, Batch *b, int block, int grid)
     if (!Ev.E) AE.ntiles = 0;
     const int nt = AH.ntiles + AE.ntiles;
     if (nt == 0) return 0;
-    const int lds_vals = std::max(H.E ? (int)H.tab[bi].lds_vals : 0, Ev.E ? (int)Ev.tab[bi].lds_vals : 0);
+    int lds_vals = std::max(H.E ? (int)H.tab[bi].lds_vals : 0, Ev.E ? (int)Ev.tab[bi].lds_vals : 0);
+    lds_vals = (lds_vals + 1) & ~1;  // even: the zero fill writes pairs
     const size_t lds = sizeof(double) * ((size_t)lds_vals + block);
     const int g = grid > 0 ? std::min(grid, nt) : nt;
     const bool csc = b->layout == ELFEL_CSC;
@@ -315,9 +308,7 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
                 for (int64_t e = 0; e < p.E; e++) coef[p.elem_base + e] = p.has_coef ? p.coef[e] : u;
             }
         }
-        if ((rc = ensure(c, K.conn, 4 * conn.size())) || (rc = ensure(c, K.xy, 8 * xy.size())) ||
-            (rc = ensure(c, K.geom, 8 * (size_t)K.E * nen * 2)))
-            return rc;
+        if ((rc = ensure(c, K.conn, 4 * conn.size())) || (rc = ensure(c, K.xy, 8 * xy.size()))) return rc;
         HIPCHK(c, hipMemcpy(K.conn.p, conn.data(), 4 * conn.size(), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(K.xy.p, xy.data(), 8 * xy.size(), hipMemcpyHostToDevice));
         if (any_coef) {

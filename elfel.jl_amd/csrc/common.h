@@ -17,26 +17,12 @@
 #include <vector>
 
 #include "../../include/elfel_hip.h"
+#include "qtable.h"
 
-#define ELFEL_MAXQ 4     // quadrature points per element (triangle 1/3, square 1x1/2x2)
-#define ELFEL_MAXBF 6    // basis functions of the scalar element (T3 3, Q4 4, T6 6)
-
-// ---- quadrature / basis tables --------------------------------------------
-// Tabulated once on the host from the reference formulas (RefShapes.jl,
-// FElements.jl) and handed to the kernels by value: every index into the
-// table is wave-uniform, so the loads are scalar (SGPR) loads from the kernel
-// argument segment.
-struct QTable {
-    int nq, nbf;
-    double w[ELFEL_MAXQ];
-    double N[ELFEL_MAXQ][ELFEL_MAXBF];
-    double G[ELFEL_MAXQ][ELFEL_MAXBF][2];
-    double Np[ELFEL_MAXQ][3];  // T3 basis at the same points (Taylor-Hood pressure)
-    float wf[ELFEL_MAXQ];
-    float Nf[ELFEL_MAXQ][ELFEL_MAXBF];
-    float Gf[ELFEL_MAXQ][ELFEL_MAXBF][2];
-    float Npf[ELFEL_MAXQ][3];
-};
+// Quadrature / basis tables (qtable.h) reach the FP64 kernels by value: every
+// index into the table is wave-uniform, so the loads are scalar (SGPR) loads
+// from the kernel argument segment.
+static_assert(QT_T3 == ELFEL_T3 && QT_T6 == ELFEL_T6 && QT_Q4 == ELFEL_Q4, "element codes");
 
 // Coefficients of the weak form (uniform values + optional per-element arrays).
 struct Coef {
@@ -113,8 +99,8 @@ struct elfel_hip_ctx {
     // device scalars: [0] nnz (int32), [1] error flags, [3] bad-shape count (int64, at byte 24)
     Buf dscal;
     Timing tm;
-    // solver scratch
-    Buf pcg;
+    // solver scratch: vectors, then the SELL-64 copy of the free block
+    Buf pcg, pcg_sell;
     // batched problems (batch.hip)
     void *batch = nullptr;
 };

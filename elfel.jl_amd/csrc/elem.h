@@ -74,21 +74,21 @@ __device__ __forceinline__ void geometry(const QTable &t, const R *X, const R *Y
 
 // Mixed-precision geometry.  An FP32 Jacobian loses up to ~1e-5 of the matrix
 // to the cancellation in sum_k x_k grad N_k on curved T6 elements, so J, det
-// and 1/det stay in FP64; only the results are rounded.
+// and 1/det stay in FP64; only the results are rounded.  X, Y are relative to
+// node 0 (X[0] = Y[0] = 0, so node 0 adds nothing to J) and t is a
+// compile-time table (qt_fixed), so the zero entries of grad N drop out.
 template <int NEN, int NQ>
 __device__ __forceinline__ void geometry_mixed(const QTable &t, const double *X, const double *Y,
                                                Geo<float, NEN, NQ> &g)
 {
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
-        double j11 = X[0] * t.G[q][0][0], j21 = Y[0] * t.G[q][0][0];
-        double j12 = X[0] * t.G[q][0][1], j22 = Y[0] * t.G[q][0][1];
+        double j11 = 0.0, j21 = 0.0, j12 = 0.0, j22 = 0.0;
 #pragma unroll
         for (int k = 1; k < NEN; k++) {
-            j11 = fma(X[k], t.G[q][k][0], j11);
-            j21 = fma(Y[k], t.G[q][k][0], j21);
-            j12 = fma(X[k], t.G[q][k][1], j12);
-            j22 = fma(Y[k], t.G[q][k][1], j22);
+            const double a = t.G[q][k][0], b = t.G[q][k][1];
+            if (a != 0.0) { j11 = fma(X[k], a, j11); j21 = fma(Y[k], a, j21); }
+            if (b != 0.0) { j12 = fma(X[k], b, j12); j22 = fma(Y[k], b, j22); }
         }
         const double det = j11 * j22 - j21 * j12;
         const double inv = 1.0 / det;
@@ -98,8 +98,10 @@ __device__ __forceinline__ void geometry_mixed(const QTable &t, const double *X,
 #pragma unroll
         for (int k = 0; k < NEN; k++) {
             const double a = t.G[q][k][0], b = t.G[q][k][1];
-            g.gx[q][k] = (float)fma(a, i11, b * i12);
-            g.gy[q][k] = (float)fma(b, i22, a * i21);
+            const double gx = a == 0.0 ? b * i12 : (b == 0.0 ? a * i11 : fma(a, i11, b * i12));
+            const double gy = b == 0.0 ? a * i21 : (a == 0.0 ? b * i22 : fma(b, i22, a * i21));
+            g.gx[q][k] = (float)gx;
+            g.gy[q][k] = (float)gy;
         }
     }
 }

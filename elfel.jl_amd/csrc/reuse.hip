@@ -84,7 +84,7 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     const int nen = F0.nen, nc = F0.ncomp, nd = nen * nc;
     const int ncol = nd + (stokes ? 3 : 0);
     const int ncolp = (ncol + 3) & ~3;
-    const int vcap = REUSE_VCAP_PER_THREAD * cap;
+    const int vcap = std::min(std::max(env_int("ELFEL_HIP_VCAP", REUSE_VCAP_PER_THREAD), 8), 24) * cap;
     Adj A0, A1;
     adjacency(F0, A0);
     if (stokes) adjacency(*F1, A1);
@@ -132,6 +132,13 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
                         std::copy(tp.begin() + k * ncolp, tp.begin() + (k + 1) * ncolp, T.pos.begin() + o * ncolp);
                         o++;
                     }
+        }
+        // pad to cap tasks, so that tile t's tasks start at t * cap (the kernel
+        // computes a thread's task index without reading the tile first)
+        for (int k = cur.y; k < cap; k++) {
+            T.elem.push_back(0);
+            T.info.push_back(TASK_PAD);
+            T.pos.insert(T.pos.end(), ncolp, (uint8_t)0);
         }
         T.tiles.push_back(cur);
         T.trows.push_back(cr);
@@ -207,27 +214,38 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
 // tiles that read the same element records close together; laying the sorted
 // list out so that workgroup b (dispatched to XCD b mod 8) walks a contiguous
 // chunk keeps those re-reads inside one XCD's L2.
-static void order_tiles(ReuseTables &T, int nxcd)
+static void order_tiles(ReuseTables &T, int nxcd, int cap)
 {
     const size_t nt = T.tiles.size();
     if (nt < 2) return;
     std::vector<std::pair<int32_t, int32_t>> key(nt);
     for (size_t t = 0; t < nt; t++) {
         int32_t m = INT32_MAX;
-        for (int k = 0; k < T.tiles[t].y; k++) m = std::min(m, T.elem[T.tiles[t].x + k]);
+        for (int k = 0; k < T.tiles[t].y; k++) m = std::min(m, T.elem[(size_t)T.tiles[t].x + k]);
         key[t] = {m, (int32_t)t};
     }
     std::stable_sort(key.begin(), key.end());
     std::vector<int64_t> off(nxcd + 1, 0);
     for (int x = 0; x < nxcd; x++) off[x + 1] = off[x] + (int64_t)(nt / nxcd + ((size_t)x < nt % nxcd ? 1 : 0));
     std::vector<int4> tiles(nt), trows(nt);
+    std::vector<int32_t> elem(T.elem.size());
+    std::vector<uint32_t> info(T.info.size());
+    std::vector<uint8_t> pos(T.pos.size());
+    const size_t ncolp = (size_t)T.ncolp;
     for (size_t b = 0; b < nt; b++) {
-        const size_t pos = off[b % nxcd] + b / nxcd;
-        tiles[b] = T.tiles[key[pos].second];
-        trows[b] = T.trows[key[pos].second];
+        const size_t o = (size_t)key[off[b % nxcd] + b / nxcd].second;
+        tiles[b] = T.tiles[o];
+        trows[b] = T.trows[o];
+        tiles[b].x = (int)(b * cap);  // the tasks move with their tile
+        std::copy(T.elem.begin() + o * cap, T.elem.begin() + (o + 1) * cap, elem.begin() + b * cap);
+        std::copy(T.info.begin() + o * cap, T.info.begin() + (o + 1) * cap, info.begin() + b * cap);
+        std::copy(T.pos.begin() + o * cap * ncolp, T.pos.begin() + (o + 1) * cap * ncolp, pos.begin() + b * cap * ncolp);
     }
     T.tiles.swap(tiles);
     T.trows.swap(trows);
+    T.elem.swap(elem);
+    T.info.swap(info);
+    T.pos.swap(pos);
 }
 
 int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P)
@@ -276,7 +294,7 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
     if (nnz) HIPCHK(c, hipMemcpyAsync(P.ind.p, c->ind.p, 4 * nnz, hipMemcpyDeviceToDevice, c->stream));
     ReuseTables T;
     if ((rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, block, T))) return rc;
-    if (env_int("ELFEL_HIP_TILE_ORDER", 1)) order_tiles(T, 8);
+    if (env_int("ELFEL_HIP_TILE_ORDER", 1)) order_tiles(T, 8, block);
     if ((rc = upload_tables(c, T, P))) return rc;
     P.block = block;
     P.kind = pb->kind; P.fe = pb->fe; P.layout = pb->layout;

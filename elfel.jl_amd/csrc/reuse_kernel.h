@@ -18,12 +18,18 @@
 //   bit  9     owner is a pressure dof (Taylor-Hood)
 //   bits 10-23 LDS offset of the group's first row in the tile
 //   bits 24-31 row length L (two-row groups) or tile-local row (one-row groups)
+// A tile holds exactly BLOCK task slots (tile t's tasks start at t * BLOCK);
+// unused slots carry TASK_PAD (rows code 3) and element 0, and add nothing.
 // task_pos[task][j]: position of element column j in the owner's sorted
 // column list (uint8).
 #pragma once
 #include "elem.h"
 
 #define REUSE_BLOCK 256
+#define TASK_PAD (3u << 7)
+#ifndef REUSE_WAVES_PER_EU
+#define REUSE_WAVES_PER_EU 1   // occupancy hint for the mixed kernels (1 = none)
+#endif
 #define REUSE_VCAP_PER_THREAD 24   // LDS doubles for tile values per thread of the block (48 KiB at 256)
 #define REUSE_VCAP (REUSE_VCAP_PER_THREAD * REUSE_BLOCK)
 #define REUSE_COND_MAX 1024.0  // sum_k |grad N_k|^2 * det J: above it an element counts as badly shaped
@@ -248,11 +254,20 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
     const int tid = threadIdx.x;
     const int4 td = A.tiles[tile];
     const int4 tr = A.tile_rows[tile];
-    for (int v = tid; v < td.w; v += BLOCK) acc[v] = 0.0;
+    {   // zero the tile's sums, two per lane (the LDS block is 16-byte aligned and sized to an even count)
+        double2 *a2 = (double2 *)acc;
+        const int n2 = (td.w + 1) >> 1;
+#pragma unroll 4
+        for (int v = tid; v < n2; v += BLOCK) a2[v] = make_double2(0.0, 0.0);
+    }
     if constexpr (KIND == ELFEL_HEAT)
         for (int v = tid; v < tr.y; v += BLOCK) accF[v] = 0.0;
 
-    const bool active = tid < td.y;
+    // every slot holds a task record; pad slots skip the arithmetic and add nothing
+    const int64_t task = (int64_t)tile * BLOCK + tid;
+    const int32_t e = A.task_elem[task];
+    const uint32_t info = A.task_info[task];
+    const bool active = ((info >> 7) & 3) != 3;
     int rank = -1, rowcode = 0, vbase = 0, L8 = 0;
     bool pown = false;
     R out[2][ND + 3];
@@ -261,9 +276,6 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
     constexpr int NPW = (NCOL + 3) / 4;
     uint32_t posw[NPW];
     if (active) {
-        const int64_t task = (int64_t)td.x + tid;
-        const int32_t e = A.task_elem[task];
-        const uint32_t info = A.task_info[task];
         const uint32_t *pw = (const uint32_t *)(A.task_pos + task * A.ncolp);
 #pragma unroll
         for (int w = 0; w < NPW; w++) posw[w] = pw[w];
@@ -301,17 +313,19 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
             }
         }
         Geo<R, NEN, NQ> g;
-        if constexpr (MIXED) geometry_mixed<NEN, NQ>(tab, X, Y, g);
+        constexpr QTable CT = qt_fixed<NEN, NQ>();
+        if constexpr (MIXED) geometry_mixed<NEN, NQ>(CT, X, Y, g);
         else geometry<double, NEN, NQ>(tab, X, Y, g);
         R Dr[9];
 #pragma unroll
         for (int i = 0; i < 9; i++) Dr[i] = (R)(cf.scale_e ? sc * cf.D[i] : cf.D[i]);
-        owner_values<KIND, R, NEN, NC, NQ, CSC>(tab, g, Dr, (R)kapd, (R)cf.mu, (R)Qd, a, comp0, nr, powner, out, &fo);
+        owner_values<KIND, R, NEN, NC, NQ, CSC>(MIXED ? CT : tab, g, Dr, (R)kapd, (R)cf.mu, (R)Qd, a, comp0, nr, powner,
+                                                out, &fo);
     }
     __syncthreads();
     const int nranks = tr.z;
     for (int s = 0; s < nranks; s++) {
-        if (rank == s) {
+        if (active && rank == s) {
             // A task's positions are distinct (distinct dofs of one element), so
             // all of its partial sums are read before any is written back: one
             // LDS round trip per rank instead of one per entry.
@@ -333,14 +347,31 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
         }
         __syncthreads();
     }
+    // Coalesced write of the tile's values, 16 bytes per lane (the pair of
+    // non-temporal stores merges into one): an odd first (or last) value of the
+    // range goes out alone.  Non-temporal stores stream the values past the
+    // caches -- nothing in this launch reads them again -- and cut the kernel
+    // time by a quarter on MI355X.
     double *dst = A.vals + td.z;
-    for (int v = tid; v < td.w; v += BLOCK) dst[v] = acc[v];
+    const int head = td.z & 1;
+    const int npair = (td.w - head) >> 1;
+    if (tid == 0) {
+        if (head && td.w > 0) __builtin_nontemporal_store(acc[0], &dst[0]);
+        if (((td.w - head) & 1) && td.w > head) __builtin_nontemporal_store(acc[td.w - 1], &dst[td.w - 1]);
+    }
+#pragma unroll 4
+    for (int p = tid; p < npair; p += BLOCK) {
+        const int v = head + 2 * p;
+        __builtin_nontemporal_store(acc[v], &dst[v]);
+        __builtin_nontemporal_store(acc[v + 1], &dst[v + 1]);
+    }
     if constexpr (KIND == ELFEL_HEAT)
         for (int v = tid; v < tr.y; v += BLOCK) A.F[tr.x + v] = accF[v];
 }
 
 template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_reuse(ReuseArgs A, Coef cf, QTable tab, int lds_vals)
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? REUSE_WAVES_PER_EU : 1)))
+void k_reuse(ReuseArgs A, Coef cf, QTable tab, int lds_vals)
 {
     extern __shared__ double lds[];
     reuse_tile<KIND, R, NEN, NC, NQ, CSC, BLOCK>(A, cf, tab, lds, lds_vals, blockIdx.x);
@@ -408,6 +439,7 @@ template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC, int BLOCK>
 int launch_reuse(elfel_hip_ctx *c, const ReuseArgs &A, const Coef &cf, const QTable &tab, int lds_vals)
 {
     if (A.ntiles <= 0) return 0;
+    lds_vals = (lds_vals + 1) & ~1;  // even: the zero fill writes pairs, the load-vector sums stay 16-byte aligned
     const size_t lds = sizeof(double) * ((size_t)lds_vals + (KIND == ELFEL_HEAT ? BLOCK : 0));
     hipLaunchKernelGGL((k_reuse<KIND, R, NEN, NC, NQ, CSC, BLOCK>), dim3(A.ntiles), dim3(BLOCK), lds, c->stream, A,
                        cf, tab, lds_vals);
