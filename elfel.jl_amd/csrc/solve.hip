@@ -23,14 +23,17 @@
 // slice), built once per solve from the CSR result: one thread per row, every
 // load coalesced, data columns left out (their part of b is formed once with
 // the CSR matrix).
-// Per iteration three kernels: q = A p with the partial sums of p.q; the
-// x / r / z update with the partials of r.z and r.r; p = z + beta p.  Each
-// reduction finishes in the last workgroup to arrive (a counter, then a fixed-
-// order sum of the partials), so every run gives the same bits; the CG scalars
-// stay on the device and the host reads the residual every CHECK iterations.
+// Per iteration three kernels on a fixed grid of 4 workgroups per CU
+// (grid-stride loops): q = A p with one partial of p.q per workgroup; the
+// x / r / z update (every workgroup first sums the p.q and previous r.z
+// partials itself, in the same order) with partials of r.z and r.r; and
+// p = z + beta p.  No atomics and no extra reduction launches; every run gives
+// the same bits.  The CG scalars stay on the device and the host reads the
+// residual every CHECK iterations.
 #include <math.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "common.h"
 
@@ -40,8 +43,8 @@
 
 namespace {
 
-// Scalars (double) on the device.
-enum { S_RZ = 0, S_PQ = 1, S_RR = 2, S_ALPHA = 3, S_BETA = 4, S_BB = 5, S_RR64 = 6, S_NS = 8 };
+// Scalars (double) on the device, for the host's convergence checks.
+enum { S_RZ = 0, S_RR = 1, S_BB = 2, S_RR64 = 3, S_NS = 8 };
 
 __device__ __forceinline__ double block_sum(double v, double *sh)
 {
@@ -51,29 +54,16 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
     if (lane == 0) sh[w] = v;
     __syncthreads();
     double s = 0.0;
-    if (threadIdx.x == 0)
-        for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += sh[k];
-    return s;  // valid in thread 0
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += sh[k];
+    return s;  // the same value in every thread
 }
 
-// True in every thread of the last workgroup to finish; the partials written
-// before the call are then visible to it.
-__device__ __forceinline__ bool last_block(unsigned *counter)
-{
-    __shared__ bool last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (last) __threadfence();  // acquire: the other workgroups' partials
-    return last;
-}
-
-// Fixed-order sum of n partials by one workgroup (valid in thread 0).
-__device__ __forceinline__ double sum_partials(const double *p, int n, double *sh)
+// Fixed-order sum of the g partials of the previous kernel (every block of
+// the grid computes it, in the same order, so all get the same bits).
+__device__ __forceinline__ double sum_partials(const double *p, int g, double *sh)
 {
     double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v += p[i];
+    for (int i = threadIdx.x; i < g; i += blockDim.x) v += p[i];
     return block_sum(v, sh);
 }
 
@@ -145,120 +135,116 @@ __device__ __forceinline__ T sell_row(int64_t i, const uint32_t *__restrict__ of
     return acc;
 }
 
-// q = A p, then alpha = rz / (p.q) in the last workgroup.
+// The CG kernels run on a fixed grid of G workgroups with grid-stride loops
+// over the rows; each writes one partial per workgroup per dot product.
+//   parts layout: pq[G], rz[2][G] (by iteration parity), rr[G], aux[G]
+
+// q = A p; partial p.q
 template <typename T>
 __global__ __launch_bounds__(SV_BLOCK) void k_cg_spmv(int64_t n, const uint32_t *__restrict__ off,
                                                       const int32_t *__restrict__ col, const T *__restrict__ val,
                                                       const T *__restrict__ p, T *__restrict__ q,
-                                                      double *__restrict__ part, unsigned *counter,
-                                                      double *__restrict__ sc)
+                                                      double *__restrict__ part_pq)
 {
     __shared__ double sh[SV_BLOCK / 64];
-    const int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x;
     double d = 0.0;
-    if (i < n) {
+    for (int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * SV_BLOCK) {
         const T y = sell_row<T>(i, off, col, val, p);
         q[i] = y;
-        d = (double)p[i] * (double)y;
+        d += (double)p[i] * (double)y;
     }
     const double s = block_sum(d, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-    if (last_block(counter)) {
-        const double pq = sum_partials(part, gridDim.x, sh);
-        if (threadIdx.x == 0) {
-            sc[S_PQ] = pq;
-            sc[S_ALPHA] = pq != 0.0 ? sc[S_RZ] / pq : 0.0;
-            *counter = 0u;
-        }
-    }
+    if (threadIdx.x == 0) part_pq[blockIdx.x] = s;
 }
 
-// x += alpha p; r -= alpha q; z = M r; then rz_new, rr and beta = rz_new / rz.
+// alpha = rz_old / pq; x += alpha p; r -= alpha q; z = M r; partials r.z (into
+// rz_new) and r.r
 template <typename T>
 __global__ __launch_bounds__(SV_BLOCK) void k_cg_update(int64_t n, const T *__restrict__ dinv, T *__restrict__ x,
                                                         T *__restrict__ r, T *__restrict__ z,
                                                         const T *__restrict__ p, const T *__restrict__ q,
-                                                        double *__restrict__ part, unsigned *counter,
-                                                        double *__restrict__ sc)
+                                                        const double *__restrict__ part_pq,
+                                                        const double *__restrict__ rz_old,
+                                                        double *__restrict__ rz_new, double *__restrict__ part_rr)
 {
     __shared__ double sh[SV_BLOCK / 64];
-    const T alpha = (T)sc[S_ALPHA];
-    const int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x;
-    double rz = 0.0, rr = 0.0;
-    if (i < n) {
+    const int G = gridDim.x;
+    const double pq = sum_partials(part_pq, G, sh);
+    const double rz = sum_partials(rz_old, G, sh);
+    const T alpha = (T)(pq != 0.0 ? rz / pq : 0.0);
+    double a = 0.0, b = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x; i < n; i += (int64_t)G * SV_BLOCK) {
         x[i] += alpha * p[i];
         const T ri = r[i] - alpha * q[i];
         const T zi = dinv[i] * ri;
         r[i] = ri;
         z[i] = zi;
-        rz = (double)ri * (double)zi;
-        rr = (double)ri * (double)ri;
+        a += (double)ri * (double)zi;
+        b += (double)ri * (double)ri;
     }
-    const double s1 = block_sum(rz, sh);
-    const double s2 = block_sum(rr, sh);
-    if (threadIdx.x == 0) { part[2 * blockIdx.x] = s1; part[2 * blockIdx.x + 1] = s2; }
-    if (last_block(counter)) {
-        double a = 0.0, b = 0.0;
-        for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) { a += part[2 * k]; b += part[2 * k + 1]; }
-        a = block_sum(a, sh);
-        b = block_sum(b, sh);
-        if (threadIdx.x == 0) {
-            const double rz_old = sc[S_RZ];
-            sc[S_BETA] = rz_old != 0.0 ? a / rz_old : 0.0;
-            sc[S_RZ] = a;
-            sc[S_RR] = b;
-            *counter = 0u;
-        }
-    }
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    if (threadIdx.x == 0) { rz_new[blockIdx.x] = a; part_rr[blockIdx.x] = b; }
 }
 
+// beta = rz_new / rz_old; p = z + beta p.  Workgroup 0 also leaves rz and
+// ||r||^2 for the host.
 template <typename T>
-__global__ __launch_bounds__(SV_BLOCK) void k_cg_dir(int64_t n, const double *__restrict__ sc,
-                                                     const T *__restrict__ z, T *__restrict__ p)
+__global__ __launch_bounds__(SV_BLOCK) void k_cg_dir(int64_t n, const double *__restrict__ rz_new,
+                                                     const double *__restrict__ rz_old,
+                                                     const double *__restrict__ part_rr, const T *__restrict__ z,
+                                                     T *__restrict__ p, double *__restrict__ sc)
 {
-    const T beta = (T)sc[S_BETA];
-    const int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x;
-    if (i < n) p[i] = z[i] + beta * p[i];
+    __shared__ double sh[SV_BLOCK / 64];
+    const int G = gridDim.x;
+    const double a = sum_partials(rz_new, G, sh);
+    const double b = sum_partials(rz_old, G, sh);
+    const T beta = (T)(b != 0.0 ? a / b : 0.0);
+    for (int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x; i < n; i += (int64_t)G * SV_BLOCK)
+        p[i] = z[i] + beta * p[i];
+    if (blockIdx.x == 0) {
+        const double rr = sum_partials(part_rr, G, sh);
+        if (threadIdx.x == 0) { sc[S_RZ] = a; sc[S_RR] = rr; }
+    }
 }
 
-// r = (T) r64, z = M r, and rz = r.z, rr = r.r; p = z when `restart`.
+// r = (T) r64, z = M r (p = z when `restart`); partials r.z (into rz_out) and r.r
 template <typename T>
 __global__ __launch_bounds__(SV_BLOCK) void k_cg_reset(int64_t n, const double *__restrict__ r64,
                                                        const T *__restrict__ dinv, T *__restrict__ r,
                                                        T *__restrict__ z, T *__restrict__ p, int restart,
-                                                       double *__restrict__ part, unsigned *counter,
-                                                       double *__restrict__ sc)
+                                                       double *__restrict__ rz_out, double *__restrict__ part_rr)
 {
     __shared__ double sh[SV_BLOCK / 64];
-    const int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x;
-    double rz = 0.0, rr = 0.0;
-    if (i < n) {
+    double a = 0.0, b = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * SV_BLOCK) {
         const T ri = (T)r64[i];
         const T zi = dinv[i] * ri;
         r[i] = ri;
         z[i] = zi;
         if (restart) p[i] = zi;
-        rz = (double)ri * (double)zi;
-        rr = (double)ri * (double)ri;
+        a += (double)ri * (double)zi;
+        b += (double)ri * (double)ri;
     }
-    const double s1 = block_sum(rz, sh);
-    const double s2 = block_sum(rr, sh);
-    if (threadIdx.x == 0) { part[2 * blockIdx.x] = s1; part[2 * blockIdx.x + 1] = s2; }
-    if (last_block(counter)) {
-        double a = 0.0, b = 0.0;
-        for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) { a += part[2 * k]; b += part[2 * k + 1]; }
-        a = block_sum(a, sh);
-        b = block_sum(b, sh);
-        if (threadIdx.x == 0) {
-            sc[S_RZ] = a;
-            sc[S_RR] = b;
-            *counter = 0u;
-        }
+    a = block_sum(a, sh);
+    b = block_sum(b, sh);
+    if (threadIdx.x == 0) { rz_out[blockIdx.x] = a; part_rr[blockIdx.x] = b; }
+}
+
+// One workgroup: sc[slot] = sum of g partials (and sc[slot2] of a second set).
+__global__ void k_reduce(const double *__restrict__ p, int g, double *__restrict__ sc, int slot,
+                         const double *__restrict__ p2, int slot2)
+{
+    __shared__ double sh[1024 / 64];
+    const double a = sum_partials(p, g, sh);
+    const double b = p2 ? sum_partials(p2, g, sh) : 0.0;
+    if (threadIdx.x == 0) {
+        sc[slot] = a;
+        if (p2) sc[slot2] = b;
     }
 }
 
-// x64 += (double) x32, x32 = 0 (FP32 iterate flushed into the FP64 solution);
-// r64 = b - A64 x64 and ||r64||^2 follow in k_sell_resid.
+// x64 += (double) x32, x32 = 0 (FP32 iterate flushed into the FP64 solution)
 __global__ void k_flush(int64_t n, float *__restrict__ x32, double *__restrict__ x64)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -268,60 +254,42 @@ __global__ void k_flush(int64_t n, float *__restrict__ x32, double *__restrict__
     }
 }
 
-// r = b - A x over the free block (SELL, FP64), ||r||^2 into sc[slot].
+// r = b - A x over the free block (SELL, FP64); partial ||r||^2
 __global__ __launch_bounds__(SV_BLOCK) void k_sell_resid(int64_t n, const uint32_t *__restrict__ off,
                                                          const int32_t *__restrict__ col,
                                                          const double *__restrict__ val,
                                                          const double *__restrict__ x, const double *__restrict__ b,
-                                                         double *__restrict__ r, double *__restrict__ part,
-                                                         unsigned *counter, double *__restrict__ sc, int slot)
+                                                         double *__restrict__ r, double *__restrict__ part)
 {
     __shared__ double sh[SV_BLOCK / 64];
-    const int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x;
     double rr = 0.0;
-    if (i < n) {
+    for (int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * SV_BLOCK) {
         const double v = b[i] - sell_row<double>(i, off, col, val, x);
         r[i] = v;
-        rr = v * v;
+        rr += v * v;
     }
     const double s = block_sum(rr, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
-    if (last_block(counter)) {
-        const double t = sum_partials(part, gridDim.x, sh);
-        if (threadIdx.x == 0) {
-            sc[slot] = t;
-            *counter = 0u;
-        }
-    }
 }
 
-// b = F_f - K_fd x_d with the CSR matrix (data columns only), ||b||^2.
+// b = F_f - K_fd x_d with the CSR matrix (data columns only); partial ||b||^2
 __global__ __launch_bounds__(SV_BLOCK) void k_rhs(int64_t nfree, const int32_t *__restrict__ ptr,
                                                   const int32_t *__restrict__ ind, const double *__restrict__ val,
-                                                  const double *__restrict__ xfull, const double *F,
-                                                  double *b, double *__restrict__ part, unsigned *counter,
-                                                  double *__restrict__ sc)
+                                                  const double *__restrict__ xfull, const double *F, double *b,
+                                                  double *__restrict__ part)
 {
     __shared__ double sh[SV_BLOCK / 64];
-    const int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x;
     double bb = 0.0;
-    if (i < nfree) {
+    for (int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x; i < nfree; i += (int64_t)gridDim.x * SV_BLOCK) {
         double acc = 0.0;
         for (int32_t j = ptr[i]; j < ptr[i + 1]; j++)
             if (ind[j] >= nfree) acc += val[j] * xfull[ind[j]];
         const double v = F[i] - acc;
         b[i] = v;
-        bb = v * v;
+        bb += v * v;
     }
     const double s = block_sum(bb, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
-    if (last_block(counter)) {
-        const double t = sum_partials(part, gridDim.x, sh);
-        if (threadIdx.x == 0) {
-            sc[S_BB] = t;
-            *counter = 0u;
-        }
-    }
 }
 
 __global__ void k_diag_inv(int64_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ ind,
@@ -340,12 +308,6 @@ __global__ void k_diag_inv(int64_t n, const int32_t *__restrict__ ptr, const int
     const double inv = d != 0.0 ? 1.0 / d : 1.0;
     d64[i] = inv;
     if (d32) d32[i] = (float)inv;
-}
-
-__global__ void k_fill_u32(int64_t n, uint32_t *__restrict__ a, uint32_t v)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
 }
 
 template <typename T>
@@ -374,14 +336,21 @@ const double *sell_vals<double>(const Sell &S) { return S.v64; }
 template <>
 const float *sell_vals<float>(const Sell &S) { return S.v32; }
 
-// One PCG iteration on the stream.
+struct Parts {
+    double *pq, *rz[2], *rr, *aux;
+    int G;
+};
+
+// One PCG iteration on the stream; `par` is the iteration parity (the rz
+// partials of this iteration go to rz[par], the previous ones are rz[par ^ 1]).
 template <typename T>
-void cg_step(hipStream_t st, int64_t n, const Sell &S, const Vecs<T> &v, double *part, unsigned *counter, double *sc)
+void cg_step(hipStream_t st, int64_t n, const Sell &S, const Vecs<T> &v, const Parts &P, int par, double *sc)
 {
-    const dim3 g(ceil_div(n, SV_BLOCK)), b(SV_BLOCK);
-    hipLaunchKernelGGL((k_cg_spmv<T>), g, b, 0, st, n, S.off, S.col, sell_vals<T>(S), v.p, v.q, part, counter, sc);
-    hipLaunchKernelGGL((k_cg_update<T>), g, b, 0, st, n, v.dinv, v.x, v.r, v.z, v.p, v.q, part, counter, sc);
-    hipLaunchKernelGGL((k_cg_dir<T>), g, b, 0, st, n, sc, v.z, v.p);
+    const dim3 g(P.G), b(SV_BLOCK);
+    hipLaunchKernelGGL((k_cg_spmv<T>), g, b, 0, st, n, S.off, S.col, sell_vals<T>(S), v.p, v.q, P.pq);
+    hipLaunchKernelGGL((k_cg_update<T>), g, b, 0, st, n, v.dinv, v.x, v.r, v.z, v.p, v.q, P.pq, P.rz[par ^ 1],
+                       P.rz[par], P.rr);
+    hipLaunchKernelGGL((k_cg_dir<T>), g, b, 0, st, n, P.rz[par], P.rz[par ^ 1], P.rr, v.z, v.p, sc);
 }
 
 }  // namespace
@@ -401,27 +370,30 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
     }
     const bool mixed = precision == ELFEL_MIXED;
     const int64_t nslices = (n + 63) / 64;
-    const int nb = ceil_div(n, SV_BLOCK);
     hipStream_t st = c->stream;
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    // the CG grid: 4 workgroups per CU (grid-stride loops), fewer for small systems
+    const int G = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, SV_BLOCK), 4 * (int64_t)prop.multiProcessorCount));
 
     // ---- scratch layout (8-byte words first, then 4-byte) -----------------
-    // doubles: xfull[N], b[n], r64[n], dinv64[n], part[2 nb + 8], sc[S_NS],
-    //          fp64 vectors r z p q [4n] (FP64 solve only)
-    // uint32:  counter[2], off[nslices + 1], len/size scratch[n]
-    // after the solve-independent part: SELL col[nnzp] (int32), v64[nnzp], v32[nnzp]
-    const size_t nd = (size_t)N + 3 * (size_t)n + 2 * (size_t)nb + 8 + S_NS + (mixed ? 0 : 4 * (size_t)n);
-    const size_t nf = mixed ? 6 * (size_t)n : 0;  // x32 r z p q dinv32
-    const size_t nu32 = 4 + (size_t)nslices + 1 + (size_t)std::max<int64_t>(n, nslices);
-    size_t bytes = 8 * nd + 4 * nf + 4 * nu32 + 256;
-    int rc = ensure(c, c->pcg, bytes);
+    // doubles: xfull[N], b[n], r64[n], dinv64[n], parts[5 G], sc[S_NS],
+    //          FP64 vectors r z p q [4n] (FP64 solve only)
+    // floats:  x r z p q dinv [6n] (mixed solve only)
+    // uint32:  off[nslices + 1], len[max(n, nslices)], total[4]
+    const size_t nd = (size_t)N + 3 * (size_t)n + 5 * (size_t)G + S_NS + (mixed ? 0 : 4 * (size_t)n);
+    const size_t nf = mixed ? 6 * (size_t)n : 0;
+    const size_t nu32 = (size_t)nslices + 1 + (size_t)std::max<int64_t>(n, nslices) + 4;
+    int rc = ensure(c, c->pcg, 8 * nd + 4 * nf + 4 * nu32 + 256);
     if (rc) return rc;
-    char *base = (char *)c->pcg.p;
-    double *dp = (double *)base;
+    double *dp = (double *)c->pcg.p;
     double *xfull = dp; dp += N;
     double *b = dp; dp += n;
     double *r64 = dp; dp += n;
     double *dinv64 = dp; dp += n;
-    double *part = dp; dp += 2 * (size_t)nb + 8;
+    Parts P;
+    P.G = G;
+    P.pq = dp; dp += G; P.rz[0] = dp; dp += G; P.rz[1] = dp; dp += G; P.rr = dp; dp += G; P.aux = dp; dp += G;
     double *sc = dp; dp += S_NS;
     Vecs<double> v64{};
     if (!mixed) {
@@ -436,24 +408,22 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
         v32.dinv = fp; fp += n;
     }
     uint32_t *up = (uint32_t *)fp;
-    unsigned *counter = up; up += 4;
     uint32_t *off = up; up += nslices + 1;
-    uint32_t *len = up;
+    uint32_t *len = up; up += std::max<int64_t>(n, nslices);
+    uint32_t *total = up;
 
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (int64_t)4, counter, 0u);
     HIPCHK(c, hipMemcpyAsync(xfull, x, 8 * N, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemcpyAsync(b, F, 8 * n, hipMemcpyHostToDevice, st));
 
     // ---- SELL-64 of the free block ------------------------------------------
     hipLaunchKernelGGL(k_sell_rowlen, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, ptr, ind, len);
     hipLaunchKernelGGL(k_sell_width, dim3(ceil_div(nslices * 64, 256)), dim3(256), 0, st, n, nslices, len, off);
-    uint32_t *total = counter + 2;
     if ((rc = exclusive_scan_u32(c, off, nslices, total))) return rc;
     uint32_t nnzp = 0;
     HIPCHK(c, hipMemcpyAsync(&nnzp, total, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
     HIPCHK(c, hipMemcpyAsync(off + nslices, total, 4, hipMemcpyDeviceToDevice, st));
-    // SELL arrays in a second buffer (grown separately: the size is known only now)
+    // SELL arrays in a buffer of their own (the size is known only now)
     const size_t sbytes = (size_t)nnzp * (4 + 8 + (mixed ? 4 : 0)) + 256;
     if ((rc = ensure(c, c->pcg_sell, sbytes))) return rc;
     int32_t *scol = (int32_t *)c->pcg_sell.p;
@@ -466,68 +436,81 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
     HIPCHK(c, hipGetLastError());
     const Sell S{off, scol, sv64, sv32};
 
-    // b = F_f - K_fd x_d, ||b||
-    hipLaunchKernelGGL(k_rhs, dim3(nb), dim3(SV_BLOCK), 0, st, n, ptr, ind, val, xfull, b, b, part, counter, sc);
-    HIPCHK(c, hipGetLastError());
-    auto true_residual = [&](double *out_norm) -> int {
-        hipLaunchKernelGGL(k_sell_resid, dim3(nb), dim3(SV_BLOCK), 0, st, n, S.off, S.col, S.v64,
-                           (const double *)xfull, (const double *)b, r64, part, counter, sc, (int)S_RR64);
-        double h[S_NS];
-        HIPCHK(c, hipMemcpyAsync(h, sc, sizeof(h), hipMemcpyDeviceToHost, st));
+    auto read_sc = [&](double *h) -> int {
+        HIPCHK(c, hipMemcpyAsync(h, sc, sizeof(double) * S_NS, hipMemcpyDeviceToHost, st));
         HIPCHK(c, hipStreamSynchronize(st));
-        *out_norm = sqrt(std::max(h[S_RR64], 0.0));
         return 0;
     };
-    double hbb[S_NS];
-    HIPCHK(c, hipMemcpyAsync(hbb, sc, sizeof(hbb), hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipStreamSynchronize(st));
-    const double bnorm = hbb[S_BB] > 0 ? sqrt(hbb[S_BB]) : 1.0;
+    // b = F_f - K_fd x_d, ||b||
+    hipLaunchKernelGGL(k_rhs, dim3(G), dim3(SV_BLOCK), 0, st, n, ptr, ind, val, (const double *)xfull,
+                       (const double *)b, b, P.aux);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, (const double *)P.aux, G, sc, (int)S_BB,
+                       (const double *)nullptr, 0);
+    HIPCHK(c, hipGetLastError());
+    double h[S_NS];
+    if ((rc = read_sc(h))) return rc;
+    const double bnorm = h[S_BB] > 0 ? sqrt(h[S_BB]) : 1.0;
     const double target = tol * bnorm;
+    // r64 = b - A x (FP64), its norm
+    auto true_residual = [&](double *out_norm) -> int {
+        hipLaunchKernelGGL(k_sell_resid, dim3(G), dim3(SV_BLOCK), 0, st, n, S.off, S.col, S.v64,
+                           (const double *)xfull, (const double *)b, r64, P.aux);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, (const double *)P.aux, G, sc, (int)S_RR64,
+                           (const double *)nullptr, 0);
+        HIPCHK(c, hipGetLastError());
+        double hh[S_NS];
+        int r2 = read_sc(hh);
+        if (r2) return r2;
+        *out_norm = sqrt(std::max(hh[S_RR64], 0.0));
+        return 0;
+    };
+    // r = (T) r64, z = M r (p = z on restart); the rz partials go where the next
+    // step reads the previous ones (rz[par ^ 1]); sc gets rz and ||r||^2
+    int par = 0;
+    auto reset = [&](auto &v, int restart) {
+        using T = typename std::remove_pointer<decltype(v.r)>::type;
+        hipLaunchKernelGGL((k_cg_reset<T>), dim3(G), dim3(SV_BLOCK), 0, st, n, (const double *)r64,
+                           (const T *)v.dinv, v.r, v.z, v.p, restart, P.rz[par ^ 1], P.rr);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, (const double *)P.rr, G, sc, (int)S_RR,
+                           (const double *)P.rz[par ^ 1], (int)S_RZ);
+    };
 
     int it = 0, nupd = 0;
     double rn = 0.0;
     if ((rc = true_residual(&rn))) return rc;
     if (!mixed) {
-        hipLaunchKernelGGL((k_cg_reset<double>), dim3(nb), dim3(SV_BLOCK), 0, st, n, (const double *)r64,
-                           (const double *)v64.dinv, v64.r, v64.z, v64.p, 1, part, counter, sc);
+        reset(v64, 1);
         nupd = 1;
         while (rn > target && it < maxit) {
             // iterate until the recurrence residual reaches the target ...
             double rrec = 0.0;
             for (;;) {
-                double h[S_NS];
-                HIPCHK(c, hipMemcpyAsync(h, sc, sizeof(h), hipMemcpyDeviceToHost, st));
-                HIPCHK(c, hipStreamSynchronize(st));
+                if ((rc = read_sc(h))) return rc;
                 rrec = h[S_RR];
                 if (!(rrec > target * target) || it >= maxit) break;
-                for (int k = 0; k < CHECK; k++) cg_step<double>(st, n, S, v64, part, counter, sc);
+                for (int k = 0; k < CHECK; k++, par ^= 1) cg_step<double>(st, n, S, v64, P, par, sc);
                 HIPCHK(c, hipGetLastError());
                 it += CHECK;
             }
             // ... then check the true residual and continue from it if needed
             if ((rc = true_residual(&rn))) return rc;
             if (rn > target && it < maxit) {
-                const int restart = !(sqrt(std::max(rrec, 0.0)) >= 0.5 * rn);
-                hipLaunchKernelGGL((k_cg_reset<double>), dim3(nb), dim3(SV_BLOCK), 0, st, n, (const double *)r64,
-                                   (const double *)v64.dinv, v64.r, v64.z, v64.p, restart, part, counter, sc);
+                reset(v64, !(sqrt(std::max(rrec, 0.0)) >= 0.5 * rn));
                 nupd++;
             }
         }
     } else {
         hipLaunchKernelGGL(k_fill<float>, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, v32.x, 0.f);
-        hipLaunchKernelGGL((k_cg_reset<float>), dim3(nb), dim3(SV_BLOCK), 0, st, n, (const double *)r64,
-                           (const float *)v32.dinv, v32.r, v32.z, v32.p, 1, part, counter, sc);
+        reset(v32, 1);
         double rlast = rn;
         while (rn > target && it < maxit) {
             const double goal = std::max(target, RU_DROP * rlast);
             double rrec = 0.0;  // recurrence residual norm^2 at the last check
             for (;;) {
-                double h[S_NS];
-                HIPCHK(c, hipMemcpyAsync(h, sc, sizeof(h), hipMemcpyDeviceToHost, st));
-                HIPCHK(c, hipStreamSynchronize(st));
+                if ((rc = read_sc(h))) return rc;
                 rrec = h[S_RR];
                 if (!(rrec > goal * goal) || it >= maxit) break;
-                for (int k = 0; k < CHECK; k++) cg_step<float>(st, n, S, v32, part, counter, sc);
+                for (int k = 0; k < CHECK; k++, par ^= 1) cg_step<float>(st, n, S, v32, P, par, sc);
                 HIPCHK(c, hipGetLastError());
                 it += CHECK;
             }
@@ -539,10 +522,7 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
             // The search direction is kept while the recurrence still tracks the true residual; once
             // it has run far below it (FP32 round-off, or a tiny system solved exactly) p no longer
             // belongs to the new residual and the iteration restarts from p = z.
-            const int restart = !(sqrt(std::max(rrec, 0.0)) >= 0.5 * rn);
-            if (rn > target && it < maxit)
-                hipLaunchKernelGGL((k_cg_reset<float>), dim3(nb), dim3(SV_BLOCK), 0, st, n, (const double *)r64,
-                                   (const float *)v32.dinv, v32.r, v32.z, v32.p, restart, part, counter, sc);
+            if (rn > target && it < maxit) reset(v32, !(sqrt(std::max(rrec, 0.0)) >= 0.5 * rn));
         }
     }
     HIPCHK(c, hipGetLastError());
