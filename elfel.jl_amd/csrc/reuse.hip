@@ -83,7 +83,7 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
 {
     const int nen = F0.nen, nc = F0.ncomp, nd = nen * nc;
     const int ncol = nd + (stokes ? 3 : 0);
-    const int ncolp = (ncol + 3) & ~3;
+    const int ncolp = (ncol + 2 + 3) & ~3;  // positions, then a 16-bit first-contribution mask in the last two bytes
     const int vcap = std::min(std::max(env_int("ELFEL_HIP_VCAP", REUSE_VCAP_PER_THREAD), 8), 24) * cap;
     Adj A0, A1;
     adjacency(F0, A0);
@@ -147,6 +147,7 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     };
     int64_t r = 0;
     int32_t colbuf[16];
+    std::vector<uint8_t> seen;
     while (r < n) {
         const int32_t node = inv_node[r];
         const int field = inv_fc[r] >> 4, comp = inv_fc[r] & 15;
@@ -175,6 +176,7 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
         const int rowcode = nr == 2 ? 2 : (comp == 1 ? 1 : 0);
         const int l8 = nr == 2 ? L : cr.y;
         const int32_t *list = hind.data() + hptr[r];
+        seen.assign(L, 0);
         for (int s = 0; s < deg; s++) {
             const int32_t e = A->elem[A->ptr[node] + s];
             const int a = A->loc[A->ptr[node] + s];
@@ -184,16 +186,20 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
                     colbuf[nj++] = F0.h_dof[(int64_t)F0.h_conn[(int64_t)e * nen + k] * nc + cc];
             if (stokes && field == 0)
                 for (int k = 0; k < 3; k++) colbuf[nj++] = F1->h_dof[F1->h_conn[(int64_t)e * 3 + k]];
-            for (int j = 0; j < ncolp; j++) {
+            uint32_t first = 0;  // bit j: this task is the first (lowest rank) to add into column j's entry
+            for (int j = 0; j < ncolp - 2; j++) {
                 uint8_t pv = 0;
                 if (j < nj) {
                     const int32_t *it = std::lower_bound(list, list + L, colbuf[j]);
                     if (it == list + L || *it != colbuf[j])
                         return set_err(c, ELFEL_EINVAL, "internal: element column missing from the pattern");
                     pv = (uint8_t)(it - list);
+                    if (!seen[pv]) { seen[pv] = 1; first |= 1u << j; }
                 }
                 T.pos.push_back(pv);
             }
+            T.pos.push_back((uint8_t)(first & 0xFF));
+            T.pos.push_back((uint8_t)(first >> 8));
             T.elem.push_back(e);
             T.info.push_back((uint32_t)a | ((uint32_t)s << 3) | ((uint32_t)rowcode << 7) |
                              ((uint32_t)(field == 1) << 9) | ((uint32_t)vbase << 10) | ((uint32_t)l8 << 24));

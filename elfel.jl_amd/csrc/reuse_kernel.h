@@ -21,7 +21,9 @@
 // A tile holds exactly BLOCK task slots (tile t's tasks start at t * BLOCK);
 // unused slots carry TASK_PAD (rows code 3) and element 0, and add nothing.
 // task_pos[task][j]: position of element column j in the owner's sorted
-// column list (uint8).
+// column list (uint8); the row's last two bytes hold a mask whose bit j says
+// that this task is the first to add into column j's entry (it stores instead
+// of adding, so the tile's sums need no zero fill).
 #pragma once
 #include "elem.h"
 
@@ -254,12 +256,6 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
     const int tid = threadIdx.x;
     const int4 td = A.tiles[tile];
     const int4 tr = A.tile_rows[tile];
-    {   // zero the tile's sums, two per lane (the LDS block is 16-byte aligned and sized to an even count)
-        double2 *a2 = (double2 *)acc;
-        const int n2 = (td.w + 1) >> 1;
-#pragma unroll 4
-        for (int v = tid; v < n2; v += BLOCK) a2[v] = make_double2(0.0, 0.0);
-    }
     if constexpr (KIND == ELFEL_HEAT)
         for (int v = tid; v < tr.y; v += BLOCK) accF[v] = 0.0;
 
@@ -272,8 +268,9 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
     bool pown = false;
     R out[2][ND + 3];
     R fo = R(0);
-    // column positions, prefetched as 32-bit words before the barriered adds
-    constexpr int NPW = (NCOL + 3) / 4;
+    // column positions and the first-contribution mask, prefetched as 32-bit
+    // words before the barriered adds
+    constexpr int NPW = (NCOL + 2 + 3) / 4;
     uint32_t posw[NPW];
     if (active) {
         const uint32_t *pw = (const uint32_t *)(A.task_pos + task * A.ncolp);
@@ -331,6 +328,7 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
             // LDS round trip per rank instead of one per entry.
             const int ncols = (STOKES && pown) ? ND : NCOL;
             const int nr = rowcode == 2 ? 2 : 1;
+            const uint32_t fmask = posw[NPW - 1] >> 16;
 #pragma unroll
             for (int c = 0; c < 2; c++) {
                 if (c >= nr) continue;
@@ -338,7 +336,7 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
                 double cur[NCOL];
 #pragma unroll
                 for (int j = 0; j < NCOL; j++)
-                    if (j < ncols) cur[j] = row[(posw[j >> 2] >> (8 * (j & 3))) & 0xFF];
+                    if (j < ncols) cur[j] = ((fmask >> j) & 1u) ? 0.0 : row[(posw[j >> 2] >> (8 * (j & 3))) & 0xFF];
 #pragma unroll
                 for (int j = 0; j < NCOL; j++)
                     if (j < ncols) row[(posw[j >> 2] >> (8 * (j & 3))) & 0xFF] = cur[j] + (double)out[c][j];
@@ -439,7 +437,7 @@ template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC, int BLOCK>
 int launch_reuse(elfel_hip_ctx *c, const ReuseArgs &A, const Coef &cf, const QTable &tab, int lds_vals)
 {
     if (A.ntiles <= 0) return 0;
-    lds_vals = (lds_vals + 1) & ~1;  // even: the zero fill writes pairs, the load-vector sums stay 16-byte aligned
+    lds_vals = (lds_vals + 1) & ~1;  // even: the load-vector sums stay 16-byte aligned
     const size_t lds = sizeof(double) * ((size_t)lds_vals + (KIND == ELFEL_HEAT ? BLOCK : 0));
     hipLaunchKernelGGL((k_reuse<KIND, R, NEN, NC, NQ, CSC, BLOCK>), dim3(A.ntiles), dim3(BLOCK), lds, c->stream, A,
                        cf, tab, lds_vals);

@@ -11,11 +11,14 @@
 // 1996): whenever the recurrence residual has fallen by RU_DROP since the last
 // update, the FP32 iterate is added into the FP64 solution, the FP64 residual
 // b - A x is formed with the FP64 matrix and replaces the FP32 one, and the
-// search direction is kept (unless the recurrence has drifted far below the
-// true residual: then p restarts from z).  The FP32 iterations keep their
-// Krylov space, so they take about as many steps as FP64 PCG (+9% on heat Q4
-// 256x256 with lognormal conductivity, in a numpy emulation of this code) at
-// about 0.6 of the bytes each.
+// search direction is kept unless the recurrence has drifted below half the
+// true residual (then p restarts from z).  An FP32 iteration moves about 0.6
+// of the bytes of an FP64 one.  The FP32 recurrence loses accuracy as the
+// condition number grows, so the iteration count rises with the mesh: on heat
+// Q4 with lognormal(0, 1) conductivity, +9% over FP64 PCG at 256x256 and +56%
+// at 1024x1024 (numpy emulation of this code), +45% at 4096x4096 (measured:
+// 19224 against 13280 iterations).  Updates at every 10x drop, or never
+// restarting, fail to converge at 4096x4096.
 // Convergence is declared on the FP64 residual: ||b - A x|| / ||b|| <= tol.
 //
 // Storage: the free-free block in SELL-64 (one slice per 64 consecutive rows,
@@ -39,7 +42,12 @@
 
 #define SV_BLOCK 256
 #define CHECK 8
+#ifndef RU_DROP
 #define RU_DROP 1e-2   // reliable update when the recurrence residual has dropped by this factor
+#endif
+#ifndef RU_KEEP
+#define RU_KEEP 0.5    // keep p at an update while ||r recurrence|| >= RU_KEEP ||r true||
+#endif
 
 namespace {
 
@@ -495,7 +503,7 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
             // ... then check the true residual and continue from it if needed
             if ((rc = true_residual(&rn))) return rc;
             if (rn > target && it < maxit) {
-                reset(v64, !(sqrt(std::max(rrec, 0.0)) >= 0.5 * rn));
+                reset(v64, !(sqrt(std::max(rrec, 0.0)) >= RU_KEEP * rn));
                 nupd++;
             }
         }
@@ -522,7 +530,7 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
             // The search direction is kept while the recurrence still tracks the true residual; once
             // it has run far below it (FP32 round-off, or a tiny system solved exactly) p no longer
             // belongs to the new residual and the iteration restarts from p = z.
-            if (rn > target && it < maxit) reset(v32, !(sqrt(std::max(rrec, 0.0)) >= 0.5 * rn));
+            if (rn > target && it < maxit) reset(v32, !(sqrt(std::max(rrec, 0.0)) >= RU_KEEP * rn));
         }
     }
     HIPCHK(c, hipGetLastError());
