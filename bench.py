@@ -212,6 +212,7 @@ def bench_single(args, L, W):
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": cs["kernel"], "kernel_ms": kern_ms, "algorithmic_bytes": kernel_bytes}
+    roofline.update(measured_traffic(args.config, args.N, cs["N"]))
 
     extra = {}
     if args.config == "c5" and not args.no_pcg:
@@ -250,6 +251,27 @@ def bench_single(args, L, W):
     }
     line.update(extra)
     print(json.dumps(line))
+
+
+TRAFFIC_FILES = {"c2": "k_reuse_elasticity_t6_400x400_traffic.json",
+                 "c3": "k_reuse_stokes_t6t3_500x500_traffic.json",
+                 "c5": "k_reuse_heat_q4_4096x4096_and_pcg_traffic.json"}
+
+
+def measured_traffic(cfgname, N_arg, N):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs of the same workload (scaled by the
+    device-copy calibration), when the workload is the profiled size."""
+    f = TRAFFIC_FILES.get(cfgname)
+    if not f or N_arg:
+        return {}
+    path = os.path.join(ROOT, "profiles", f)
+    if not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        t = json.load(fh)
+    return {"traffic": t["traffic_bytes"], "traffic_fetch": t["fetch_bytes"], "traffic_write": t["write_bytes"],
+            "traffic_source": t["source"] + " (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, x2 / x1 copy calibration)"}
 
 
 def bench_pcg(ctx, L, cs, res, args_maxit=60000):

@@ -132,8 +132,11 @@ int elfel_hip_geometry_stats(elfel_hip_ctx *ctx, double *min_det, int64_t *n_bad
  * K[1:nu,1:nu] \ (F - K*T)[1:nu]) for SPD problems with Jacobi-PCG on the
  * matrix of the last assembly (CSR).  x holds the data values on entry
  * (x[nfree:]) and the solution on return.  precision = ELFEL_FP64: FP64 PCG;
- * ELFEL_MIXED: FP32 PCG inner iterations inside FP64 iterative refinement.
- * Convergence: ||b - A x|| / ||b|| <= tol in FP64 on the free block. */
+ * ELFEL_MIXED: FP32 PCG iterations with reliable updates (the FP32 iterate is
+ * added into the FP64 solution and the FP64 residual replaces the FP32 one
+ * at every 100x drop).  Convergence: ||b - A x|| / ||b|| <= tol in FP64 on
+ * the free block.  *iters = PCG iterations, *outer = FP64 residual
+ * replacements, *relres = the final FP64 relative residual. */
 int elfel_hip_pcg(elfel_hip_ctx *ctx, int64_t nfree, const double *F, double *x, int precision,
                   double tol, int maxit, int *iters, int *outer, double *relres);
 
