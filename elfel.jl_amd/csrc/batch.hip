@@ -195,7 +195,7 @@ static int batch_launch(elfel_hip_ctx *c, Batch *b, int block, int grid)
     const int nt = AH.ntiles + AE.ntiles;
     if (nt == 0) return 0;
     int lds_vals = std::max(H.E ? (int)H.tab[bi].lds_vals : 0, Ev.E ? (int)Ev.tab[bi].lds_vals : 0);
-    lds_vals = (lds_vals + 1) & ~1;  // even: the load-vector sums stay 16-byte aligned
+    lds_vals = (lds_span(lds_vals) + 1) & ~1;  // even: the load-vector sums stay 16-byte aligned
     const size_t lds = sizeof(double) * ((size_t)lds_vals + block);
     const int g = grid > 0 ? std::min(grid, nt) : nt;
     const bool csc = b->layout == ELFEL_CSC;

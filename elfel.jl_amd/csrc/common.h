@@ -56,7 +56,7 @@ struct Pattern {
     int ncolmax = 0;             // bytes of pos per task
     int maxrank = 0;
     int lds_vals = 0;            // largest tile value count
-    int block = 0;               // threads per workgroup = task capacity of a tile
+    int block = 0;               // task slots per tile (threads per workgroup x tasks per thread)
     double symbolic_ms = 0;      // host wall time of the last symbolic build
     Buf ptr, ind;                // CSR pattern (int32)
     Buf tile;                    // int4 per tile: task0, ntasks, v0, nvals | (row0 in .w)

@@ -83,7 +83,7 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
 {
     const int nen = F0.nen, nc = F0.ncomp, nd = nen * nc;
     const int ncol = nd + (stokes ? 3 : 0);
-    const int ncolp = (ncol + 2 + 3) & ~3;  // positions, then a 16-bit first-contribution mask in the last two bytes
+    const int ncolp = (ncol + 3) & ~3;  // one byte per column: position (bits 0-6), first contribution (bit 7)
     const int vcap = std::min(std::max(env_int("ELFEL_HIP_VCAP", REUSE_VCAP_PER_THREAD), 8), 24) * cap;
     Adj A0, A1;
     adjacency(F0, A0);
@@ -164,9 +164,10 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
             A = field == 0 ? &A0 : &A1;
             deg = A->ptr[node + 1] - A->ptr[node];
         }
-        if (deg > cap || nr * L > vcap || deg > 16 || L > 255)
+        if (deg > cap || nr * L > vcap || deg > 16 || L > 128)
             return set_err(c, ELFEL_ENOTSUP, "node too highly connected for the reuse tiles");
-        if (open && (cur.y + deg > cap || cur.w + nr * L > vcap || cr.y + nr > cap)) close();
+        if (open && (cur.y + deg > cap || cur.w + nr * L > vcap || cr.y + nr > std::min(cap, REUSE_MAX_TILE_ROWS)))
+            close();
         if (!open) {
             cur = int4{(int)T.elem.size(), 0, hptr[r], 0};
             cr = int4{(int)r, 0, 0, 0};
@@ -186,20 +187,18 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
                     colbuf[nj++] = F0.h_dof[(int64_t)F0.h_conn[(int64_t)e * nen + k] * nc + cc];
             if (stokes && field == 0)
                 for (int k = 0; k < 3; k++) colbuf[nj++] = F1->h_dof[F1->h_conn[(int64_t)e * 3 + k]];
-            uint32_t first = 0;  // bit j: this task is the first (lowest rank) to add into column j's entry
-            for (int j = 0; j < ncolp - 2; j++) {
+            for (int j = 0; j < ncolp; j++) {
                 uint8_t pv = 0;
                 if (j < nj) {
                     const int32_t *it = std::lower_bound(list, list + L, colbuf[j]);
                     if (it == list + L || *it != colbuf[j])
                         return set_err(c, ELFEL_EINVAL, "internal: element column missing from the pattern");
                     pv = (uint8_t)(it - list);
-                    if (!seen[pv]) { seen[pv] = 1; first |= 1u << j; }
+                    // bit 7: this task is the first (lowest rank) to add into the entry
+                    if (!seen[pv]) { seen[pv] = 1; pv |= 0x80; }
                 }
                 T.pos.push_back(pv);
             }
-            T.pos.push_back((uint8_t)(first & 0xFF));
-            T.pos.push_back((uint8_t)(first >> 8));
             T.elem.push_back(e);
             T.info.push_back((uint32_t)a | ((uint32_t)s << 3) | ((uint32_t)rowcode << 7) |
                              ((uint32_t)(field == 1) << 9) | ((uint32_t)vbase << 10) | ((uint32_t)l8 << 24));
@@ -299,10 +298,11 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
     HIPCHK(c, hipMemcpyAsync(P.ptr.p, c->ptr.p, 4 * (n + 1), hipMemcpyDeviceToDevice, c->stream));
     if (nnz) HIPCHK(c, hipMemcpyAsync(P.ind.p, c->ind.p, 4 * nnz, hipMemcpyDeviceToDevice, c->stream));
     ReuseTables T;
-    if ((rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, block, T))) return rc;
-    if (env_int("ELFEL_HIP_TILE_ORDER", 1)) order_tiles(T, 8, block);
+    const int cap = block * reuse_tpt(pb->kind);  // task slots per tile
+    if ((rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, cap, T))) return rc;
+    if (env_int("ELFEL_HIP_TILE_ORDER", 1)) order_tiles(T, 8, cap);
     if ((rc = upload_tables(c, T, P))) return rc;
-    P.block = block;
+    P.block = cap;
     P.kind = pb->kind; P.fe = pb->fe; P.layout = pb->layout;
     P.key_mesh0 = F0.mesh_version; P.key_dof0 = F0.dof_version;
     P.key_mesh1 = stokes ? F1->mesh_version : 0; P.key_dof1 = stokes ? F1->dof_version : 0;
@@ -328,7 +328,7 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
                                : pick_reuse64(pb->kind, pb->fe, tab.nq, pb->layout == ELFEL_CSC, block);
     if (!fn) return set_err(c, ELFEL_ENOTSUP, "unsupported (kind, element, rule) for the reuse method");
     int rc;
-    if (!fresh || P.block != block) {
+    if (!fresh || P.block != block * reuse_tpt(pb->kind)) {
         hipEvent_t ev = nullptr;
         tm_begin(c, ELFEL_PH_SYMBOLIC, &ev);
         rc = build_symbolic(c, pb, tab, cf, block);

@@ -21,9 +21,9 @@
 // A tile holds exactly BLOCK task slots (tile t's tasks start at t * BLOCK);
 // unused slots carry TASK_PAD (rows code 3) and element 0, and add nothing.
 // task_pos[task][j]: position of element column j in the owner's sorted
-// column list (uint8); the row's last two bytes hold a mask whose bit j says
-// that this task is the first to add into column j's entry (it stores instead
-// of adding, so the tile's sums need no zero fill).
+// column list (bits 0-6, rows of at most 128 entries); bit 7 says that this
+// task is the first to add into that entry (it stores instead of adding, so
+// the tile's sums need no zero fill).
 #pragma once
 #include "elem.h"
 
@@ -242,8 +242,20 @@ __device__ __forceinline__ void owner_values(const QTable &tab, const Geo<R, NEN
     }
 }
 
-// One tile: a workgroup of BLOCK threads, one task per thread.
-template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC, int BLOCK>
+#ifndef REUSE_LDS_SKEW
+#define REUSE_LDS_SKEW 0
+#endif
+// LDS slot of tile value v: with the skew, one spare slot after every 32
+// values spreads the strided accesses of a rank step over more banks.
+__device__ __forceinline__ int lds_slot(int v) { return REUSE_LDS_SKEW ? v + (v >> 5) : v; }
+__host__ __device__ constexpr int lds_span(int nvals) { return REUSE_LDS_SKEW ? nvals + (nvals >> 5) + 1 : nvals; }
+
+// One tile: a workgroup of BLOCK threads, TPT tasks per thread (task slot
+// u * BLOCK + tid).  The loads of all of a thread's tasks are issued stage by
+// stage (task records, then node numbers and coefficients, then coordinates)
+// before any arithmetic, so light tasks (heat: one row, NEN columns) overlap
+// their memory latency.
+template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC, int BLOCK, int TPT>
 __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, const QTable &tab, double *lds,
                                            int lds_vals, int tile)
 {
@@ -251,6 +263,7 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
     constexpr bool STOKES = (KIND == ELFEL_STOKES_GEN || KIND == ELFEL_STOKES_VECLAP);
     constexpr int NCOL = ND + (STOKES ? 3 : 0);
     constexpr bool MIXED = sizeof(R) == 4;
+    constexpr int SLOTS = BLOCK * TPT;
     double *acc = lds;
     double *accF = lds + lds_vals;
     const int tid = threadIdx.x;
@@ -260,88 +273,120 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
         for (int v = tid; v < tr.y; v += BLOCK) accF[v] = 0.0;
 
     // every slot holds a task record; pad slots skip the arithmetic and add nothing
-    const int64_t task = (int64_t)tile * BLOCK + tid;
-    const int32_t e = A.task_elem[task];
-    const uint32_t info = A.task_info[task];
-    const bool active = ((info >> 7) & 3) != 3;
-    int rank = -1, rowcode = 0, vbase = 0, L8 = 0;
-    bool pown = false;
-    R out[2][ND + 3];
-    R fo = R(0);
-    // column positions and the first-contribution mask, prefetched as 32-bit
+    int64_t task[TPT];
+    uint32_t info[TPT];
+    int32_t e[TPT];
+    bool active[TPT];
+#pragma unroll
+    for (int u = 0; u < TPT; u++) {
+        task[u] = (int64_t)tile * SLOTS + u * BLOCK + tid;
+        e[u] = A.task_elem[task[u]];
+        info[u] = A.task_info[task[u]];
+    }
+    // column positions and first-contribution flags, prefetched as 32-bit
     // words before the barriered adds
-    constexpr int NPW = (NCOL + 2 + 3) / 4;
-    uint32_t posw[NPW];
-    if (active) {
-        const uint32_t *pw = (const uint32_t *)(A.task_pos + task * A.ncolp);
+    constexpr int NPW = (NCOL + 3) / 4;
+    uint32_t posw[TPT][NPW];
 #pragma unroll
-        for (int w = 0; w < NPW; w++) posw[w] = pw[w];
-        const int a = info & 7;
-        rank = (info >> 3) & 15;
-        rowcode = (info >> 7) & 3;
-        const bool powner = STOKES && ((info >> 9) & 1);
-        pown = powner;
-        vbase = (info >> 10) & 0x3FFF;
-        L8 = info >> 24;
-        const int comp0 = rowcode == 1 ? 1 : 0;
-        const int nr = rowcode == 2 ? 2 : 1;
-        const double sc = cf.scale_e ? cf.scale_e[e] : 1.0;
-        const double kapd = cf.kappa_e ? cf.kappa_e[e] : cf.kappa;
-        const double Qd = cf.Q_e ? cf.Q_e[e] : cf.Q;
-        // Geometry in FP64 from the gathered records (absolute coordinates for the
-        // FP64 kernel, coordinates relative to the element's first node for the
-        // mixed kernel).
-        double X[NEN], Y[NEN];
+    for (int u = 0; u < TPT; u++) {
+        active[u] = ((info[u] >> 7) & 3) != 3;
+        const uint32_t *pw = (const uint32_t *)(A.task_pos + task[u] * A.ncolp);
+#pragma unroll
+        for (int w = 0; w < NPW; w++) posw[u][w] = pw[w];
+    }
+    // node numbers (gather fused in) and per-element coefficients
+    int32_t nd[TPT][NEN];
+    double sc[TPT], kapd[TPT], Qd[TPT];
+#pragma unroll
+    for (int u = 0; u < TPT; u++) {
+        sc[u] = 1.0; kapd[u] = cf.kappa; Qd[u] = cf.Q;
+        if (active[u]) {
+            if (!A.geom) {
+#pragma unroll
+                for (int k = 0; k < NEN; k++) nd[u][k] = A.conn[(int64_t)e[u] * NEN + k];
+            }
+            if (cf.scale_e) sc[u] = cf.scale_e[e[u]];
+            if (cf.kappa_e) kapd[u] = cf.kappa_e[e[u]];
+            if (cf.Q_e) Qd[u] = cf.Q_e[e[u]];
+        }
+    }
+    // coordinates: FP64, absolute for the FP64 kernel, relative to the
+    // element's first node for the mixed kernel
+    double X[TPT][NEN], Y[TPT][NEN];
+#pragma unroll
+    for (int u = 0; u < TPT; u++) {
+        if (!active[u]) continue;
         if (A.geom) {
-            const double *gp = (const double *)A.geom + (int64_t)e * (2 * NEN);
+            const double *gp = (const double *)A.geom + (int64_t)e[u] * (2 * NEN);
 #pragma unroll
-            for (int k = 0; k < NEN; k++) { X[k] = gp[2 * k]; Y[k] = gp[2 * k + 1]; }
+            for (int k = 0; k < NEN; k++) { X[u][k] = gp[2 * k]; Y[u][k] = gp[2 * k + 1]; }
         } else {
-            // gather fused in: the element's node numbers, then its coordinates
-            int32_t nd[NEN];
 #pragma unroll
-            for (int k = 0; k < NEN; k++) nd[k] = A.conn[(int64_t)e * NEN + k];
+            for (int k = 0; k < NEN; k++) { const double2 p = A.xy[nd[u][k]]; X[u][k] = p.x; Y[u][k] = p.y; }
+        }
+    }
+    int rank[TPT], rowcode[TPT], vbase[TPT], L8[TPT];
+    bool pown[TPT];
+    R out[TPT][2][ND + 3];
+    R fo[TPT];
 #pragma unroll
-            for (int k = 0; k < NEN; k++) { const double2 p = A.xy[nd[k]]; X[k] = p.x; Y[k] = p.y; }
-            if constexpr (MIXED) {
-                const double x0 = X[0], y0 = Y[0];
+    for (int u = 0; u < TPT; u++) {
+        rank[u] = -1; rowcode[u] = 0; vbase[u] = 0; L8[u] = 0; pown[u] = false; fo[u] = R(0);
+        if (!active[u]) continue;
+        const uint32_t in = info[u];
+        const int a = in & 7;
+        rank[u] = (in >> 3) & 15;
+        rowcode[u] = (in >> 7) & 3;
+        pown[u] = STOKES && ((in >> 9) & 1);
+        vbase[u] = (in >> 10) & 0x3FFF;
+        L8[u] = in >> 24;
+        const int comp0 = rowcode[u] == 1 ? 1 : 0;
+        const int nr = rowcode[u] == 2 ? 2 : 1;
+        if constexpr (MIXED) {
+            const double x0 = X[u][0], y0 = Y[u][0];
+            if (!A.geom) {
 #pragma unroll
-                for (int k = 0; k < NEN; k++) { X[k] -= x0; Y[k] -= y0; }
+                for (int k = 0; k < NEN; k++) { X[u][k] -= x0; Y[u][k] -= y0; }
             }
         }
         Geo<R, NEN, NQ> g;
         constexpr QTable CT = qt_fixed<NEN, NQ>();
-        if constexpr (MIXED) geometry_mixed<NEN, NQ>(CT, X, Y, g);
-        else geometry<double, NEN, NQ>(tab, X, Y, g);
+        if constexpr (MIXED) geometry_mixed<NEN, NQ>(CT, X[u], Y[u], g);
+        else geometry<double, NEN, NQ>(tab, X[u], Y[u], g);
         R Dr[9];
 #pragma unroll
-        for (int i = 0; i < 9; i++) Dr[i] = (R)(cf.scale_e ? sc * cf.D[i] : cf.D[i]);
-        owner_values<KIND, R, NEN, NC, NQ, CSC>(MIXED ? CT : tab, g, Dr, (R)kapd, (R)cf.mu, (R)Qd, a, comp0, nr, powner,
-                                                out, &fo);
+        for (int i = 0; i < 9; i++) Dr[i] = (R)(cf.scale_e ? sc[u] * cf.D[i] : cf.D[i]);
+        owner_values<KIND, R, NEN, NC, NQ, CSC>(MIXED ? CT : tab, g, Dr, (R)kapd[u], (R)cf.mu, (R)Qd[u], a, comp0,
+                                                nr, pown[u], out[u], &fo[u]);
     }
     __syncthreads();
     const int nranks = tr.z;
     for (int s = 0; s < nranks; s++) {
-        if (active && rank == s) {
+#pragma unroll
+        for (int u = 0; u < TPT; u++) {
+            if (!(active[u] && rank[u] == s)) continue;
             // A task's positions are distinct (distinct dofs of one element), so
             // all of its partial sums are read before any is written back: one
             // LDS round trip per rank instead of one per entry.
-            const int ncols = (STOKES && pown) ? ND : NCOL;
-            const int nr = rowcode == 2 ? 2 : 1;
-            const uint32_t fmask = posw[NPW - 1] >> 16;
+            const int ncols = (STOKES && pown[u]) ? ND : NCOL;
+            const int nr = rowcode[u] == 2 ? 2 : 1;
 #pragma unroll
             for (int c = 0; c < 2; c++) {
                 if (c >= nr) continue;
-                double *row = acc + vbase + c * L8;
+                const int rbase = vbase[u] + c * L8[u];
                 double cur[NCOL];
 #pragma unroll
-                for (int j = 0; j < NCOL; j++)
-                    if (j < ncols) cur[j] = ((fmask >> j) & 1u) ? 0.0 : row[(posw[j >> 2] >> (8 * (j & 3))) & 0xFF];
+                for (int j = 0; j < NCOL; j++) {
+                    const uint32_t pb = (posw[u][j >> 2] >> (8 * (j & 3))) & 0xFFu;
+                    if (j < ncols) cur[j] = (pb & 0x80u) ? 0.0 : acc[lds_slot(rbase + (int)(pb & 0x7Fu))];
+                }
 #pragma unroll
-                for (int j = 0; j < NCOL; j++)
-                    if (j < ncols) row[(posw[j >> 2] >> (8 * (j & 3))) & 0xFF] = cur[j] + (double)out[c][j];
+                for (int j = 0; j < NCOL; j++) {
+                    const uint32_t pb = (posw[u][j >> 2] >> (8 * (j & 3))) & 0xFFu;
+                    if (j < ncols) acc[lds_slot(rbase + (int)(pb & 0x7Fu))] = cur[j] + (double)out[u][c][j];
+                }
             }
-            if constexpr (KIND == ELFEL_HEAT) accF[L8] += (double)fo;
+            if constexpr (KIND == ELFEL_HEAT) accF[L8[u]] += (double)fo[u];
         }
         __syncthreads();
     }
@@ -354,25 +399,30 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
     const int head = td.z & 1;
     const int npair = (td.w - head) >> 1;
     if (tid == 0) {
-        if (head && td.w > 0) __builtin_nontemporal_store(acc[0], &dst[0]);
-        if (((td.w - head) & 1) && td.w > head) __builtin_nontemporal_store(acc[td.w - 1], &dst[td.w - 1]);
+        if (head && td.w > 0) __builtin_nontemporal_store(acc[lds_slot(0)], &dst[0]);
+        if (((td.w - head) & 1) && td.w > head) __builtin_nontemporal_store(acc[lds_slot(td.w - 1)], &dst[td.w - 1]);
     }
 #pragma unroll 4
     for (int p = tid; p < npair; p += BLOCK) {
         const int v = head + 2 * p;
-        __builtin_nontemporal_store(acc[v], &dst[v]);
-        __builtin_nontemporal_store(acc[v + 1], &dst[v + 1]);
+        __builtin_nontemporal_store(acc[lds_slot(v)], &dst[v]);
+        __builtin_nontemporal_store(acc[lds_slot(v + 1)], &dst[v + 1]);
     }
     if constexpr (KIND == ELFEL_HEAT)
         for (int v = tid; v < tr.y; v += BLOCK) A.F[tr.x + v] = accF[v];
 }
+
+// Tasks per thread of the single-problem kernels: heat tasks are light (one
+// row of NEN columns), two per thread overlap their loads.
+__host__ __device__ constexpr int reuse_tpt(int kind) { return kind == ELFEL_HEAT ? 2 : 1; }
+#define REUSE_MAX_TILE_ROWS 256  // tile-local row numbers of one-row groups are 8-bit
 
 template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC, int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? REUSE_WAVES_PER_EU : 1)))
 void k_reuse(ReuseArgs A, Coef cf, QTable tab, int lds_vals)
 {
     extern __shared__ double lds[];
-    reuse_tile<KIND, R, NEN, NC, NQ, CSC, BLOCK>(A, cf, tab, lds, lds_vals, blockIdx.x);
+    reuse_tile<KIND, R, NEN, NC, NQ, CSC, BLOCK, reuse_tpt(KIND)>(A, cf, tab, lds, lds_vals, blockIdx.x);
 }
 
 // Batched problems of uneven size in one persistent launch: heat-T3 (1-point)
@@ -385,8 +435,8 @@ __global__ __launch_bounds__(BLOCK) void k_reuse_batch(ReuseArgs H, ReuseArgs Ev
     extern __shared__ double lds[];
     const int nt = H.ntiles + Ev.ntiles;
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
-        if (t < H.ntiles) reuse_tile<ELFEL_HEAT, float, 3, 1, 1, CSC, BLOCK>(H, cfh, th, lds, lds_vals, t);
-        else reuse_tile<ELFEL_ELAST, float, 6, 2, 3, CSC, BLOCK>(Ev, cfe, te, lds, lds_vals, t - H.ntiles);
+        if (t < H.ntiles) reuse_tile<ELFEL_HEAT, float, 3, 1, 1, CSC, BLOCK, 1>(H, cfh, th, lds, lds_vals, t);
+        else reuse_tile<ELFEL_ELAST, float, 6, 2, 3, CSC, BLOCK, 1>(Ev, cfe, te, lds, lds_vals, t - H.ntiles);
         __syncthreads();
     }
 }
@@ -437,8 +487,8 @@ template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC, int BLOCK>
 int launch_reuse(elfel_hip_ctx *c, const ReuseArgs &A, const Coef &cf, const QTable &tab, int lds_vals)
 {
     if (A.ntiles <= 0) return 0;
-    lds_vals = (lds_vals + 1) & ~1;  // even: the load-vector sums stay 16-byte aligned
-    const size_t lds = sizeof(double) * ((size_t)lds_vals + (KIND == ELFEL_HEAT ? BLOCK : 0));
+    lds_vals = (lds_span(lds_vals) + 1) & ~1;  // even: the load-vector sums stay 16-byte aligned
+    const size_t lds = sizeof(double) * ((size_t)lds_vals + (KIND == ELFEL_HEAT ? REUSE_MAX_TILE_ROWS : 0));
     hipLaunchKernelGGL((k_reuse<KIND, R, NEN, NC, NQ, CSC, BLOCK>), dim3(A.ntiles), dim3(BLOCK), lds, c->stream, A,
                        cf, tab, lds_vals);
     HIPCHK(c, hipGetLastError());
