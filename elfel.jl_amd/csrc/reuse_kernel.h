@@ -8,8 +8,10 @@
 // copy of the tile's CSR/CSC value range.  Tasks of a group are ordered by
 // element, and the adds run rank by rank with a barrier in between, so every
 // entry is summed in element order from 0.0: the order in which
-// SparseArrays.sparse combines the duplicates of the COO stream.  The tile's
-// values are then written to HBM once, coalesced; no atomics, no sort.
+// SparseArrays.sparse combines the duplicates of the COO stream (within a rank
+// step no two tasks touch the same entry, so Stokes tiles may use LDS atomic
+// adds without changing a bit).  The tile's values are then written to HBM
+// once, coalesced, with non-temporal stores; no global atomics, no sort.
 //
 // Task descriptor (uint32 info):
 //   bits 0-2   local node a of the owner in the element
@@ -18,8 +20,10 @@
 //   bit  9     owner is a pressure dof (Taylor-Hood)
 //   bits 10-23 LDS offset of the group's first row in the tile
 //   bits 24-31 row length L (two-row groups) or tile-local row (one-row groups)
-// A tile holds exactly BLOCK task slots (tile t's tasks start at t * BLOCK);
-// unused slots carry TASK_PAD (rows code 3) and element 0, and add nothing.
+// A tile holds exactly BLOCK * TPT task slots (tile t's tasks start at
+// t * BLOCK * TPT; TPT tasks per thread); unused slots carry TASK_PAD (rows
+// code 3) and element 0, and add nothing.  Within a tile the tasks are laid out
+// rank by rank.
 // task_pos[task][j]: position of element column j in the owner's sorted
 // column list (bits 0-6, rows of at most 128 entries); bit 7 says that this
 // task is the first to add into that entry (it stores instead of adding, so
@@ -245,6 +249,11 @@ __device__ __forceinline__ void owner_values(const QTable &tab, const Geo<R, NEN
 #ifndef REUSE_LDS_SKEW
 #define REUSE_LDS_SKEW 0
 #endif
+// Stokes tasks add 15 columns into one or two rows: there one LDS atomic add
+// per entry (ds_add_f64, FP64, no return) beats the read-all / write-all
+// round trip (C3 kernel 189 against 206 us); for the other kinds it does not.
+template <int KIND>
+__host__ __device__ constexpr bool reuse_lds_add() { return KIND == ELFEL_STOKES_GEN || KIND == ELFEL_STOKES_VECLAP; }
 // LDS slot of tile value v: with the skew, one spare slot after every 32
 // values spreads the strided accesses of a rank step over more banks.
 __device__ __forceinline__ int lds_slot(int v) { return REUSE_LDS_SKEW ? v + (v >> 5) : v; }
@@ -271,6 +280,12 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
     const int4 tr = A.tile_rows[tile];
     if constexpr (KIND == ELFEL_HEAT)
         for (int v = tid; v < tr.y; v += BLOCK) accF[v] = 0.0;
+    if constexpr (reuse_lds_add<KIND>()) {  // the adds below need the tile's sums zeroed
+        double2 *a2 = (double2 *)acc;
+        const int n2 = (lds_span(td.w) + 1) >> 1;
+#pragma unroll 4
+        for (int v = tid; v < n2; v += BLOCK) a2[v] = make_double2(0.0, 0.0);
+    }
 
     // every slot holds a task record; pad slots skip the arithmetic and add nothing
     int64_t task[TPT];
@@ -370,6 +385,21 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
             // LDS round trip per rank instead of one per entry.
             const int ncols = (STOKES && pown[u]) ? ND : NCOL;
             const int nr = rowcode[u] == 2 ? 2 : 1;
+            if constexpr (reuse_lds_add<KIND>()) {
+                // one LDS add per entry; within a rank step no two lanes share an entry
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    if (c >= nr) continue;
+                    const int rbase = vbase[u] + c * L8[u];
+#pragma unroll
+                    for (int j = 0; j < NCOL; j++) {
+                        const uint32_t pb = (posw[u][j >> 2] >> (8 * (j & 3))) & 0x7Fu;
+                        if (j < ncols) atomicAdd(&acc[lds_slot(rbase + (int)pb)], (double)out[u][c][j]);
+                    }
+                }
+                if constexpr (KIND == ELFEL_HEAT) accF[L8[u]] += (double)fo[u];
+                continue;
+            }
 #pragma unroll
             for (int c = 0; c < 2; c++) {
                 if (c >= nr) continue;
