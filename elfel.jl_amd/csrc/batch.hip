@@ -72,6 +72,7 @@ void batch_free(elfel_hip_ctx *c)
         release(k.conn); release(k.xy); release(k.coef);
         for (auto &t : k.tab) {
             release(t.tile); release(t.tile_rows); release(t.task_elem); release(t.task_info); release(t.task_pos);
+            release(t.tile_elem);
         }
     }
     release(b->ptr); release(b->ind); release(b->val); release(b->F);
@@ -162,6 +163,8 @@ static ReuseArgs kind_args(KindData &k, int bi, double *vals, double *F)
     A.tiles = (const int4 *)P.tile.p;
     A.tile_rows = (const int4 *)P.tile_rows.p;
     A.task_elem = (const int32_t *)P.task_elem.p;
+    A.tile_elem = (const int32_t *)P.tile_elem.p;
+    A.maxu = P.maxu;
     A.task_info = (const uint32_t *)P.task_info.p;
     A.task_pos = (const uint8_t *)P.task_pos.p;
     A.ncolp = P.ncolmax;
@@ -194,8 +197,10 @@ static int batch_launch(elfel_hip_ctx *c, Batch *b, int block, int grid)
     if (!Ev.E) AE.ntiles = 0;
     const int nt = AH.ntiles + AE.ntiles;
     if (nt == 0) return 0;
-    int lds_vals = std::max(H.E ? (int)H.tab[bi].lds_vals : 0, Ev.E ? (int)Ev.tab[bi].lds_vals : 0);
-    lds_vals = (lds_span(lds_vals) + 1) & ~1;  // even: the load-vector sums stay 16-byte aligned
+    int lds_vals = std::max(H.E ? lds_span((int)H.tab[bi].lds_vals) : 0, Ev.E ? lds_span((int)Ev.tab[bi].lds_vals) : 0);
+    if (H.E)  // heat-T3 element records (9 + 3 floats) alias the value sums
+        lds_vals = std::max(lds_vals, (H.tab[bi].maxu * 12 * 4 + 7) / 8);
+    lds_vals = (lds_vals + 1) & ~1;  // even: the load-vector sums stay 16-byte aligned
     const size_t lds = sizeof(double) * ((size_t)lds_vals + block);
     const int g = grid > 0 ? std::min(grid, nt) : nt;
     const bool csc = b->layout == ELFEL_CSC;
@@ -335,7 +340,13 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
                     cat.tiles.push_back(td);
                     cat.trows.push_back(tr);
                 }
-                for (int32_t e : t.elem) cat.elem.push_back(e + (int32_t)p.elem_base);
+                if (t.telem.empty()) {
+                    for (int32_t e : t.elem) cat.elem.push_back(e + (int32_t)p.elem_base);
+                } else {  // element-staged (heat) tiles: tasks hold tile-local indices
+                    cat.elem.insert(cat.elem.end(), t.elem.begin(), t.elem.end());
+                    for (int32_t e : t.telem) cat.telem.push_back(e + (int32_t)p.elem_base);
+                    cat.maxu = std::max(cat.maxu, t.maxu);
+                }
                 cat.info.insert(cat.info.end(), t.info.begin(), t.info.end());
                 cat.pos.insert(cat.pos.end(), t.pos.begin(), t.pos.end());
             }

@@ -113,6 +113,9 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     // threads that add in one rank step sit in as few waves as possible and
     // every other wave skips that step.
     const bool by_rank = rank_order_enabled();
+    // scalar problems: the kernel computes each distinct element of a tile once
+    // (element-staged tiles, reuse_kernel.h)
+    const bool staged = !stokes && nc == 1;
     std::vector<int32_t> te;
     std::vector<uint32_t> ti;
     std::vector<uint8_t> tp;
@@ -132,6 +135,20 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
                         std::copy(tp.begin() + k * ncolp, tp.begin() + (k + 1) * ncolp, T.pos.begin() + o * ncolp);
                         o++;
                     }
+        }
+        if (staged) {
+            // the tile's distinct elements (ascending); tasks then name their
+            // element by its index in this list
+            const size_t t0 = (size_t)cur.x, nt = (size_t)cur.y;
+            te.assign(T.elem.begin() + t0, T.elem.end());
+            std::sort(te.begin(), te.end());
+            te.erase(std::unique(te.begin(), te.end()), te.end());
+            for (size_t k = 0; k < nt; k++)
+                T.elem[t0 + k] = (int32_t)(std::lower_bound(te.begin(), te.end(), T.elem[t0 + k]) - te.begin());
+            cr.w = (int)te.size();
+            T.maxu = std::max(T.maxu, cr.w);
+            T.telem.insert(T.telem.end(), te.begin(), te.end());
+            T.telem.insert(T.telem.end(), (size_t)cap - te.size(), te.empty() ? 0 : te[0]);
         }
         // pad to cap tasks, so that tile t's tasks start at t * cap (the kernel
         // computes a thread's task index without reading the tile first)
@@ -236,6 +253,7 @@ static void order_tiles(ReuseTables &T, int nxcd, int cap)
     std::vector<int32_t> elem(T.elem.size());
     std::vector<uint32_t> info(T.info.size());
     std::vector<uint8_t> pos(T.pos.size());
+    std::vector<int32_t> telem(T.telem.size());
     const size_t ncolp = (size_t)T.ncolp;
     for (size_t b = 0; b < nt; b++) {
         const size_t o = (size_t)key[off[b % nxcd] + b / nxcd].second;
@@ -245,12 +263,15 @@ static void order_tiles(ReuseTables &T, int nxcd, int cap)
         std::copy(T.elem.begin() + o * cap, T.elem.begin() + (o + 1) * cap, elem.begin() + b * cap);
         std::copy(T.info.begin() + o * cap, T.info.begin() + (o + 1) * cap, info.begin() + b * cap);
         std::copy(T.pos.begin() + o * cap * ncolp, T.pos.begin() + (o + 1) * cap * ncolp, pos.begin() + b * cap * ncolp);
+        if (!T.telem.empty())
+            std::copy(T.telem.begin() + o * cap, T.telem.begin() + (o + 1) * cap, telem.begin() + b * cap);
     }
     T.tiles.swap(tiles);
     T.trows.swap(trows);
     T.elem.swap(elem);
     T.info.swap(info);
     T.pos.swap(pos);
+    T.telem.swap(telem);
 }
 
 int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P)
@@ -261,7 +282,8 @@ int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P)
         (rc = ensure(c, P.tile_rows, sizeof(int4) * std::max<size_t>(T.trows.size(), 1))) ||
         (rc = ensure(c, P.task_elem, 4 * std::max<int64_t>(ntasks, 1))) ||
         (rc = ensure(c, P.task_info, 4 * std::max<int64_t>(ntasks, 1))) ||
-        (rc = ensure(c, P.task_pos, std::max<size_t>(T.pos.size(), 1))))
+        (rc = ensure(c, P.task_pos, std::max<size_t>(T.pos.size(), 1))) ||
+        (rc = ensure(c, P.tile_elem, 4 * std::max<size_t>(T.telem.size(), 1))))
         return rc;
     if (!T.tiles.empty()) {
         HIPCHK(c, hipMemcpy(P.tile.p, T.tiles.data(), sizeof(int4) * T.tiles.size(), hipMemcpyHostToDevice));
@@ -272,11 +294,14 @@ int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P)
         HIPCHK(c, hipMemcpy(P.task_info.p, T.info.data(), 4 * ntasks, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(P.task_pos.p, T.pos.data(), T.pos.size(), hipMemcpyHostToDevice));
     }
+    if (!T.telem.empty())
+        HIPCHK(c, hipMemcpy(P.tile_elem.p, T.telem.data(), 4 * T.telem.size(), hipMemcpyHostToDevice));
     P.ntiles = (int64_t)T.tiles.size();
     P.ntasks = ntasks;
     P.ncolmax = T.ncolp;
     P.maxrank = T.maxrank;
     P.lds_vals = T.maxv;
+    P.maxu = T.maxu;
     return 0;
 }
 
@@ -357,6 +382,8 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
     A.tiles = (const int4 *)P.tile.p;
     A.tile_rows = (const int4 *)P.tile_rows.p;
     A.task_elem = (const int32_t *)P.task_elem.p;
+    A.tile_elem = (const int32_t *)P.tile_elem.p;
+    A.maxu = P.maxu;
     A.task_info = (const uint32_t *)P.task_info.p;
     A.task_pos = (const uint8_t *)P.task_pos.p;
     A.ncolp = P.ncolmax;

@@ -43,7 +43,9 @@
 struct ReuseArgs {
     const int4 *tiles;        // task0, ntasks, v0, nvals
     const int4 *tile_rows;    // row0, nrows, nranks, 0
-    const int32_t *task_elem;
+    const int32_t *task_elem;  // element of the task (scalar problems: its index in tile_elem)
+    const int32_t *tile_elem;  // scalar problems: distinct elements of each tile, [ntiles][slots]
+    int maxu;                  // scalar problems: most distinct elements in a tile
     const uint32_t *task_info;
     const uint8_t *task_pos;  // [ntasks][ncolp]
     int ncolp;
@@ -259,6 +261,111 @@ __host__ __device__ constexpr bool reuse_lds_add() { return KIND == ELFEL_STOKES
 __device__ __forceinline__ int lds_slot(int v) { return REUSE_LDS_SKEW ? v + (v >> 5) : v; }
 __host__ __device__ constexpr int lds_span(int nvals) { return REUSE_LDS_SKEW ? nvals + (nvals >> 5) + 1 : nvals; }
 
+// Element-staged heat tiles.  A scalar task owns one row (CSR) or column (CSC)
+// of NEN entries, so per-task geometry and loads would dominate; instead the
+// tile's distinct elements (tile_elem, ~half the task count) are computed once
+// each: the whole element matrix ke[i][j] and load vector fe[j], into LDS
+// records that alias the tile's value sums.  Each task then reads its row or
+// column of its element's record.  The FP64 records repeat the reference's
+// operations (ke[i,j] += (grad N_i . grad N_j) kappa JxW, fe[j] += N_j Q JxW, in
+// point order), so every entry has the same bits as the per-task form.
+template <typename R, int NEN, int NQ, bool CSC, int BLOCK, int TPT>
+__device__ __forceinline__ void heat_tile(const ReuseArgs &A, const Coef &cf, const QTable &tab, double *lds, int tile,
+                                          int nU, const int32_t *e, const uint32_t *info, const bool *active,
+                                          int *rank, int *vbase, int *L8, R (*out)[2][NEN + 3], R *fo)
+{
+    constexpr bool MIXED = sizeof(R) == 4;
+    constexpr int RECN = NEN * NEN + NEN;
+    constexpr int SLOTS = BLOCK * TPT;
+    constexpr QTable CT = qt_fixed<NEN, NQ>();
+    R *rec = (R *)lds;
+    for (int le = threadIdx.x; le < nU; le += BLOCK) {
+        const int32_t eg = A.tile_elem[(int64_t)tile * SLOTS + le];
+        double X[NEN], Y[NEN];
+        if (A.geom) {
+            const double *gp = (const double *)A.geom + (int64_t)eg * (2 * NEN);
+#pragma unroll
+            for (int k = 0; k < NEN; k++) { X[k] = gp[2 * k]; Y[k] = gp[2 * k + 1]; }
+        } else {
+            int32_t nd[NEN];
+#pragma unroll
+            for (int k = 0; k < NEN; k++) nd[k] = A.conn[(int64_t)eg * NEN + k];
+#pragma unroll
+            for (int k = 0; k < NEN; k++) { const double2 p = A.xy[nd[k]]; X[k] = p.x; Y[k] = p.y; }
+            if constexpr (MIXED) {
+                const double x0 = X[0], y0 = Y[0];
+#pragma unroll
+                for (int k = 0; k < NEN; k++) { X[k] -= x0; Y[k] -= y0; }
+            }
+        }
+        const double kap = cf.kappa_e ? cf.kappa_e[eg] : cf.kappa;
+        const double Qv = cf.Q_e ? cf.Q_e[eg] : cf.Q;
+        Geo<R, NEN, NQ> g;
+        R *r = rec + (size_t)le * RECN;
+        if constexpr (MIXED) {
+            geometry_mixed<NEN, NQ>(CT, X, Y, g);
+            float wx[NQ][NEN], wy[NQ][NEN];
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const float w = (float)kap * g.jxw[q];
+#pragma unroll
+                for (int j = 0; j < NEN; j++) { wx[q][j] = g.gx[q][j] * w; wy[q][j] = g.gy[q][j] * w; }
+            }
+#pragma unroll
+            for (int i = 0; i < NEN; i++)
+#pragma unroll
+                for (int j = 0; j < NEN; j++) {
+                    float acc = 0.f;
+#pragma unroll
+                    for (int q = 0; q < NQ; q++) acc = fmaf(g.gx[q][i], wx[q][j], fmaf(g.gy[q][i], wy[q][j], acc));
+                    r[i * NEN + j] = acc;
+                }
+#pragma unroll
+            for (int j = 0; j < NEN; j++) {
+                float acc = 0.f;
+#pragma unroll
+                for (int q = 0; q < NQ; q++) acc += CT.Nf[q][j] * (float)Qv * g.jxw[q];
+                r[NEN * NEN + j] = acc;
+            }
+        } else {
+            geometry<double, NEN, NQ>(tab, X, Y, g);
+#pragma unroll
+            for (int i = 0; i < NEN; i++)
+#pragma unroll
+                for (int j = 0; j < NEN; j++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int q = 0; q < NQ; q++)
+                        acc += (g.gx[q][i] * g.gx[q][j] + g.gy[q][i] * g.gy[q][j]) * (kap * g.jxw[q]);
+                    r[i * NEN + j] = acc;
+                }
+#pragma unroll
+            for (int j = 0; j < NEN; j++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int q = 0; q < NQ; q++) acc += tab.N[q][j] * Qv * g.jxw[q];
+                r[NEN * NEN + j] = acc;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < TPT; u++) {
+        rank[u] = -1; vbase[u] = 0; L8[u] = 0; fo[u] = R(0);
+        if (!active[u]) continue;
+        const uint32_t in = info[u];
+        const int a = in & 7;
+        rank[u] = (in >> 3) & 15;
+        vbase[u] = (in >> 10) & 0x3FFF;
+        L8[u] = in >> 24;
+        const R *r = rec + (size_t)e[u] * RECN;
+#pragma unroll
+        for (int j = 0; j < NEN; j++) out[u][0][j] = CSC ? r[j * NEN + a] : r[a * NEN + j];
+        fo[u] = r[NEN * NEN + a];
+    }
+    // the caller's barrier separates these reads from the sums that overwrite the records
+}
+
 // One tile: a workgroup of BLOCK threads, TPT tasks per thread (task slot
 // u * BLOCK + tid).  The loads of all of a thread's tasks are issued stage by
 // stage (task records, then node numbers and coefficients, then coordinates)
@@ -309,6 +416,15 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
 #pragma unroll
         for (int w = 0; w < NPW; w++) posw[u][w] = pw[w];
     }
+    int rank[TPT], rowcode[TPT], vbase[TPT], L8[TPT];
+    bool pown[TPT];
+    R out[TPT][2][ND + 3];
+    R fo[TPT];
+#pragma unroll
+    for (int u = 0; u < TPT; u++) { rowcode[u] = 0; pown[u] = false; }
+    if constexpr (KIND == ELFEL_HEAT) {
+        heat_tile<R, NEN, NQ, CSC, BLOCK, TPT>(A, cf, tab, lds, tile, tr.w, e, info, active, rank, vbase, L8, out, fo);
+    } else {
     // node numbers (gather fused in) and per-element coefficients
     int32_t nd[TPT][NEN];
     double sc[TPT], kapd[TPT], Qd[TPT];
@@ -340,10 +456,6 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
             for (int k = 0; k < NEN; k++) { const double2 p = A.xy[nd[u][k]]; X[u][k] = p.x; Y[u][k] = p.y; }
         }
     }
-    int rank[TPT], rowcode[TPT], vbase[TPT], L8[TPT];
-    bool pown[TPT];
-    R out[TPT][2][ND + 3];
-    R fo[TPT];
 #pragma unroll
     for (int u = 0; u < TPT; u++) {
         rank[u] = -1; rowcode[u] = 0; vbase[u] = 0; L8[u] = 0; pown[u] = false; fo[u] = R(0);
@@ -374,6 +486,7 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
         owner_values<KIND, R, NEN, NC, NQ, CSC>(MIXED ? CT : tab, g, Dr, (R)kapd[u], (R)cf.mu, (R)Qd[u], a, comp0,
                                                 nr, pown[u], out[u], &fo[u]);
     }
+    }  // not heat
     __syncthreads();
     const int nranks = tr.z;
     for (int s = 0; s < nranks; s++) {
@@ -498,7 +611,8 @@ struct ReuseTables {
     std::vector<int32_t> elem;
     std::vector<uint32_t> info;
     std::vector<uint8_t> pos;
-    int ncolp = 0, maxv = 0, maxrank = 0;
+    std::vector<int32_t> telem;  // scalar problems: distinct elements per tile (padded to the slot count)
+    int ncolp = 0, maxv = 0, maxrank = 0, maxu = 0;
 };
 int pattern_to_host(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf,
                     std::vector<int32_t> &hptr, std::vector<int32_t> &hind);
@@ -517,7 +631,10 @@ template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC, int BLOCK>
 int launch_reuse(elfel_hip_ctx *c, const ReuseArgs &A, const Coef &cf, const QTable &tab, int lds_vals)
 {
     if (A.ntiles <= 0) return 0;
-    lds_vals = (lds_span(lds_vals) + 1) & ~1;  // even: the load-vector sums stay 16-byte aligned
+    lds_vals = lds_span(lds_vals);
+    if constexpr (KIND == ELFEL_HEAT)  // element records alias the value sums
+        lds_vals = std::max(lds_vals, (int)(((size_t)A.maxu * (NEN * NEN + NEN) * sizeof(R) + 7) / 8));
+    lds_vals = (lds_vals + 1) & ~1;  // even: the load-vector sums stay 16-byte aligned
     const size_t lds = sizeof(double) * ((size_t)lds_vals + (KIND == ELFEL_HEAT ? REUSE_MAX_TILE_ROWS : 0));
     hipLaunchKernelGGL((k_reuse<KIND, R, NEN, NC, NQ, CSC, BLOCK>), dim3(A.ntiles), dim3(BLOCK), lds, c->stream, A,
                        cf, tab, lds_vals);
