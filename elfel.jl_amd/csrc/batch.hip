@@ -198,8 +198,8 @@ static int batch_launch(elfel_hip_ctx *c, Batch *b, int block, int grid)
     const int nt = AH.ntiles + AE.ntiles;
     if (nt == 0) return 0;
     int lds_vals = std::max(H.E ? lds_span((int)H.tab[bi].lds_vals) : 0, Ev.E ? lds_span((int)Ev.tab[bi].lds_vals) : 0);
-    if (H.E)  // heat-T3 element records (9 + 3 floats) alias the value sums
-        lds_vals = std::max(lds_vals, (H.tab[bi].maxu * 12 * 4 + 7) / 8);
+    if (H.E)  // heat-T3 element records (floats) alias the value sums
+        lds_vals = std::max(lds_vals, (H.tab[bi].maxu * heat_rec<3>() * 4 + 7) / 8);
     lds_vals = (lds_vals + 1) & ~1;  // even: the load-vector sums stay 16-byte aligned
     const size_t lds = sizeof(double) * ((size_t)lds_vals + block);
     const int g = grid > 0 ? std::min(grid, nt) : nt;
@@ -342,7 +342,7 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
                 }
                 if (t.telem.empty()) {
                     for (int32_t e : t.elem) cat.elem.push_back(e + (int32_t)p.elem_base);
-                } else {  // element-staged (heat) tiles: tasks hold tile-local indices
+                } else {  // element-staged tiles: tasks hold tile-local indices
                     cat.elem.insert(cat.elem.end(), t.elem.begin(), t.elem.end());
                     for (int32_t e : t.telem) cat.telem.push_back(e + (int32_t)p.elem_base);
                     cat.maxu = std::max(cat.maxu, t.maxu);

@@ -64,7 +64,7 @@ struct Pattern {
     Buf task_elem;               // int32 per task
     Buf task_info;               // uint32 per task
     Buf task_pos;                // uint8 [ntasks][ncolmax]
-    Buf tile_elem;               // int32 [ntiles][slots]: distinct elements of a tile (scalar problems)
+    Buf tile_elem;               // int32 [ntiles][slots]: distinct elements of a tile (heat)
     int maxu = 0;                // most distinct elements in one tile
 };
 

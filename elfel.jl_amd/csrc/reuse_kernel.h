@@ -43,9 +43,9 @@
 struct ReuseArgs {
     const int4 *tiles;        // task0, ntasks, v0, nvals
     const int4 *tile_rows;    // row0, nrows, nranks, 0
-    const int32_t *task_elem;  // element of the task (scalar problems: its index in tile_elem)
-    const int32_t *tile_elem;  // scalar problems: distinct elements of each tile, [ntiles][slots]
-    int maxu;                  // scalar problems: most distinct elements in a tile
+    const int32_t *task_elem;  // element of the task (heat: its index in tile_elem)
+    const int32_t *tile_elem;  // heat: distinct elements of each tile, [ntiles][slots]
+    int maxu;                  // heat: most distinct elements in a tile
     const uint32_t *task_info;
     const uint8_t *task_pos;  // [ntasks][ncolp]
     int ncolp;
@@ -261,6 +261,12 @@ __host__ __device__ constexpr bool reuse_lds_add() { return KIND == ELFEL_STOKES
 __device__ __forceinline__ int lds_slot(int v) { return REUSE_LDS_SKEW ? v + (v >> 5) : v; }
 __host__ __device__ constexpr int lds_span(int nvals) { return REUSE_LDS_SKEW ? nvals + (nvals >> 5) + 1 : nvals; }
 
+// Record length of the element-staged heat tiles, odd: the lanes of a wave
+// read records of different elements, and an odd stride spreads them over the
+// LDS banks.
+template <int NEN>
+__host__ __device__ constexpr int heat_rec() { return (NEN * NEN + NEN) | 1; }
+
 // Element-staged heat tiles.  A scalar task owns one row (CSR) or column (CSC)
 // of NEN entries, so per-task geometry and loads would dominate; instead the
 // tile's distinct elements (tile_elem, ~half the task count) are computed once
@@ -275,7 +281,7 @@ __device__ __forceinline__ void heat_tile(const ReuseArgs &A, const Coef &cf, co
                                           int *rank, int *vbase, int *L8, R (*out)[2][NEN + 3], R *fo)
 {
     constexpr bool MIXED = sizeof(R) == 4;
-    constexpr int RECN = NEN * NEN + NEN;
+    constexpr int RECN = heat_rec<NEN>();
     constexpr int SLOTS = BLOCK * TPT;
     constexpr QTable CT = qt_fixed<NEN, NQ>();
     R *rec = (R *)lds;
@@ -611,7 +617,7 @@ struct ReuseTables {
     std::vector<int32_t> elem;
     std::vector<uint32_t> info;
     std::vector<uint8_t> pos;
-    std::vector<int32_t> telem;  // scalar problems: distinct elements per tile (padded to the slot count)
+    std::vector<int32_t> telem;  // heat: distinct elements per tile (padded to the slot count)
     int ncolp = 0, maxv = 0, maxrank = 0, maxu = 0;
 };
 int pattern_to_host(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf,
@@ -633,7 +639,7 @@ int launch_reuse(elfel_hip_ctx *c, const ReuseArgs &A, const Coef &cf, const QTa
     if (A.ntiles <= 0) return 0;
     lds_vals = lds_span(lds_vals);
     if constexpr (KIND == ELFEL_HEAT)  // element records alias the value sums
-        lds_vals = std::max(lds_vals, (int)(((size_t)A.maxu * (NEN * NEN + NEN) * sizeof(R) + 7) / 8));
+        lds_vals = std::max(lds_vals, (int)(((size_t)A.maxu * heat_rec<NEN>() * sizeof(R) + 7) / 8));
     lds_vals = (lds_vals + 1) & ~1;  // even: the load-vector sums stay 16-byte aligned
     const size_t lds = sizeof(double) * ((size_t)lds_vals + (KIND == ELFEL_HEAT ? REUSE_MAX_TILE_ROWS : 0));
     hipLaunchKernelGGL((k_reuse<KIND, R, NEN, NC, NQ, CSC, BLOCK>), dim3(A.ntiles), dim3(BLOCK), lds, c->stream, A,
