@@ -464,10 +464,15 @@ def cpu_baseline(cfgname, cs):
     p = _oracle_problem(O, sample_cs["oracle"])
     te, ts, nnz = O.time_assembly(p, reps=1, nthreads=1)
     E = sample_cs["E"]
+    # the same element loop on every host core (sparse() stays serial), for reference
+    nth = max(1, min(16, os.cpu_count() or 1))
+    te_n, ts_n, _ = O.time_assembly(p, reps=1, nthreads=nth)
     return {"value": E / (te + ts), "unit": "elements/s", "cores": 1, "kind": "port",
             "sample": (sample or f"the full mesh ({E} elements)") + f", 1 repetition: element loop {te:.3f} s + "
                       f"sparse() {ts:.3f} s, single thread (the reference's path is serial)",
-            "element_loop_s": te, "sparse_s": ts, "host_cores_visible": os.cpu_count()}
+            "element_loop_s": te, "sparse_s": ts, "host_cores_visible": os.cpu_count(),
+            "threaded_element_loop": {"threads": nth, "element_loop_s": te_n, "sparse_s": ts_n,
+                                      "elements_per_s": E / (te_n + ts_n)}}
 
 
 def cpu_baseline_batch(probs, sample):
