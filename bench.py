@@ -43,8 +43,8 @@ METRIC = "elements assembled/s + HBM GB/s: FP32-mixed vs FP64 GPU path at <=1e-6
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", choices=["c1", "c2", "c3", "c4", "c5"], default="c2")
     ap.add_argument("--N", type=int, default=0, help="grid size (default: the configuration's)")
     ap.add_argument("--fp64-steps", type=int, default=10)

@@ -25,7 +25,9 @@
 // each slice as wide as its longest row, entries column-major inside the
 // slice), built once per solve from the CSR result: one thread per row, every
 // load coalesced, data columns left out (their part of b is formed once with
-// the CSR matrix).
+// the CSR matrix).  When every free column lies within a 16-bit offset of its
+// row (banded numberings: structured grids, RCM-ordered meshes) the columns
+// are stored as int16 offsets, 2 bytes per entry instead of 4.
 // Per iteration three kernels on a fixed grid of 4 workgroups per CU
 // (grid-stride loops): q = A p with one partial of p.q per workgroup; the
 // x / r / z update (every workgroup first sums the p.q and previous r.z
@@ -76,15 +78,31 @@ __device__ __forceinline__ double sum_partials(const double *p, int g, double *s
 }
 
 // ---- SELL-64 construction ----------------------------------------------------
+// Free entries per row, and whether a free column lies farther than a 16-bit
+// offset from the row (then the column indices stay 32-bit).
 __global__ void k_sell_rowlen(int64_t nfree, const int32_t *__restrict__ ptr, const int32_t *__restrict__ ind,
-                              uint32_t *__restrict__ len)
+                              uint32_t *__restrict__ len, uint32_t *__restrict__ far)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nfree) return;
-    uint32_t k = 0;
-    for (int32_t j = ptr[i]; j < ptr[i + 1]; j++) k += ind[j] < nfree ? 1u : 0u;
+    uint32_t k = 0, f = 0;
+    for (int32_t j = ptr[i]; j < ptr[i + 1]; j++) {
+        if (ind[j] >= nfree) continue;
+        k++;
+        const int64_t d = (int64_t)ind[j] - i;
+        f |= (d < -32768 || d > 32767) ? 1u : 0u;
+    }
     len[i] = k;
+    far[i] = f;
 }
+
+// Column of entry k of a SELL row: 32-bit index, or 16-bit offset from the row.
+template <typename C>
+__device__ __forceinline__ int32_t sell_col(C c, int64_t row);
+template <>
+__device__ __forceinline__ int32_t sell_col<int32_t>(int32_t c, int64_t row) { return c; }
+template <>
+__device__ __forceinline__ int32_t sell_col<int16_t>(int16_t c, int64_t row) { return (int32_t)row + (int32_t)c; }
 
 // One wave per slice: width = longest row; writes 64 * width (the slice size).
 __global__ void k_sell_width(int64_t nfree, int64_t nslices, const uint32_t *__restrict__ len,
@@ -99,9 +117,10 @@ __global__ void k_sell_width(int64_t nfree, int64_t nslices, const uint32_t *__r
     if (lane == 0) size[s] = 64u * w;  // lane 0 holds the maximum
 }
 
+template <typename C>
 __global__ void k_sell_fill(int64_t nfree, const int32_t *__restrict__ ptr, const int32_t *__restrict__ ind,
                             const double *__restrict__ val, const uint32_t *__restrict__ off,
-                            int32_t *__restrict__ col, double *__restrict__ v64, float *__restrict__ v32)
+                            C *__restrict__ col, double *__restrict__ v64, float *__restrict__ v32)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t s = i >> 6;
@@ -114,7 +133,7 @@ __global__ void k_sell_fill(int64_t nfree, const int32_t *__restrict__ ptr, cons
             const int32_t cj = ind[j];
             if (cj >= nfree) continue;
             const size_t at = (size_t)o + (size_t)k * 64 + lane;
-            col[at] = cj;
+            col[at] = sizeof(C) == 4 ? (C)cj : (C)(cj - i);
             v64[at] = val[j];
             if (v32) v32[at] = (float)val[j];
             k++;
@@ -122,24 +141,24 @@ __global__ void k_sell_fill(int64_t nfree, const int32_t *__restrict__ ptr, cons
     }
     for (; k < w; k++) {  // padding: a zero on the row's own column (or row 0 beyond nfree)
         const size_t at = (size_t)o + (size_t)k * 64 + lane;
-        col[at] = i < nfree ? (int32_t)i : 0;
+        col[at] = sizeof(C) == 4 ? (C)(i < nfree ? i : 0) : (C)0;  // rows past nfree are never read
         v64[at] = 0.0;
         if (v32) v32[at] = 0.f;
     }
 }
 
-template <typename T>
-__device__ __forceinline__ T sell_row(int64_t i, const uint32_t *__restrict__ off, const int32_t *__restrict__ col,
+template <typename T, typename C>
+__device__ __forceinline__ T sell_row(int64_t i, const uint32_t *__restrict__ off, const C *__restrict__ col,
                                       const T *__restrict__ val, const T *__restrict__ x)
 {
     const int64_t s = i >> 6;
     const int lane = (int)(i & 63);
     const uint32_t o = off[s], w = (off[s + 1] - o) >> 6;
     T acc = T(0);
-    const int32_t *c = col + o + lane;
+    const C *c = col + o + lane;
     const T *v = val + o + lane;
 #pragma unroll 4
-    for (uint32_t k = 0; k < w; k++) acc += v[(size_t)k * 64] * x[c[(size_t)k * 64]];
+    for (uint32_t k = 0; k < w; k++) acc += v[(size_t)k * 64] * x[sell_col<C>(c[(size_t)k * 64], i)];
     return acc;
 }
 
@@ -148,16 +167,16 @@ __device__ __forceinline__ T sell_row(int64_t i, const uint32_t *__restrict__ of
 //   parts layout: pq[G], rz[2][G] (by iteration parity), rr[G], aux[G]
 
 // q = A p; partial p.q
-template <typename T>
+template <typename T, typename C>
 __global__ __launch_bounds__(SV_BLOCK) void k_cg_spmv(int64_t n, const uint32_t *__restrict__ off,
-                                                      const int32_t *__restrict__ col, const T *__restrict__ val,
+                                                      const C *__restrict__ col, const T *__restrict__ val,
                                                       const T *__restrict__ p, T *__restrict__ q,
                                                       double *__restrict__ part_pq)
 {
     __shared__ double sh[SV_BLOCK / 64];
     double d = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * SV_BLOCK) {
-        const T y = sell_row<T>(i, off, col, val, p);
+        const T y = sell_row<T, C>(i, off, col, val, p);
         q[i] = y;
         d += (double)p[i] * (double)y;
     }
@@ -263,8 +282,9 @@ __global__ void k_flush(int64_t n, float *__restrict__ x32, double *__restrict__
 }
 
 // r = b - A x over the free block (SELL, FP64); partial ||r||^2
+template <typename C>
 __global__ __launch_bounds__(SV_BLOCK) void k_sell_resid(int64_t n, const uint32_t *__restrict__ off,
-                                                         const int32_t *__restrict__ col,
+                                                         const C *__restrict__ col,
                                                          const double *__restrict__ val,
                                                          const double *__restrict__ x, const double *__restrict__ b,
                                                          double *__restrict__ r, double *__restrict__ part)
@@ -272,7 +292,7 @@ __global__ __launch_bounds__(SV_BLOCK) void k_sell_resid(int64_t n, const uint32
     __shared__ double sh[SV_BLOCK / 64];
     double rr = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * SV_BLOCK) {
-        const double v = b[i] - sell_row<double>(i, off, col, val, x);
+        const double v = b[i] - sell_row<double, C>(i, off, col, val, x);
         r[i] = v;
         rr += v * v;
     }
@@ -332,7 +352,8 @@ struct Vecs {
 
 struct Sell {
     const uint32_t *off;
-    const int32_t *col;
+    const void *col;  // int32 columns, or int16 offsets from the row when c16
+    bool c16;
     const double *v64;
     const float *v32;
 };
@@ -355,7 +376,12 @@ template <typename T>
 void cg_step(hipStream_t st, int64_t n, const Sell &S, const Vecs<T> &v, const Parts &P, int par, double *sc)
 {
     const dim3 g(P.G), b(SV_BLOCK);
-    hipLaunchKernelGGL((k_cg_spmv<T>), g, b, 0, st, n, S.off, S.col, sell_vals<T>(S), v.p, v.q, P.pq);
+    if (S.c16)
+        hipLaunchKernelGGL((k_cg_spmv<T, int16_t>), g, b, 0, st, n, S.off, (const int16_t *)S.col, sell_vals<T>(S), v.p,
+                           v.q, P.pq);
+    else
+        hipLaunchKernelGGL((k_cg_spmv<T, int32_t>), g, b, 0, st, n, S.off, (const int32_t *)S.col, sell_vals<T>(S), v.p,
+                           v.q, P.pq);
     hipLaunchKernelGGL((k_cg_update<T>), g, b, 0, st, n, v.dinv, v.x, v.r, v.z, v.p, v.q, P.pq, P.rz[par ^ 1],
                        P.rz[par], P.rr);
     hipLaunchKernelGGL((k_cg_dir<T>), g, b, 0, st, n, P.rz[par], P.rz[par ^ 1], P.rr, v.z, v.p, sc);
@@ -391,7 +417,7 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
     // uint32:  off[nslices + 1], len[max(n, nslices)], total[4]
     const size_t nd = (size_t)N + 3 * (size_t)n + 5 * (size_t)G + S_NS + (mixed ? 0 : 4 * (size_t)n);
     const size_t nf = mixed ? 6 * (size_t)n : 0;
-    const size_t nu32 = (size_t)nslices + 1 + (size_t)std::max<int64_t>(n, nslices) + 4;
+    const size_t nu32 = (size_t)nslices + 1 + (size_t)std::max<int64_t>(n, nslices) + (size_t)n + 4;
     int rc = ensure(c, c->pcg, 8 * nd + 4 * nf + 4 * nu32 + 256);
     if (rc) return rc;
     double *dp = (double *)c->pcg.p;
@@ -418,31 +444,40 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
     uint32_t *up = (uint32_t *)fp;
     uint32_t *off = up; up += nslices + 1;
     uint32_t *len = up; up += std::max<int64_t>(n, nslices);
+    uint32_t *far = up; up += n;
     uint32_t *total = up;
 
     HIPCHK(c, hipMemcpyAsync(xfull, x, 8 * N, hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemcpyAsync(b, F, 8 * n, hipMemcpyHostToDevice, st));
 
     // ---- SELL-64 of the free block ------------------------------------------
-    hipLaunchKernelGGL(k_sell_rowlen, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, ptr, ind, len);
+    hipLaunchKernelGGL(k_sell_rowlen, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, ptr, ind, len, far);
     hipLaunchKernelGGL(k_sell_width, dim3(ceil_div(nslices * 64, 256)), dim3(256), 0, st, n, nslices, len, off);
     if ((rc = exclusive_scan_u32(c, off, nslices, total))) return rc;
-    uint32_t nnzp = 0;
-    HIPCHK(c, hipMemcpyAsync(&nnzp, total, 4, hipMemcpyDeviceToHost, st));
+    if ((rc = exclusive_scan_u32(c, far, n, total + 1))) return rc;
+    uint32_t tot[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(tot, total, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
+    const uint32_t nnzp = tot[0];
+    const bool c16 = tot[1] == 0;  // every free column within a 16-bit offset of its row
     HIPCHK(c, hipMemcpyAsync(off + nslices, total, 4, hipMemcpyDeviceToDevice, st));
     // SELL arrays in a buffer of their own (the size is known only now)
-    const size_t sbytes = (size_t)nnzp * (4 + 8 + (mixed ? 4 : 0)) + 256;
+    const size_t cbytes = (size_t)nnzp * (c16 ? 2 : 4);
+    const size_t sbytes = ((cbytes + 15) & ~(size_t)15) + (size_t)nnzp * (8 + (mixed ? 4 : 0)) + 256;
     if ((rc = ensure(c, c->pcg_sell, sbytes))) return rc;
-    int32_t *scol = (int32_t *)c->pcg_sell.p;
-    double *sv64 = (double *)((char *)c->pcg_sell.p + (((size_t)nnzp * 4 + 15) & ~(size_t)15));
+    void *scol = c->pcg_sell.p;
+    double *sv64 = (double *)((char *)c->pcg_sell.p + ((cbytes + 15) & ~(size_t)15));
     float *sv32 = mixed ? (float *)(sv64 + nnzp) : nullptr;
-    hipLaunchKernelGGL(k_sell_fill, dim3(ceil_div(nslices * 64, 256)), dim3(256), 0, st, n, ptr, ind, val, off, scol,
-                       sv64, sv32);
+    if (c16)
+        hipLaunchKernelGGL((k_sell_fill<int16_t>), dim3(ceil_div(nslices * 64, 256)), dim3(256), 0, st, n, ptr, ind, val,
+                           off, (int16_t *)scol, sv64, sv32);
+    else
+        hipLaunchKernelGGL((k_sell_fill<int32_t>), dim3(ceil_div(nslices * 64, 256)), dim3(256), 0, st, n, ptr, ind, val,
+                           off, (int32_t *)scol, sv64, sv32);
     hipLaunchKernelGGL(k_diag_inv, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, ptr, ind, val, dinv64,
                        mixed ? v32.dinv : nullptr);
     HIPCHK(c, hipGetLastError());
-    const Sell S{off, scol, sv64, sv32};
+    const Sell S{off, scol, c16, sv64, sv32};
 
     auto read_sc = [&](double *h) -> int {
         HIPCHK(c, hipMemcpyAsync(h, sc, sizeof(double) * S_NS, hipMemcpyDeviceToHost, st));
@@ -461,8 +496,12 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
     const double target = tol * bnorm;
     // r64 = b - A x (FP64), its norm
     auto true_residual = [&](double *out_norm) -> int {
-        hipLaunchKernelGGL(k_sell_resid, dim3(G), dim3(SV_BLOCK), 0, st, n, S.off, S.col, S.v64,
-                           (const double *)xfull, (const double *)b, r64, P.aux);
+        if (S.c16)
+            hipLaunchKernelGGL((k_sell_resid<int16_t>), dim3(G), dim3(SV_BLOCK), 0, st, n, S.off,
+                               (const int16_t *)S.col, S.v64, (const double *)xfull, (const double *)b, r64, P.aux);
+        else
+            hipLaunchKernelGGL((k_sell_resid<int32_t>), dim3(G), dim3(SV_BLOCK), 0, st, n, S.off,
+                               (const int32_t *)S.col, S.v64, (const double *)xfull, (const double *)b, r64, P.aux);
         hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, (const double *)P.aux, G, sc, (int)S_RR64,
                            (const double *)nullptr, 0);
         HIPCHK(c, hipGetLastError());
