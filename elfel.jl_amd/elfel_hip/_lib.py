@@ -31,6 +31,7 @@ EXPORTED = [
     "elfel_hip_timing", "elfel_hip_timing_read",
     "elfel_hip_batch_begin", "elfel_hip_batch_add", "elfel_hip_batch_build", "elfel_hip_batch_assemble",
     "elfel_hip_batch_info", "elfel_hip_batch_copy", "elfel_hip_batch_stats",
+    "elfel_hip_number_dofs", "elfel_hip_gather_sysvec", "elfel_hip_scatter_sysvec",
 ]
 
 
@@ -84,6 +85,10 @@ def load():
     lib.elfel_hip_copy_vector.argtypes = [P, f64p, ctypes.c_int64]
     lib.elfel_hip_synchronize.argtypes = [P]
     lib.elfel_hip_geometry_stats.argtypes = [P, f64p, i64p]
+    lib.elfel_hip_number_dofs.argtypes = [P, ctypes.c_int32, i64p, ctypes.POINTER(ctypes.c_int32),
+                                          ctypes.POINTER(ctypes.c_uint8), i64p]
+    lib.elfel_hip_gather_sysvec.argtypes = [P, ctypes.c_int, ctypes.c_int64, f64p, ctypes.c_int64, f64p]
+    lib.elfel_hip_scatter_sysvec.argtypes = [P, ctypes.c_int, ctypes.c_int64, f64p, ctypes.c_int64, f64p]
     lib.elfel_hip_pcg.argtypes = [P, ctypes.c_int64, f64p, f64p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), f64p]
     lib.elfel_hip_batch_begin.argtypes = [P]
@@ -237,6 +242,40 @@ class Context:
         md, nf = ctypes.c_double(), ctypes.c_int64()
         self._chk(self.lib.elfel_hip_geometry_stats(self.h, ctypes.byref(md), ctypes.byref(nf)), "geometry_stats")
         return md.value, nf.value
+
+    # ---- dof numbering and system vectors -------------------------------------
+    def number_dofs(self, isdatum_list):
+        """numberdofs! on the device.  isdatum_list: one (nterms, ncomp) bool
+        array per space; returns the 1-based dof numbers in the same shapes."""
+        nterms = np.array([a.shape[0] for a in isdatum_list], np.int64)
+        ncomp = np.array([a.shape[1] for a in isdatum_list], np.int32)
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(a, np.uint8).ravel() for a in isdatum_list]))
+        out = np.zeros(flat.size, np.int64)
+        self._chk(self.lib.elfel_hip_number_dofs(self.h, len(isdatum_list), _ptr(nterms, ctypes.c_int64),
+                                                 ncomp.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                 flat.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                                 _ptr(out, ctypes.c_int64)), "number_dofs")
+        res, o = [], 0
+        for a in isdatum_list:
+            res.append(out[o:o + a.size].reshape(a.shape).copy())
+            o += a.size
+        return res
+
+    def gather_sysvec(self, field, dofvals, v):
+        """v[dofnums - 1] = dofvals for the field bound by set_dofs (in place)."""
+        d = _f64(dofvals).ravel()
+        assert v.dtype == np.float64 and v.flags.c_contiguous
+        self._chk(self.lib.elfel_hip_gather_sysvec(self.h, field, d.size, _ptr(d, ctypes.c_double), v.size,
+                                                   _ptr(v, ctypes.c_double)), "gather_sysvec")
+        return v
+
+    def scatter_sysvec(self, field, v, dofvals):
+        """dofvals = v[dofnums - 1] where the dof lies within v (in place)."""
+        vv = _f64(v)
+        assert dofvals.dtype == np.float64 and dofvals.flags.c_contiguous
+        self._chk(self.lib.elfel_hip_scatter_sysvec(self.h, field, vv.size, _ptr(vv, ctypes.c_double), dofvals.size,
+                                                    _ptr(dofvals, ctypes.c_double)), "scatter_sysvec")
+        return dofvals
 
     def pcg(self, nfree, F, x, precision=FP64, tol=1e-10, maxit=100000):
         F = _f64(F)

@@ -140,6 +140,28 @@ int elfel_hip_geometry_stats(elfel_hip_ctx *ctx, double *min_det, int64_t *n_bad
 int elfel_hip_pcg(elfel_hip_ctx *ctx, int64_t nfree, const double *F, double *x, int precision,
                   double tol, int maxit, int *iters, int *outer, double *relres);
 
+/* ---- dof numbering and system vectors ----------------------------------------
+ * numberdofs! over nspaces spaces (src/FESpaces.jl:250-273 with
+ * src/FEFields.jl:137-177): isdatum holds, space after space, nterms[s] x
+ * ncomp[s] flags (entity-major, component-minor, nonzero = prescribed);
+ * dofnums receives the 1-based numbers in the same layout: the free dofs of
+ * each space consecutively (first space from 1, later spaces from the
+ * previous space's highest free number + 1), then the data dofs likewise
+ * (first from the last space's highest free number + 1). */
+int elfel_hip_number_dofs(elfel_hip_ctx *ctx, int32_t nspaces, const int64_t *nterms, const int32_t *ncomp,
+                          const uint8_t *isdatum, int64_t *dofnums);
+/* gathersysvec! of one field (src/FEFields.jl:247-257): v[dofnums[k] - 1] =
+ * dofvals[k] for every dof of the field bound by set_dofs; nvals = Nn * ncomp;
+ * entries of v that no dof maps to keep their values.  A dof number outside
+ * 1..ndofs -> ELFEL_ERANGE (BoundsError). */
+int elfel_hip_gather_sysvec(elfel_hip_ctx *ctx, int field, int64_t nvals, const double *dofvals, int64_t ndofs,
+                            double *v);
+/* scattersysvec! of one field (src/FEFields.jl:264-279): dofvals[k] =
+ * v[dofnums[k] - 1] for the dofs numbered within v (others keep theirs); an
+ * unnumbered dof -> ELFEL_ERANGE. */
+int elfel_hip_scatter_sysvec(elfel_hip_ctx *ctx, int field, int64_t ndofs, const double *v, int64_t nvals,
+                             double *dofvals);
+
 /* ---- batched problems ------------------------------------------------------
  * Many independent problems of uneven size (heat T3 1-point and plane
  * elasticity T6 3-point, mixed precision) assembled by one persistent fused
