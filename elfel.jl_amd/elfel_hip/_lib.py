@@ -32,6 +32,7 @@ EXPORTED = [
     "elfel_hip_batch_begin", "elfel_hip_batch_add", "elfel_hip_batch_build", "elfel_hip_batch_assemble",
     "elfel_hip_batch_info", "elfel_hip_batch_copy", "elfel_hip_batch_stats",
     "elfel_hip_number_dofs", "elfel_hip_gather_sysvec", "elfel_hip_scatter_sysvec",
+    "elfel_hip_colliding_errors",
 ]
 
 
@@ -89,6 +90,7 @@ def load():
                                           ctypes.POINTER(ctypes.c_uint8), i64p]
     lib.elfel_hip_gather_sysvec.argtypes = [P, ctypes.c_int, ctypes.c_int64, f64p, ctypes.c_int64, f64p]
     lib.elfel_hip_scatter_sysvec.argtypes = [P, ctypes.c_int, ctypes.c_int64, f64p, ctypes.c_int64, f64p]
+    lib.elfel_hip_colliding_errors.argtypes = [P, ctypes.c_int64, f64p, f64p, f64p]
     lib.elfel_hip_pcg.argtypes = [P, ctypes.c_int64, f64p, f64p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), f64p]
     lib.elfel_hip_batch_begin.argtypes = [P]
@@ -276,6 +278,15 @@ class Context:
         self._chk(self.lib.elfel_hip_scatter_sysvec(self.h, field, vv.size, _ptr(vv, ctypes.c_double), dofvals.size,
                                                     _ptr(dofvals, ctypes.c_double)), "scatter_sysvec")
         return dofvals
+
+    def colliding_errors(self, U):
+        """(pressure, velocity) L2 errors of the system vector U against the
+        colliding-flow exact fields; field 0 vector T6, field 1 scalar T3."""
+        U = _f64(U)
+        ep, ev = ctypes.c_double(), ctypes.c_double()
+        self._chk(self.lib.elfel_hip_colliding_errors(self.h, U.size, _ptr(U, ctypes.c_double), ctypes.byref(ep),
+                                                      ctypes.byref(ev)), "colliding_errors")
+        return ep.value, ev.value
 
     def pcg(self, nfree, F, x, precision=FP64, tol=1e-10, maxit=100000):
         F = _f64(F)

@@ -162,6 +162,18 @@ int elfel_hip_gather_sysvec(elfel_hip_ctx *ctx, int field, int64_t nvals, const 
 int elfel_hip_scatter_sysvec(elfel_hip_ctx *ctx, int field, int64_t ndofs, const double *v, int64_t nvals,
                              double *dofvals);
 
+/* ---- post-processing -------------------------------------------------------
+ * L2 errors of a Taylor-Hood solution against the colliding-flow exact
+ * fields p = 60x^2y - 20y^3, ux = 20xy^3, uy = 5x^4 - 5y^4
+ * (test/test_stokes.jl:354-356): ep = sqrt(sum_e sum_q JxW (p_h - p)^2) over
+ * field 1 (scalar T3) and ev = sqrt(sum_e sum_q JxW |u_h - u|^2) over field 0
+ * (vector T6), 3-point rules, U the full system vector (ndofs entries,
+ * 1-based dof numbers of set_dofs).  Replaces evaluate_pressure_error and
+ * evaluate_velocity_error (test/test_stokes.jl:438-496, :675-731); the
+ * element terms are summed in a fixed tree order, so the norms agree with
+ * the reference's left-to-right sum to rounding. */
+int elfel_hip_colliding_errors(elfel_hip_ctx *ctx, int64_t ndofs, const double *U, double *ep, double *ev);
+
 /* ---- batched problems ------------------------------------------------------
  * Many independent problems of uneven size (heat T3 1-point and plane
  * elasticity T6 3-point, mixed precision) assembled by one persistent fused
