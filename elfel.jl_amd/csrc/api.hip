@@ -1,8 +1,10 @@
 // api.hip -- the extern "C" entry points of libelfel_hip.so (include/elfel_hip.h).
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <new>
 
 #include "common.h"
@@ -21,41 +23,61 @@ int set_err(elfel_hip_ctx *c, int code, const std::string &msg)
     return code;
 }
 
+// Every buffer starts at its own "colour": an offset of a few KiB past the
+// allocation's (2 MiB-aligned) base, different for successive allocations.
+// Arrays indexed by the same element or node number (coefficients, node
+// numbers, coordinates) are read together; with power-of-two sizes (4096^2
+// elements) their bases would otherwise agree in every low address bit and
+// the simultaneous reads would fall on the same HBM channels (the heat
+// assembly on 4096^2 quads: 1.24 against 1.19 ms per launch with colours).
+static size_t next_colour()
+{
+    static const bool on = [] {
+        const char *v = getenv("ELFEL_HIP_BUF_COLOUR");
+        return !(v && *v && atoi(v) == 0);
+    }();
+    static std::atomic<unsigned> k{0};
+    if (!on) return 0;
+    return (size_t)((((k++) * 37u) % 61u) + 1u) * 768u;  // multiples of 256 B, up to ~46 KiB
+}
+
 int ensure(elfel_hip_ctx *c, Buf &b, size_t bytes)
 {
     if (bytes == 0) bytes = 16;
     if (b.bytes >= bytes) return 0;
-    if (b.p) {
+    if (b.base) {
         // The stream may still use the old buffer.
         hipError_t e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) return set_err(c, ELFEL_EHIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
-        hipFree(b.p);
-        b.p = nullptr;
+        hipFree(b.base);
+        b.p = b.base = nullptr;
         b.bytes = 0;
     }
+    const size_t colour = next_colour();
     size_t want = bytes + bytes / 8;
-    hipError_t e = hipMalloc(&b.p, want);
+    hipError_t e = hipMalloc(&b.base, want + colour);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        b.p = nullptr;
-        e = hipMalloc(&b.p, bytes);
+        b.base = nullptr;
+        e = hipMalloc(&b.base, bytes + colour);
         if (e != hipSuccess) {
             (void)hipGetLastError();
-            b.p = nullptr;
+            b.base = nullptr;
             char msg[128];
             snprintf(msg, sizeof msg, "hipMalloc of %zu bytes failed", bytes);
             return set_err(c, ELFEL_ENOMEM, msg);
         }
         want = bytes;
     }
+    b.p = (char *)b.base + colour;
     b.bytes = want;
     return 0;
 }
 
 void release(Buf &b)
 {
-    if (b.p) hipFree(b.p);
-    b.p = nullptr;
+    if (b.base) hipFree(b.base);
+    b.p = b.base = nullptr;
     b.bytes = 0;
 }
 

@@ -264,7 +264,7 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
         for (int i = 0; i < 9; i++) cf.D[i] = pb.D[i];
         if ((rc = pattern_to_host(c, &pb, tab, cf, hptrs[k], hinds[k]))) return rc;
         for (int bi = 0; bi < 2; bi++)
-            if ((rc = tile_tables(c, false, c->f[0], nullptr, p.ndofs, hptrs[k], hinds[k], bi ? 256 : 128, tabs[bi][k])))
+            if ((rc = tile_tables(c, false, c->f[0], nullptr, p.ndofs, hptrs[k], hinds[k], bi ? 256 : 128, bi ? 256 : 128, tabs[bi][k])))
                 return rc;
         p.nnz = (int64_t)hinds[k].size();
         p.elem_base = koff_E[p.kind];

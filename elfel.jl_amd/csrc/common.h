@@ -33,7 +33,8 @@ struct Coef {
 
 // ---- device buffers ---------------------------------------------------------
 struct Buf {
-    void *p = nullptr;
+    void *p = nullptr;     // start of the usable bytes (base + colour offset)
+    void *base = nullptr;  // what hipMalloc returned
     size_t bytes = 0;
 };
 

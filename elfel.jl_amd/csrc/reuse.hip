@@ -79,7 +79,8 @@ int pattern_to_host(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable 
 
 // Row groups in dof order packed into tiles of <= cap tasks and <= 24 cap values.
 int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1, int64_t n,
-                const std::vector<int32_t> &hptr, const std::vector<int32_t> &hind, int cap, ReuseTables &T)
+                const std::vector<int32_t> &hptr, const std::vector<int32_t> &hind, int cap, int ucap,
+                ReuseTables &T)
 {
     const int nen = F0.nen, nc = F0.ncomp, nd = nen * nc;
     const int ncol = nd + (stokes ? 3 : 0);
@@ -117,6 +118,11 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     // (element-staged tiles, reuse_kernel.h); for elasticity the staging was
     // measured slower (C2 83 against 68 us) and tasks compute their own geometry
     const bool staged = !stokes && nc == 1;
+    // staged tiles hold at most ucap distinct elements, one per thread of the
+    // workgroup: a second element on one thread would add a whole dependent
+    // load chain (element -> nodes -> coordinates) to the tile's critical path
+    std::vector<int32_t> stamp(staged ? F0.E : 0, -1);
+    int tile_id = 0, nu_open = 0;
     std::vector<int32_t> te;
     std::vector<uint32_t> ti;
     std::vector<uint8_t> tp;
@@ -162,6 +168,8 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
         T.trows.push_back(cr);
         T.maxv = std::max(T.maxv, cur.w);
         open = false;
+        tile_id++;
+        nu_open = 0;
     };
     int64_t r = 0;
     int32_t colbuf[16];
@@ -182,10 +190,16 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
             A = field == 0 ? &A0 : &A1;
             deg = A->ptr[node + 1] - A->ptr[node];
         }
-        if (deg > cap || nr * L > vcap || deg > 16 || L > 128)
+        if (deg > cap || nr * L > vcap || deg > 16 || L > 128 || (staged && deg > ucap))
             return set_err(c, ELFEL_ENOTSUP, "node too highly connected for the reuse tiles");
-        if (open && (cur.y + deg > cap || cur.w + nr * L > vcap || cr.y + nr > std::min(cap, REUSE_MAX_TILE_ROWS)))
+        int newu = 0;
+        if (staged)
+            for (int s = 0; s < deg; s++) newu += stamp[A->elem[A->ptr[node] + s]] != tile_id;
+        if (open && (cur.y + deg > cap || cur.w + nr * L > vcap || cr.y + nr > std::min(cap, REUSE_MAX_TILE_ROWS) ||
+                     nu_open + newu > ucap)) {
             close();
+            newu = deg;  // a node's elements are distinct
+        }
         if (!open) {
             cur = int4{(int)T.elem.size(), 0, hptr[r], 0};
             cr = int4{(int)r, 0, 0, 0};
@@ -217,12 +231,14 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
                 }
                 T.pos.push_back(pv);
             }
+            if (staged) stamp[e] = tile_id;
             T.elem.push_back(e);
             T.info.push_back((uint32_t)a | ((uint32_t)s << 3) | ((uint32_t)rowcode << 7) |
                              ((uint32_t)(field == 1) << 9) | ((uint32_t)vbase << 10) | ((uint32_t)l8 << 24));
             T.maxrank = std::max(T.maxrank, s + 1);
             cr.z = std::max(cr.z, s + 1);
         }
+        nu_open += newu;
         cur.y += deg;
         cur.w += nr * L;
         cr.y += nr;
@@ -325,7 +341,7 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
     if (nnz) HIPCHK(c, hipMemcpyAsync(P.ind.p, c->ind.p, 4 * nnz, hipMemcpyDeviceToDevice, c->stream));
     ReuseTables T;
     const int cap = block * reuse_tpt(pb->kind);  // task slots per tile
-    if ((rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, cap, T))) return rc;
+    if ((rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, cap, block, T))) return rc;
     if (env_int("ELFEL_HIP_TILE_ORDER", 1)) order_tiles(T, 8, cap);
     if ((rc = upload_tables(c, T, P))) return rc;
     P.block = cap;

@@ -275,18 +275,21 @@ __host__ __device__ constexpr int heat_rec() { return (NEN * NEN + NEN) | 1; }
 // column of its element's record.  The FP64 records repeat the reference's
 // operations (ke[i,j] += (grad N_i . grad N_j) kappa JxW, fe[j] += N_j Q JxW, in
 // point order), so every entry has the same bits as the per-task form.
+// The tile holds at most BLOCK distinct elements (the tile builder closes a
+// tile before a row would exceed that), so thread t computes element t of the
+// list, eg: its number, loaded by the caller before the task records.
 template <typename R, int NEN, int NQ, bool CSC, int BLOCK, int TPT>
-__device__ __forceinline__ void heat_tile(const ReuseArgs &A, const Coef &cf, const QTable &tab, double *lds, int tile,
-                                          int nU, const int32_t *e, const uint32_t *info, const bool *active,
-                                          int *rank, int *vbase, int *L8, R (*out)[2][NEN + 3], R *fo)
+__device__ __forceinline__ void heat_tile(const ReuseArgs &A, const Coef &cf, const QTable &tab, double *lds,
+                                          int32_t eg, int nU, const int32_t *e, const uint32_t *info,
+                                          const bool *active, int *rank, int *vbase, int *L8, R (*out)[2][NEN + 3],
+                                          R *fo)
 {
     constexpr bool MIXED = sizeof(R) == 4;
     constexpr int RECN = heat_rec<NEN>();
-    constexpr int SLOTS = BLOCK * TPT;
     constexpr QTable CT = qt_fixed<NEN, NQ>();
     R *rec = (R *)lds;
-    for (int le = threadIdx.x; le < nU; le += BLOCK) {
-        const int32_t eg = A.tile_elem[(int64_t)tile * SLOTS + le];
+    const int le = threadIdx.x;
+    if (le < nU) {
         double X[NEN], Y[NEN];
         if (A.geom) {
             const double *gp = (const double *)A.geom + (int64_t)eg * (2 * NEN);
@@ -405,6 +408,11 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
     uint32_t info[TPT];
     int32_t e[TPT];
     bool active[TPT];
+    // heat: this thread's element of the tile's list, first in the dependent
+    // chain element -> nodes -> coordinates, so issued before everything else
+    // (the list is padded to SLOTS entries per tile)
+    int32_t eg = 0;
+    if constexpr (KIND == ELFEL_HEAT) eg = A.tile_elem[(int64_t)tile * SLOTS + tid];
 #pragma unroll
     for (int u = 0; u < TPT; u++) {
         task[u] = (int64_t)tile * SLOTS + u * BLOCK + tid;
@@ -429,7 +437,7 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
 #pragma unroll
     for (int u = 0; u < TPT; u++) { rowcode[u] = 0; pown[u] = false; }
     if constexpr (KIND == ELFEL_HEAT) {
-        heat_tile<R, NEN, NQ, CSC, BLOCK, TPT>(A, cf, tab, lds, tile, tr.w, e, info, active, rank, vbase, L8, out, fo);
+        heat_tile<R, NEN, NQ, CSC, BLOCK, TPT>(A, cf, tab, lds, eg, tr.w, e, info, active, rank, vbase, L8, out, fo);
     } else {
     // node numbers (gather fused in) and per-element coefficients
     int32_t nd[TPT][NEN];
@@ -623,7 +631,8 @@ struct ReuseTables {
 int pattern_to_host(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf,
                     std::vector<int32_t> &hptr, std::vector<int32_t> &hind);
 int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1, int64_t n,
-                const std::vector<int32_t> &hptr, const std::vector<int32_t> &hind, int cap, ReuseTables &T);
+                const std::vector<int32_t> &hptr, const std::vector<int32_t> &hind, int cap, int ucap,
+                ReuseTables &T);
 int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P);
 
 typedef int (*reuse_launch_fn)(elfel_hip_ctx *, const ReuseArgs &, const Coef &, const QTable &, int);
@@ -637,6 +646,7 @@ template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC, int BLOCK>
 int launch_reuse(elfel_hip_ctx *c, const ReuseArgs &A, const Coef &cf, const QTable &tab, int lds_vals)
 {
     if (A.ntiles <= 0) return 0;
+    if (KIND == ELFEL_HEAT && A.maxu > BLOCK) return set_err(c, ELFEL_EINVAL, "internal: heat tile exceeds one element per thread");
     lds_vals = lds_span(lds_vals);
     if constexpr (KIND == ELFEL_HEAT)  // element records alias the value sums
         lds_vals = std::max(lds_vals, (int)(((size_t)A.maxu * heat_rec<NEN>() * sizeof(R) + 7) / 8));
