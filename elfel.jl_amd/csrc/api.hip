@@ -41,6 +41,21 @@ static size_t next_colour()
     return (size_t)((((k++) * 37u) % 61u) + 1u) * 768u;  // multiples of 256 B, up to ~46 KiB
 }
 
+// dst[padded(i, gap)] = src[i] for n entries of esize bytes (device to device,
+// on the context's stream): full blocks of 4096 entries by one strided copy,
+// the last partial block by a plain one.
+int copy_padded(elfel_hip_ctx *c, void *dst, const void *src, int64_t n, size_t esize, int gap)
+{
+    const int64_t blk = (int64_t)1 << PAD_LOG2, rows = n >> PAD_LOG2, rem = n & (blk - 1);
+    if (rows)
+        HIPCHK(c, hipMemcpy2DAsync(dst, esize * (size_t)(blk + gap), src, esize * (size_t)blk, esize * (size_t)blk,
+                                   (size_t)rows, hipMemcpyDeviceToDevice, c->stream));
+    if (rem)
+        HIPCHK(c, hipMemcpyAsync((char *)dst + esize * (size_t)(rows * (blk + gap)), (const char *)src + esize * (size_t)(rows * blk),
+                                 esize * (size_t)rem, hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
 int ensure(elfel_hip_ctx *c, Buf &b, size_t bytes)
 {
     if (bytes == 0) bytes = 16;
@@ -183,7 +198,7 @@ extern "C" int elfel_hip_free(elfel_hip_ctx *c)
                    &c->hist, &c->scan_tmp, &c->flags, &c->scal, &c->fkeys[0], &c->fkeys[1], &c->fvals[0],
                    &c->fvals[1], &c->stage64, &c->ptr, &c->ind, &c->val, &c->F, &c->dscal, &c->pcg, &c->pcg_sell,
                    &c->pat.ptr, &c->pat.ind, &c->pat.tile, &c->pat.tile_rows, &c->pat.task_elem,
-                   &c->pat.task_info, &c->pat.task_pos, &c->pat.tile_elem};
+                   &c->pat.task_info, &c->pat.task_pos, &c->pat.tile_elem, &c->pat.conn_pad, &c->pat.xy_pad};
     for (Buf *b : bufs) release(*b);
     batch_free(c);
     for (auto &p : c->tm.pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
@@ -423,9 +438,15 @@ extern "C" int elfel_hip_set_coefficients(elfel_hip_ctx *c, int64_t E, const dou
     for (auto &it : items) {
         *it.flag = false;
         if (!it.src) continue;
-        int rc = ensure(c, *it.dst, 8 * E);
+        // padded storage: one host-to-device copy into the staging buffer,
+        // then a strided device copy into blocks of 4096 entries + COEF_GAP
+        int rc = ensure(c, *it.dst, 8 * padded_len(E, COEF_GAP));
+        if (!rc) rc = ensure(c, c->stage64, 8 * E);
         if (rc) return rc;
-        if (E > 0) HIPCHK(c, hipMemcpyAsync(it.dst->p, it.src, 8 * E, hipMemcpyHostToDevice, c->stream));
+        if (E > 0) {
+            HIPCHK(c, hipMemcpyAsync(c->stage64.p, it.src, 8 * E, hipMemcpyHostToDevice, c->stream));
+            if ((rc = copy_padded(c, it.dst->p, c->stage64.p, E, 8, COEF_GAP))) return rc;
+        }
         *it.flag = true;
     }
     c->coef_E = E;
@@ -463,6 +484,7 @@ static int validate_problem(elfel_hip_ctx *c, const elfel_hip_problem *pb, QTabl
     cf->kappa_e = c->has_kappa_e ? (const double *)c->kappa_e.p : nullptr;
     cf->Q_e = c->has_Q_e ? (const double *)c->Q_e.p : nullptr;
     cf->scale_e = c->has_scale_e ? (const double *)c->scale_e.p : nullptr;
+    cf->cgap = COEF_GAP;
     int64_t ndofs = F.ndofs;
     if (stokes) ndofs = std::max(ndofs, c->f[1].ndofs);
     c->nrow = c->ncol = ndofs;

@@ -172,6 +172,7 @@ static ReuseArgs kind_args(KindData &k, int bi, double *vals, double *F)
     A.geom = nullptr;  // the kernel gathers conn and xy itself
     A.conn = (const int32_t *)k.conn.p;
     A.xy = (const double2 *)k.xy.p;
+    A.ngap = A.xgap = 0;  // the concatenated arrays are stored plain
     A.vals = vals;
     A.F = F;
     return A;

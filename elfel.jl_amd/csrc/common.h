@@ -24,11 +24,28 @@
 // from the kernel argument segment.
 static_assert(QT_T3 == ELFEL_T3 && QT_T6 == ELFEL_T6 && QT_Q4 == ELFEL_Q4, "element codes");
 
-// Coefficients of the weak form (uniform values + optional per-element arrays).
+// Padded storage of the large element- and node-indexed arrays that the
+// assembly kernels gather from: after every 4096 entries the array skips a
+// gap of a few entries, so entry i sits at padded(i, gap).  On a structured
+// mesh whose rows hold a power of two of elements (4096^2 quads) the element
+// rows of an array are otherwise an exact power of two of bytes apart, and the
+// rows that concurrent tiles read map to the same L2 sets: measured 23 against
+// 73 % L2 hits and 1.17 against 0.90 ms per heat assembly on 4096^2 quads.
+// Each array kind has its own gap (in entries), so that the rows of different
+// arrays do not collide with one another either.
+#define PAD_LOG2 12
+enum { COEF_GAP = 16, CONN_GAP = 24, XY_GAP = 40 };
+__host__ __device__ __forceinline__ int64_t padded(int64_t i, int gap) { return i + (i >> PAD_LOG2) * gap; }
+// entries to allocate for n padded entries
+__host__ __device__ __forceinline__ int64_t padded_len(int64_t n, int gap) { return padded(n, gap) + 1; }
+
+// Coefficients of the weak form (uniform values + optional per-element arrays,
+// stored padded with gap cgap: 0 for plain arrays).
 struct Coef {
     double kappa, Q, mu;
     double D[9];  // column-major
     const double *kappa_e, *Q_e, *scale_e;
+    int cgap;
 };
 
 // ---- device buffers ---------------------------------------------------------
@@ -66,6 +83,7 @@ struct Pattern {
     Buf task_info;               // uint32 per task
     Buf task_pos;                // uint8 [ntasks][ncolmax]
     Buf tile_elem;               // int32 [ntiles][slots]: distinct elements of a tile (heat)
+    Buf conn_pad, xy_pad;        // field 0's connectivity and coordinates, padded (CONN_GAP, XY_GAP)
     int maxu = 0;                // most distinct elements in one tile
 };
 
@@ -119,6 +137,7 @@ int set_err(elfel_hip_ctx *c, int code, const std::string &msg);
 
 // Grows a buffer to at least `bytes` (contents are not preserved).
 int ensure(elfel_hip_ctx *c, Buf &b, size_t bytes);
+int copy_padded(elfel_hip_ctx *c, void *dst, const void *src, int64_t n, size_t esize, int gap);
 void release(Buf &b);
 
 // ---- timing -----------------------------------------------------------------

@@ -83,10 +83,10 @@ __global__ __launch_bounds__(256) void k_elem_coo(int64_t E, const double2 *__re
     Geo<double, NEN, NQ> g;
     geometry<double, NEN, NQ>(tab, X, Y, g);
     double D[9];
-    const double sc = cf.scale_e ? cf.scale_e[e] : 1.0;
+    const double sc = cf.scale_e ? cf.scale_e[padded(e, cf.cgap)] : 1.0;
 #pragma unroll
     for (int i = 0; i < 9; i++) D[i] = cf.scale_e ? sc * cf.D[i] : cf.D[i];
-    const double kap = cf.kappa_e ? cf.kappa_e[e] : cf.kappa;
+    const double kap = cf.kappa_e ? cf.kappa_e[padded(e, cf.cgap)] : cf.kappa;
     auto key = [&](int32_t r, int32_t col) -> uint64_t {
         const uint64_t mj = (uint64_t)(csc ? col : r), mn = (uint64_t)(csc ? r : col);
         return (mj << minor_bits) | mn;
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void k_elem_coo(int64_t E, const double2 *__re
     }
     if constexpr (KIND == ELFEL_HEAT) {
         if (fkeys) {
-            const double Qe = cf.Q_e ? cf.Q_e[e] : cf.Q;
+            const double Qe = cf.Q_e ? cf.Q_e[padded(e, cf.cgap)] : cf.Q;
 #pragma unroll
             for (int j = 0; j < ND; j++) {
                 fkeys[e * ND + j] = (uint64_t)d[j];

@@ -344,6 +344,12 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
     if ((rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, cap, block, T))) return rc;
     if (env_int("ELFEL_HIP_TILE_ORDER", 1)) order_tiles(T, 8, cap);
     if ((rc = upload_tables(c, T, P))) return rc;
+    // the numeric kernel gathers from padded copies of the mesh arrays
+    if ((rc = ensure(c, P.conn_pad, 4 * (size_t)F0.nen * padded_len(F0.E, CONN_GAP))) ||
+        (rc = ensure(c, P.xy_pad, 16 * (size_t)padded_len(F0.Nn, XY_GAP))) ||
+        (rc = copy_padded(c, P.conn_pad.p, F0.conn.p, F0.E, 4 * (size_t)F0.nen, CONN_GAP)) ||
+        (rc = copy_padded(c, P.xy_pad.p, F0.xy.p, F0.Nn, 16, XY_GAP)))
+        return rc;
     P.block = cap;
     P.kind = pb->kind; P.fe = pb->fe; P.layout = pb->layout;
     P.key_mesh0 = F0.mesh_version; P.key_dof0 = F0.dof_version;
@@ -406,8 +412,10 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
     A.ncolp = P.ncolmax;
     A.ntiles = (int)P.ntiles;
     A.geom = fused ? nullptr : c->geom.p;
-    A.conn = (const int32_t *)F0.conn.p;
-    A.xy = (const double2 *)F0.xy.p;
+    A.conn = (const int32_t *)P.conn_pad.p;
+    A.xy = (const double2 *)P.xy_pad.p;
+    A.ngap = CONN_GAP;
+    A.xgap = XY_GAP;
     A.vals = (double *)c->val.p;
     A.F = heat ? (double *)c->F.p : nullptr;
     tm_begin(c, ELFEL_PH_NUMERIC, &ev);

@@ -51,8 +51,9 @@ struct ReuseArgs {
     int ncolp;
     int ntiles;
     const void *geom;         // per element [NEN][2] double from a gather kernel, or null:
-    const int32_t *conn;      //   then the kernel reads conn and xy itself
-    const double2 *xy;
+    const int32_t *conn;      //   then the kernel reads conn and xy itself,
+    const double2 *xy;        //   stored padded with gaps ngap and xgap (0: plain)
+    int ngap, xgap;
     double *vals;
     double *F;
 };
@@ -290,6 +291,11 @@ __device__ __forceinline__ void heat_tile(const ReuseArgs &A, const Coef &cf, co
     R *rec = (R *)lds;
     const int le = threadIdx.x;
     if (le < nU) {
+        // the coefficients first: issued with the node numbers, they arrive
+        // with them instead of adding a load round trip after the coordinates
+        double kap = cf.kappa, Qv = cf.Q;
+        if (cf.kappa_e) kap = cf.kappa_e[padded(eg, cf.cgap)];
+        if (cf.Q_e) Qv = cf.Q_e[padded(eg, cf.cgap)];
         double X[NEN], Y[NEN];
         if (A.geom) {
             const double *gp = (const double *)A.geom + (int64_t)eg * (2 * NEN);
@@ -298,17 +304,15 @@ __device__ __forceinline__ void heat_tile(const ReuseArgs &A, const Coef &cf, co
         } else {
             int32_t nd[NEN];
 #pragma unroll
-            for (int k = 0; k < NEN; k++) nd[k] = A.conn[(int64_t)eg * NEN + k];
+            for (int k = 0; k < NEN; k++) nd[k] = A.conn[padded(eg, A.ngap) * NEN + k];
 #pragma unroll
-            for (int k = 0; k < NEN; k++) { const double2 p = A.xy[nd[k]]; X[k] = p.x; Y[k] = p.y; }
+            for (int k = 0; k < NEN; k++) { const double2 p = A.xy[padded(nd[k], A.xgap)]; X[k] = p.x; Y[k] = p.y; }
             if constexpr (MIXED) {
                 const double x0 = X[0], y0 = Y[0];
 #pragma unroll
                 for (int k = 0; k < NEN; k++) { X[k] -= x0; Y[k] -= y0; }
             }
         }
-        const double kap = cf.kappa_e ? cf.kappa_e[eg] : cf.kappa;
-        const double Qv = cf.Q_e ? cf.Q_e[eg] : cf.Q;
         Geo<R, NEN, NQ> g;
         R *r = rec + (size_t)le * RECN;
         if constexpr (MIXED) {
@@ -448,11 +452,11 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
         if (active[u]) {
             if (!A.geom) {
 #pragma unroll
-                for (int k = 0; k < NEN; k++) nd[u][k] = A.conn[(int64_t)e[u] * NEN + k];
+                for (int k = 0; k < NEN; k++) nd[u][k] = A.conn[padded(e[u], A.ngap) * NEN + k];
             }
-            if (cf.scale_e) sc[u] = cf.scale_e[e[u]];
-            if (cf.kappa_e) kapd[u] = cf.kappa_e[e[u]];
-            if (cf.Q_e) Qd[u] = cf.Q_e[e[u]];
+            if (cf.scale_e) sc[u] = cf.scale_e[padded(e[u], cf.cgap)];
+            if (cf.kappa_e) kapd[u] = cf.kappa_e[padded(e[u], cf.cgap)];
+            if (cf.Q_e) Qd[u] = cf.Q_e[padded(e[u], cf.cgap)];
         }
     }
     // coordinates: FP64, absolute for the FP64 kernel, relative to the
@@ -467,7 +471,7 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
             for (int k = 0; k < NEN; k++) { X[u][k] = gp[2 * k]; Y[u][k] = gp[2 * k + 1]; }
         } else {
 #pragma unroll
-            for (int k = 0; k < NEN; k++) { const double2 p = A.xy[nd[u][k]]; X[u][k] = p.x; Y[u][k] = p.y; }
+            for (int k = 0; k < NEN; k++) { const double2 p = A.xy[padded(nd[u][k], A.xgap)]; X[u][k] = p.x; Y[u][k] = p.y; }
         }
     }
 #pragma unroll
