@@ -351,8 +351,14 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
                 cat.info.insert(cat.info.end(), t.info.begin(), t.info.end());
                 cat.pos.insert(cat.pos.end(), t.pos.begin(), t.pos.end());
             }
-            // (laying the tiles out in XCD-contiguous chunks, as the single-problem
-            // path does, measured slower here: 1.36 against 1.32 ms)
+            // tile t runs on XCD t mod 8 (one workgroup per tile, or a persistent
+            // grid of a multiple of 8 workgroups striding the list): runs of 64
+            // neighbouring tiles, which read the same elements, go to one XCD's
+            // L2, and each XCD gets runs from every part of the list.  This cuts
+            // the fetched bytes from 3.1 to 1.3 GB per launch at the same time
+            // (1.31 ms; the launch is not fetch-bound); whole chunks per XCD
+            // left the XCDs unevenly loaded (1.35 ms)
+            order_tiles(cat, 8, bi ? 256 : 128, env_int("ELFEL_HIP_XCD_RUN_BATCH", 64));
             if ((rc = upload_tables(c, cat, K.tab[bi]))) return rc;
         }
         K.ntiles = K.tab[1].ntiles;

@@ -46,14 +46,18 @@ void adjacency(const Field &f, Adj &A)
         }
 }
 
+bool rank_order_enabled();
+
+}  // namespace
+
 int env_int(const char *name, int dflt)
 {
     const char *v = getenv(name);
     return (v && *v) ? atoi(v) : dflt;
 }
 
+namespace {
 bool rank_order_enabled() { return env_int("ELFEL_HIP_RANK_ORDER", 1) != 0; }
-
 }  // namespace
 
 // Pattern of the problem described by the context's fields (plain method, FP64
@@ -252,8 +256,12 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
 // launch order is free.  Sorting by the first element a tile touches puts the
 // tiles that read the same element records close together; laying the sorted
 // list out so that workgroup b (dispatched to XCD b mod 8) walks a contiguous
-// chunk keeps those re-reads inside one XCD's L2.
-void order_tiles(ReuseTables &T, int nxcd, int cap)
+// chunk keeps those re-reads inside one XCD's L2.  With run > 0 the sorted list
+// is cut into runs of that many tiles, dealt to the XCDs in turn: the re-reads
+// stay inside runs, and every XCD gets a sample of the whole list, which
+// balances the XCDs when tile costs vary along it (batched problems of uneven
+// size).
+void order_tiles(ReuseTables &T, int nxcd, int cap, int run)
 {
     const size_t nt = T.tiles.size();
     if (nt < 2) return;
@@ -267,8 +275,25 @@ void order_tiles(ReuseTables &T, int nxcd, int cap)
         key[t] = {m, (int32_t)t};
     }
     std::stable_sort(key.begin(), key.end());
-    std::vector<int64_t> off(nxcd + 1, 0);
-    for (int x = 0; x < nxcd; x++) off[x + 1] = off[x] + (int64_t)(nt / nxcd + ((size_t)x < nt % nxcd ? 1 : 0));
+    // src[b]: the sorted position of the tile that launch slot b runs
+    std::vector<size_t> src(nt);
+    if (run <= 0) {
+        std::vector<int64_t> off(nxcd + 1, 0);
+        for (int x = 0; x < nxcd; x++) off[x + 1] = off[x] + (int64_t)(nt / nxcd + ((size_t)x < nt % nxcd ? 1 : 0));
+        for (size_t b = 0; b < nt; b++) src[b] = (size_t)(off[b % nxcd] + b / nxcd);
+    } else {
+        std::vector<std::vector<size_t>> lists(nxcd);
+        for (size_t i = 0; i < nt; i++) lists[(i / (size_t)run) % nxcd].push_back(i);
+        std::vector<size_t> next(nxcd, 0);
+        for (size_t b = 0; b < nt; b++) {
+            int x = (int)(b % nxcd);
+            if (next[x] == lists[x].size()) {  // this XCD's runs are used up: take from the fullest list
+                for (int y = 0; y < nxcd; y++)
+                    if (lists[y].size() - next[y] > lists[x].size() - next[x]) x = y;
+            }
+            src[b] = lists[x][next[x]++];
+        }
+    }
     std::vector<int4> tiles(nt), trows(nt);
     std::vector<int32_t> elem(T.elem.size());
     std::vector<uint32_t> info(T.info.size());
@@ -276,7 +301,7 @@ void order_tiles(ReuseTables &T, int nxcd, int cap)
     std::vector<int32_t> telem(T.telem.size());
     const size_t ncolp = (size_t)T.ncolp;
     for (size_t b = 0; b < nt; b++) {
-        const size_t o = (size_t)key[off[b % nxcd] + b / nxcd].second;
+        const size_t o = (size_t)key[src[b]].second;
         tiles[b] = T.tiles[o];
         trows[b] = T.trows[o];
         tiles[b].x = (int)(b * cap);  // the tasks move with their tile
@@ -345,7 +370,7 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
     ReuseTables T;
     const int cap = block * reuse_tpt(pb->kind);  // task slots per tile
     if ((rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, cap, block, T))) return rc;
-    if (env_int("ELFEL_HIP_TILE_ORDER", 1)) order_tiles(T, 8, cap);
+    if (env_int("ELFEL_HIP_TILE_ORDER", 1)) order_tiles(T, 8, cap, env_int("ELFEL_HIP_XCD_RUN", 0));
     if ((rc = upload_tables(c, T, P))) return rc;
     // the numeric kernel gathers from padded copies of the mesh arrays
     if ((rc = ensure(c, P.conn_pad, 4 * (size_t)F0.nen * padded_len(F0.E, CONN_GAP))) ||

@@ -639,8 +639,10 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
                 ReuseTables &T);
 int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P);
 // Sorts the tiles by their first element and lays them out so that workgroup b
-// (dispatched to XCD b mod nxcd) walks a contiguous chunk of the sorted list.
-void order_tiles(ReuseTables &T, int nxcd, int cap);
+// (dispatched to XCD b mod nxcd) walks a contiguous chunk of the sorted list
+// (run = 0), or runs of `run` tiles dealt to the XCDs in turn.
+void order_tiles(ReuseTables &T, int nxcd, int cap, int run);
+int env_int(const char *name, int dflt);
 
 typedef int (*reuse_launch_fn)(elfel_hip_ctx *, const ReuseArgs &, const Coef &, const QTable &, int);
 reuse_launch_fn pick_reuse64(int kind, int fe, int nq, int csc, int block);
