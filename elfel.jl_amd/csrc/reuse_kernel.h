@@ -268,6 +268,63 @@ __host__ __device__ constexpr int lds_span(int nvals) { return REUSE_LDS_SKEW ? 
 template <int NEN>
 __host__ __device__ constexpr int heat_rec() { return (NEN * NEN + NEN) | 1; }
 
+// Element matrix ke[i][j] and load vector fe[j] of one heat element into its
+// LDS record r (NEN*NEN + NEN values), from node coordinates X, Y (relative to
+// node 0 for the mixed kernel), conductivity kap and source Qv.
+template <typename R, int NEN, int NQ>
+__device__ __forceinline__ void heat_record(const QTable &tab, double kap, double Qv, const double *X, const double *Y,
+                                            R *r)
+{
+    constexpr bool MIXED = sizeof(R) == 4;
+    constexpr QTable CT = qt_fixed<NEN, NQ>();
+    Geo<R, NEN, NQ> g;
+    if constexpr (MIXED) {
+        geometry_mixed<NEN, NQ>(CT, X, Y, g);
+        float wx[NQ][NEN], wy[NQ][NEN];
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            const float w = (float)kap * g.jxw[q];
+#pragma unroll
+            for (int j = 0; j < NEN; j++) { wx[q][j] = g.gx[q][j] * w; wy[q][j] = g.gy[q][j] * w; }
+        }
+#pragma unroll
+        for (int i = 0; i < NEN; i++)
+#pragma unroll
+            for (int j = 0; j < NEN; j++) {
+                float acc = 0.f;
+#pragma unroll
+                for (int q = 0; q < NQ; q++) acc = fmaf(g.gx[q][i], wx[q][j], fmaf(g.gy[q][i], wy[q][j], acc));
+                r[i * NEN + j] = acc;
+            }
+#pragma unroll
+        for (int j = 0; j < NEN; j++) {
+            float acc = 0.f;
+#pragma unroll
+            for (int q = 0; q < NQ; q++) acc += CT.Nf[q][j] * (float)Qv * g.jxw[q];
+            r[NEN * NEN + j] = acc;
+        }
+    } else {
+        geometry<double, NEN, NQ>(tab, X, Y, g);
+#pragma unroll
+        for (int i = 0; i < NEN; i++)
+#pragma unroll
+            for (int j = 0; j < NEN; j++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int q = 0; q < NQ; q++)
+                    acc += (g.gx[q][i] * g.gx[q][j] + g.gy[q][i] * g.gy[q][j]) * (kap * g.jxw[q]);
+                r[i * NEN + j] = acc;
+            }
+#pragma unroll
+        for (int j = 0; j < NEN; j++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < NQ; q++) acc += tab.N[q][j] * Qv * g.jxw[q];
+            r[NEN * NEN + j] = acc;
+        }
+    }
+}
+
 // Element-staged heat tiles.  A scalar task owns one row (CSR) or column (CSC)
 // of NEN entries, so per-task geometry and loads would dominate; instead the
 // tile's distinct elements (tile_elem, ~half the task count) are computed once
@@ -287,7 +344,6 @@ __device__ __forceinline__ void heat_tile(const ReuseArgs &A, const Coef &cf, co
 {
     constexpr bool MIXED = sizeof(R) == 4;
     constexpr int RECN = heat_rec<NEN>();
-    constexpr QTable CT = qt_fixed<NEN, NQ>();
     R *rec = (R *)lds;
     const int le = threadIdx.x;
     if (le < nU) {
@@ -313,53 +369,7 @@ __device__ __forceinline__ void heat_tile(const ReuseArgs &A, const Coef &cf, co
                 for (int k = 0; k < NEN; k++) { X[k] -= x0; Y[k] -= y0; }
             }
         }
-        Geo<R, NEN, NQ> g;
-        R *r = rec + (size_t)le * RECN;
-        if constexpr (MIXED) {
-            geometry_mixed<NEN, NQ>(CT, X, Y, g);
-            float wx[NQ][NEN], wy[NQ][NEN];
-#pragma unroll
-            for (int q = 0; q < NQ; q++) {
-                const float w = (float)kap * g.jxw[q];
-#pragma unroll
-                for (int j = 0; j < NEN; j++) { wx[q][j] = g.gx[q][j] * w; wy[q][j] = g.gy[q][j] * w; }
-            }
-#pragma unroll
-            for (int i = 0; i < NEN; i++)
-#pragma unroll
-                for (int j = 0; j < NEN; j++) {
-                    float acc = 0.f;
-#pragma unroll
-                    for (int q = 0; q < NQ; q++) acc = fmaf(g.gx[q][i], wx[q][j], fmaf(g.gy[q][i], wy[q][j], acc));
-                    r[i * NEN + j] = acc;
-                }
-#pragma unroll
-            for (int j = 0; j < NEN; j++) {
-                float acc = 0.f;
-#pragma unroll
-                for (int q = 0; q < NQ; q++) acc += CT.Nf[q][j] * (float)Qv * g.jxw[q];
-                r[NEN * NEN + j] = acc;
-            }
-        } else {
-            geometry<double, NEN, NQ>(tab, X, Y, g);
-#pragma unroll
-            for (int i = 0; i < NEN; i++)
-#pragma unroll
-                for (int j = 0; j < NEN; j++) {
-                    double acc = 0.0;
-#pragma unroll
-                    for (int q = 0; q < NQ; q++)
-                        acc += (g.gx[q][i] * g.gx[q][j] + g.gy[q][i] * g.gy[q][j]) * (kap * g.jxw[q]);
-                    r[i * NEN + j] = acc;
-                }
-#pragma unroll
-            for (int j = 0; j < NEN; j++) {
-                double acc = 0.0;
-#pragma unroll
-                for (int q = 0; q < NQ; q++) acc += tab.N[q][j] * Qv * g.jxw[q];
-                r[NEN * NEN + j] = acc;
-            }
-        }
+        heat_record<R, NEN, NQ>(tab, kap, Qv, X, Y, rec + (size_t)le * RECN);
     }
     __syncthreads();
 #pragma unroll
