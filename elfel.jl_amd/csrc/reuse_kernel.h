@@ -122,9 +122,12 @@ __device__ __forceinline__ void owner_values(const QTable &tab, const Geo<R, NEN
             *fo = f;
             return;
         } else {
+            // both owner rows are formed even for one-row tasks (rare: a node
+            // with one free and one data dof), whose second row is never read:
+            // skipping it made the compiler keep two of the entries in scratch
+            (void)nr;
 #pragma unroll
             for (int c = 0; c < 2; c++) {
-                if (c >= nr) continue;
                 const int oc = comp0 + c;
                 if constexpr (BDB) {
                     R t[NQ][3];
