@@ -156,6 +156,10 @@ static void tune_store(const std::string &key, int block, int gmult)
     fclose(f);
 }
 
+// Task slots per tile: heat tiles hold two light tasks per thread (as in the
+// single-problem kernel), elasticity tiles one.
+static int batch_slots(int kind, int block) { return kind == 0 ? 2 * block : block; }
+
 static ReuseArgs kind_args(KindData &k, int bi, double *vals, double *F)
 {
     Pattern &P = k.tab[bi];
@@ -202,7 +206,8 @@ static int batch_launch(elfel_hip_ctx *c, Batch *b, int block, int grid)
     if (H.E)  // heat-T3 element records (floats) alias the value sums
         lds_vals = std::max(lds_vals, (H.tab[bi].maxu * heat_rec<3>() * 4 + 7) / 8);
     lds_vals = (lds_vals + 1) & ~1;  // even: the load-vector sums stay 16-byte aligned
-    const size_t lds = sizeof(double) * ((size_t)lds_vals + block);
+    // + the heat tiles' load-vector sums (up to REUSE_MAX_TILE_ROWS rows)
+    const size_t lds = sizeof(double) * ((size_t)lds_vals + REUSE_MAX_TILE_ROWS);
     const int g = grid > 0 ? std::min(grid, nt) : nt;
     const bool csc = b->layout == ELFEL_CSC;
     if (block == 128) {
@@ -265,7 +270,8 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
         for (int i = 0; i < 9; i++) cf.D[i] = pb.D[i];
         if ((rc = pattern_to_host(c, &pb, tab, cf, hptrs[k], hinds[k]))) return rc;
         for (int bi = 0; bi < 2; bi++)
-            if ((rc = tile_tables(c, false, c->f[0], nullptr, p.ndofs, hptrs[k], hinds[k], bi ? 256 : 128, bi ? 256 : 128, tabs[bi][k])))
+            if ((rc = tile_tables(c, false, c->f[0], nullptr, p.ndofs, hptrs[k], hinds[k],
+                                  batch_slots(p.kind, bi ? 256 : 128), bi ? 256 : 128, tabs[bi][k])))
                 return rc;
         p.nnz = (int64_t)hinds[k].size();
         p.elem_base = koff_E[p.kind];
@@ -358,7 +364,7 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
             // the fetched bytes from 3.1 to 1.3 GB per launch at the same time
             // (1.31 ms; the launch is not fetch-bound); whole chunks per XCD
             // left the XCDs unevenly loaded (1.35 ms)
-            order_tiles(cat, 8, bi ? 256 : 128, env_int("ELFEL_HIP_XCD_RUN_BATCH", 64));
+            order_tiles(cat, 8, batch_slots(kind, bi ? 256 : 128), env_int("ELFEL_HIP_XCD_RUN_BATCH", 64));
             if ((rc = upload_tables(c, cat, K.tab[bi]))) return rc;
         }
         K.ntiles = K.tab[1].ntiles;

@@ -601,7 +601,8 @@ void k_reuse(ReuseArgs A, Coef cf, QTable tab, int lds_vals)
 
 // Batched problems of uneven size in one persistent launch: heat-T3 (1-point)
 // tiles first, then elasticity-T6 (3-point) tiles; each workgroup walks the
-// tile list with stride gridDim.x.  The tile kind is uniform per workgroup.
+// tile list with stride gridDim.x.  The tile kind is uniform per workgroup;
+// heat tiles hold two tasks per thread, elasticity tiles one.
 template <bool CSC, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_reuse_batch(ReuseArgs H, ReuseArgs Ev, Coef cfh, Coef cfe, QTable th,
                                                        QTable te, int lds_vals)
@@ -609,7 +610,7 @@ __global__ __launch_bounds__(BLOCK) void k_reuse_batch(ReuseArgs H, ReuseArgs Ev
     extern __shared__ double lds[];
     const int nt = H.ntiles + Ev.ntiles;
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
-        if (t < H.ntiles) reuse_tile<ELFEL_HEAT, float, 3, 1, 1, CSC, BLOCK, 1>(H, cfh, th, lds, lds_vals, t);
+        if (t < H.ntiles) reuse_tile<ELFEL_HEAT, float, 3, 1, 1, CSC, BLOCK, 2>(H, cfh, th, lds, lds_vals, t);
         else reuse_tile<ELFEL_ELAST, float, 6, 2, 3, CSC, BLOCK, 1>(Ev, cfe, te, lds, lds_vals, t - H.ntiles);
         __syncthreads();
     }
