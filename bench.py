@@ -218,6 +218,8 @@ def bench_single(args, L, W):
     extra = {}
     if args.config == "c5" and not args.no_pcg:
         extra["pcg"] = bench_pcg(ctx, L, cs, res, args.pcg_maxit)
+    if args.config == "c5":
+        extra["device_setup"] = device_setup_c5(L, cs)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -298,6 +300,31 @@ def measured_traffic(cfgname, N_arg, N, block=None):
             return {}
     return {"traffic": t["traffic_bytes"], "traffic_fetch": t["fetch_bytes"], "traffic_write": t["write_bytes"],
             "traffic_source": t["source"] + " (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, x2 / x1 copy calibration)"}
+
+
+def device_setup_c5(L, cs):
+    """The heat-Q4 problem set up on the device instead of the host: block
+    mesh, boundary selection and dof numbering (elfel_hip_block_mesh,
+    elfel_hip_boundary_nodes, elfel_hip_number_dofs), checked against the
+    host-built dof numbers; wall time of the three calls, host round trips
+    included."""
+    N = cs["N"]
+    f = cs["cfg"]["fesp"]
+    c2 = L.Context(0)
+    c2.block_mesh(0, L.Q4, 1.0, 1.0, N, N)  # warm-up of the kernels and buffers
+    c2.synchronize()
+    t0 = time.perf_counter()
+    E, Nn = c2.block_mesh(0, L.Q4, 1.0, 1.0, N, N)
+    t1 = time.perf_counter()
+    b = c2.boundary_nodes(0, Nn)
+    t2 = time.perf_counter()
+    isd = np.zeros((Nn, 1), bool)
+    isd[b - 1] = True
+    dn = c2.number_dofs([isd])[0]
+    t3 = time.perf_counter()
+    c2.close()
+    return {"mesh_ms": (t1 - t0) * 1e3, "boundary_ms": (t2 - t1) * 1e3, "numbering_ms": (t3 - t2) * 1e3,
+            "dofs_equal_host": bool(np.array_equal(dn, f.dofnums)), "elements": E, "nodes": Nn}
 
 
 def bench_pcg(ctx, L, cs, res, args_maxit=60000):

@@ -33,6 +33,7 @@ EXPORTED = [
     "elfel_hip_batch_info", "elfel_hip_batch_copy", "elfel_hip_batch_stats",
     "elfel_hip_number_dofs", "elfel_hip_gather_sysvec", "elfel_hip_scatter_sysvec",
     "elfel_hip_colliding_errors",
+    "elfel_hip_block_mesh", "elfel_hip_t6_to_t3", "elfel_hip_boundary_nodes", "elfel_hip_copy_mesh",
 ]
 
 
@@ -91,6 +92,11 @@ def load():
     lib.elfel_hip_gather_sysvec.argtypes = [P, ctypes.c_int, ctypes.c_int64, f64p, ctypes.c_int64, f64p]
     lib.elfel_hip_scatter_sysvec.argtypes = [P, ctypes.c_int, ctypes.c_int64, f64p, ctypes.c_int64, f64p]
     lib.elfel_hip_colliding_errors.argtypes = [P, ctypes.c_int64, f64p, f64p, f64p]
+    lib.elfel_hip_block_mesh.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int64, ctypes.c_int64, i64p, i64p]
+    lib.elfel_hip_t6_to_t3.argtypes = [P, ctypes.c_int, ctypes.c_int, i64p]
+    lib.elfel_hip_boundary_nodes.argtypes = [P, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint8), i64p]
+    lib.elfel_hip_copy_mesh.argtypes = [P, ctypes.c_int, ctypes.c_int64, i64p, ctypes.c_int64, f64p]
     lib.elfel_hip_pcg.argtypes = [P, ctypes.c_int64, f64p, f64p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), f64p]
     lib.elfel_hip_batch_begin.argtypes = [P]
@@ -278,6 +284,38 @@ class Context:
         self._chk(self.lib.elfel_hip_scatter_sysvec(self.h, field, vv.size, _ptr(vv, ctypes.c_double), dofvals.size,
                                                     _ptr(dofvals, ctypes.c_double)), "scatter_sysvec")
         return dofvals
+
+    def block_mesh(self, field, fe, A, B, nA, nB):
+        """T3block / Q4block / T6block generated on the device into `field`;
+        returns (E, Nn)."""
+        E, Nn = ctypes.c_int64(), ctypes.c_int64()
+        self._chk(self.lib.elfel_hip_block_mesh(self.h, field, fe, A, B, nA, nB, ctypes.byref(E), ctypes.byref(Nn)),
+                  "block_mesh")
+        return E.value, Nn.value
+
+    def t6_to_t3(self, src, dst):
+        """T6toT3 of field src's mesh into field dst; returns the node count."""
+        n = ctypes.c_int64()
+        self._chk(self.lib.elfel_hip_t6_to_t3(self.h, src, dst, ctypes.byref(n)), "t6_to_t3")
+        return n.value
+
+    def boundary_nodes(self, field, Nn):
+        """1-based numbers of the boundary nodes of `field`'s mesh (connectedv(boundary))."""
+        flags = np.zeros(Nn, np.uint8)
+        cnt = ctypes.c_int64()
+        self._chk(self.lib.elfel_hip_boundary_nodes(self.h, field, Nn, _ptr(flags, ctypes.c_uint8),
+                                                    ctypes.byref(cnt)), "boundary_nodes")
+        b = np.nonzero(flags)[0] + 1
+        assert b.size == cnt.value
+        return b
+
+    def copy_mesh(self, field, E, nen, Nn):
+        """(conn E x nen 1-based int64, xy Nn x 2) of `field`'s mesh."""
+        conn = np.zeros((E, nen), np.int64)
+        xy = np.zeros((Nn, 2))
+        self._chk(self.lib.elfel_hip_copy_mesh(self.h, field, E, _ptr(conn, ctypes.c_int64), Nn,
+                                               _ptr(xy, ctypes.c_double)), "copy_mesh")
+        return conn, xy
 
     def colliding_errors(self, U):
         """(pressure, velocity) L2 errors of the system vector U against the

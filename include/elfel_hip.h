@@ -162,6 +162,27 @@ int elfel_hip_gather_sysvec(elfel_hip_ctx *ctx, int field, int64_t nvals, const 
 int elfel_hip_scatter_sysvec(elfel_hip_ctx *ctx, int field, int64_t ndofs, const double *v, int64_t nvals,
                              double *dofvals);
 
+/* ---- meshes generated on the device -----------------------------------------
+ * The structured generators of the reference's drivers, written straight into
+ * field `field` (as elfel_hip_set_mesh would, with the same element and node
+ * order and the same coordinate bits as the generators): T3block
+ * (test/test_heat.jl:29), Q4block (:144) and T6block (test/test_stokes.jl:32)
+ * of an A x B rectangle with nA x nB cells; fe = ELFEL_T3, ELFEL_Q4 or
+ * ELFEL_T6.  *E and *Nn receive the element and node counts. */
+int elfel_hip_block_mesh(elfel_hip_ctx *ctx, int field, int fe, double A, double B, int64_t nA, int64_t nB,
+                         int64_t *E, int64_t *Nn);
+/* T6toT3 (test/test_stokes.jl:37): field dst receives the T3 mesh on the
+ * corner nodes of field src's T6 mesh, renumbered compactly in increasing T6
+ * node order; *Nn receives its node count. */
+int elfel_hip_t6_to_t3(elfel_hip_ctx *ctx, int src, int dst, int64_t *Nn);
+/* connectedv(boundary(mesh)) of field `field` (test/test_heat.jl:31): flags[n]
+ * = 1 for the nodes of the edges that belong to one element (T6: with the
+ * edge's midside node), else 0; Nn must equal the field's node count. */
+int elfel_hip_boundary_nodes(elfel_hip_ctx *ctx, int field, int64_t Nn, uint8_t *flags, int64_t *count);
+/* Copies field `field`'s mesh to the host: conn (E x nen, 1-based, row per
+ * element) and xy (Nn x 2); either may be null. */
+int elfel_hip_copy_mesh(elfel_hip_ctx *ctx, int field, int64_t E, int64_t *conn, int64_t Nn, double *xy);
+
 /* ---- post-processing -------------------------------------------------------
  * L2 errors of a Taylor-Hood solution against the colliding-flow exact
  * fields p = 60x^2y - 20y^3, ux = 20xy^3, uy = 5x^4 - 5y^4
