@@ -18,6 +18,9 @@
 //   3 c + (i + [j > 0]) + (i = 0 ? j : nB), c = i nB + j; every cell computes
 //   its nodes without looking at any other;
 // * T6 -> T3: the corner nodes renumbered compactly in increasing T6 order.
+#include <stdio.h>
+#include <stdlib.h>
+
 #include "common.h"
 
 namespace {
@@ -292,5 +295,85 @@ extern "C" int elfel_hip_copy_mesh(elfel_hip_ctx *c, int field, int64_t E, int64
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (conn)
         for (int64_t i = 0; i < E * f.nen; i++) conn[i] = (int64_t)f.h_conn[i] + 1;
+    return ELFEL_OK;
+}
+
+// ---- mesh files ----------------------------------------------------------------
+// The reference's on-disk mesh (MeshSteward's load(Mesh(), "name.mesh"), as the
+// tests read test/qmesh.mesh and test/mt3gen3.mesh): a three-line manifest --
+// coordinate file, shape name, connectivity file -- next to two whitespace-
+// separated text tables, one row per node (x y [z]) and one row per element
+// (1-based node numbers).
+namespace {
+
+bool read_table(const std::string &path, int want_cols_min, std::vector<std::vector<double>> &rows)
+{
+    FILE *fh = fopen(path.c_str(), "r");
+    if (!fh) return false;
+    char line[4096];
+    while (fgets(line, sizeof line, fh)) {
+        std::vector<double> r;
+        char *p = line, *end = nullptr;
+        for (;;) {
+            const double v = strtod(p, &end);
+            if (end == p) break;
+            r.push_back(v);
+            p = end;
+        }
+        if (r.empty()) continue;
+        if ((int)r.size() < want_cols_min) { fclose(fh); return false; }
+        rows.push_back(std::move(r));
+    }
+    fclose(fh);
+    return true;
+}
+
+std::string trim(const char *s)
+{
+    std::string t(s);
+    const size_t a = t.find_first_not_of(" \t\r\n"), b = t.find_last_not_of(" \t\r\n");
+    return a == std::string::npos ? std::string() : t.substr(a, b - a + 1);
+}
+
+}  // namespace
+
+extern "C" int elfel_hip_load_mesh(elfel_hip_ctx *c, int field, const char *path, int32_t *nen_out, int64_t *E_out,
+                                   int64_t *Nn_out)
+{
+    if (!c) return ELFEL_EINVAL;
+    c->err.clear();
+    if (!path || field < 0 || field > 1) return set_err(c, ELFEL_EINVAL, "elfel_hip_load_mesh: bad arguments");
+    FILE *fh = fopen(path, "r");
+    if (!fh) return set_err(c, ELFEL_EINVAL, std::string("cannot open ") + path);
+    std::vector<std::string> lines;
+    char buf[4096];
+    while (fgets(buf, sizeof buf, fh)) {
+        const std::string t = trim(buf);
+        if (!t.empty()) lines.push_back(t);
+    }
+    fclose(fh);
+    if (lines.size() < 3) return set_err(c, ELFEL_EINVAL, "mesh manifest needs three lines: xyz file, shape, conn file");
+    const std::string p(path);
+    const size_t slash = p.find_last_of('/');
+    const std::string dir = slash == std::string::npos ? std::string() : p.substr(0, slash + 1);
+    const std::string &shape = lines[1];
+    const int nen = shape == "T3" ? 3 : shape == "Q4" ? 4 : shape == "T6" ? 6 : 0;
+    if (!nen) return set_err(c, ELFEL_ENOTSUP, "mesh shape " + shape + " (T3, Q4 and T6 are supported)");
+    std::vector<std::vector<double>> xyz, cn;
+    if (!read_table(dir + lines[0], 2, xyz)) return set_err(c, ELFEL_EINVAL, "cannot read " + dir + lines[0]);
+    if (!read_table(dir + lines[2], nen, cn)) return set_err(c, ELFEL_EINVAL, "cannot read " + dir + lines[2]);
+    const int64_t Nn = (int64_t)xyz.size(), E = (int64_t)cn.size();
+    std::vector<double> xy(2 * (size_t)Nn);
+    for (int64_t n = 0; n < Nn; n++) { xy[2 * n] = xyz[n][0]; xy[2 * n + 1] = xyz[n][1]; }
+    std::vector<int64_t> conn((size_t)(E * nen));
+    for (int64_t e = 0; e < E; e++) {
+        if ((int)cn[e].size() != nen) return set_err(c, ELFEL_EINVAL, "connectivity row with a wrong node count");
+        for (int k = 0; k < nen; k++) conn[e * nen + k] = (int64_t)cn[e][k];
+    }
+    const int rc = elfel_hip_set_mesh(c, field, nen, E, conn.data(), Nn, xy.data());
+    if (rc) return rc;
+    if (nen_out) *nen_out = nen;
+    if (E_out) *E_out = E;
+    if (Nn_out) *Nn_out = Nn;
     return ELFEL_OK;
 }

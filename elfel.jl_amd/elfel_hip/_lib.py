@@ -34,6 +34,7 @@ EXPORTED = [
     "elfel_hip_number_dofs", "elfel_hip_gather_sysvec", "elfel_hip_scatter_sysvec",
     "elfel_hip_colliding_errors",
     "elfel_hip_block_mesh", "elfel_hip_t6_to_t3", "elfel_hip_boundary_nodes", "elfel_hip_copy_mesh",
+    "elfel_hip_load_mesh",
 ]
 
 
@@ -97,6 +98,7 @@ def load():
     lib.elfel_hip_t6_to_t3.argtypes = [P, ctypes.c_int, ctypes.c_int, i64p]
     lib.elfel_hip_boundary_nodes.argtypes = [P, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint8), i64p]
     lib.elfel_hip_copy_mesh.argtypes = [P, ctypes.c_int, ctypes.c_int64, i64p, ctypes.c_int64, f64p]
+    lib.elfel_hip_load_mesh.argtypes = [P, ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), i64p, i64p]
     lib.elfel_hip_pcg.argtypes = [P, ctypes.c_int64, f64p, f64p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), f64p]
     lib.elfel_hip_batch_begin.argtypes = [P]
@@ -308,6 +310,13 @@ class Context:
         b = np.nonzero(flags)[0] + 1
         assert b.size == cnt.value
         return b
+
+    def load_mesh(self, field, path):
+        """Reads a .mesh manifest into `field`; returns (nen, E, Nn)."""
+        nen, E, Nn = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64()
+        self._chk(self.lib.elfel_hip_load_mesh(self.h, field, os.fsencode(path), ctypes.byref(nen), ctypes.byref(E),
+                                               ctypes.byref(Nn)), "load_mesh")
+        return nen.value, E.value, Nn.value
 
     def copy_mesh(self, field, E, nen, Nn):
         """(conn E x nen 1-based int64, xy Nn x 2) of `field`'s mesh."""

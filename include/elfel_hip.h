@@ -179,6 +179,11 @@ int elfel_hip_t6_to_t3(elfel_hip_ctx *ctx, int src, int dst, int64_t *Nn);
  * = 1 for the nodes of the edges that belong to one element (T6: with the
  * edge's midside node), else 0; Nn must equal the field's node count. */
 int elfel_hip_boundary_nodes(elfel_hip_ctx *ctx, int field, int64_t Nn, uint8_t *flags, int64_t *count);
+/* load(Mesh(), "name.mesh") of MeshSteward, as the reference's tests read
+ * test/qmesh.mesh and test/mt3gen3.mesh: a manifest of three lines (coordinate
+ * file, shape T3/Q4/T6, connectivity file, both text tables next to the
+ * manifest) read on the host and set as field `field`'s mesh. */
+int elfel_hip_load_mesh(elfel_hip_ctx *ctx, int field, const char *path, int32_t *nen, int64_t *E, int64_t *Nn);
 /* Copies field `field`'s mesh to the host: conn (E x nen, 1-based, row per
  * element) and xy (Nn x 2); either may be null. */
 int elfel_hip_copy_mesh(elfel_hip_ctx *ctx, int field, int64_t E, int64_t *conn, int64_t Nn, double *xy);

@@ -76,9 +76,10 @@ def test_boundary_nodes_of_fixture_and_jittered_meshes(ctx):
     assert np.array_equal(ctx.boundary_nodes(0, mesh.nnodes), mesh.boundary_nodes())
 
 
-def test_device_pipeline_equals_host_built_assembly(ctx):
+def test_device_pipeline_equals_host_built_assembly():
     """heat Q4 on 64x64: block mesh, boundary, numbering on the device, then the
-    FP64 assembly; the host workload builds the same problem on the host."""
+    FP64 assembly; the host workload builds the same problem on the host.  Own
+    contexts: the per-element coefficients set here must not reach other tests."""
     N = 64
     cfg = W.heat_q4(N, sigma=1.0)
     f = cfg["fesp"]
@@ -91,6 +92,7 @@ def test_device_pipeline_equals_host_built_assembly(ctx):
     Fref = host.copy_vector(cfg["ndofs"])
     host.close()
 
+    ctx = L.Context(0)
     E, Nn = ctx.block_mesh(0, L.Q4, 1.0, 1.0, N, N)
     b = ctx.boundary_nodes(0, Nn)
     isd = np.zeros((Nn, 1), bool)
@@ -104,6 +106,7 @@ def test_device_pipeline_equals_host_built_assembly(ctx):
     for a, r in zip(got[:3], ref[:3]):
         assert np.array_equal(a, r)
     assert np.array_equal(ctx.copy_vector(Nn), Fref)
+    ctx.close()
 
 
 def test_block_mesh_at_workload_sizes(ctx):
@@ -120,3 +123,16 @@ def test_block_mesh_rejects_bad_arguments(ctx):
         ctx.block_mesh(0, L.T6, 1.0, 1.0, 0, 4)
     with pytest.raises(L.ElfelHipError):
         ctx.block_mesh(2, L.T3, 1.0, 1.0, 4, 4)
+
+
+@pytest.mark.parametrize("name", ["qmesh.mesh", "mt3gen3.mesh"])
+def test_load_mesh_matches_host_reader(ctx, name):
+    """The reference's fixture meshes (test/qmesh.mesh, test/mt3gen3.mesh) read
+    by the library equal the host reader's arrays."""
+    path = os.path.join(GOLDEN, name)
+    nen, E, Nn = ctx.load_mesh(0, path)
+    conn, xy = ctx.copy_mesh(0, E, nen, Nn)
+    m = M.load(path)
+    assert np.array_equal(conn, m.conn) and np.array_equal(xy, m.xy)
+    with pytest.raises(L.ElfelHipError):
+        ctx.load_mesh(0, os.path.join(GOLDEN, "no-such.mesh"))
