@@ -16,7 +16,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <thread>
 
 #include "reuse_kernel.h"
 
@@ -81,57 +83,45 @@ int pattern_to_host(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable 
     return 0;
 }
 
-// Row groups in dof order packed into tiles of <= cap tasks and <= 24 cap values.
-int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1, int64_t n,
-                const std::vector<int32_t> &hptr, const std::vector<int32_t> &hind, int cap, int ucap,
-                ReuseTables &T)
+namespace {
+
+// What every row range of the tile builder reads.
+struct TileInput {
+    bool stokes, staged, by_rank;
+    int nen, nc, ncol, ncolp, cap, ucap, vcap;
+    int64_t n;
+    const Field *F0, *F1;
+    const Adj *A0, *A1;
+    const std::vector<int32_t> *hptr, *hind, *inv_node;
+    const std::vector<uint8_t> *inv_fc;
+};
+
+// Rows r0..r1-1 (whole row groups) packed into tiles; tile task offsets are
+// local to T.  stamp / tile_id belong to the calling worker (a running tile
+// counter, so stamp never needs clearing).
+int tile_range(const TileInput &I, int64_t r0, int64_t r1, ReuseTables &T, std::vector<int32_t> &stamp,
+               int &tile_id, std::string &err)
 {
-    const int nen = F0.nen, nc = F0.ncomp, nd = nen * nc;
-    const int ncol = nd + (stokes ? 3 : 0);
-    const int ncolp = (ncol + 3) & ~3;  // one byte per column: position (bits 0-6), first contribution (bit 7)
-    const int vcap = std::min(std::max(env_int("ELFEL_HIP_VCAP", REUSE_VCAP_PER_THREAD), 8), 24) * cap;
-    Adj A0, A1;
-    adjacency(F0, A0);
-    if (stokes) adjacency(*F1, A1);
-    std::vector<int32_t> inv_node(n, -1);
-    std::vector<uint8_t> inv_fc(n, 0);  // field << 4 | comp
-    for (int64_t nd0 = 0; nd0 < F0.Nn; nd0++)
-        for (int cc = 0; cc < nc; cc++) {
-            const int32_t d = F0.h_dof[nd0 * nc + cc];
-            inv_node[d] = (int32_t)nd0;
-            inv_fc[d] = (uint8_t)cc;
-        }
-    if (stokes)
-        for (int64_t m = 0; m < F1->Nn; m++) {
-            const int32_t d = F1->h_dof[m];
-            inv_node[d] = (int32_t)m;
-            inv_fc[d] = (uint8_t)(1 << 4);
-        }
+    const int nen = I.nen, nc = I.nc, ncolp = I.ncolp, cap = I.cap, ucap = I.ucap, vcap = I.vcap;
+    const bool stokes = I.stokes, staged = I.staged, by_rank = I.by_rank;
+    const Field &F0 = *I.F0;
+    const Field *F1 = I.F1;
+    const std::vector<int32_t> &hptr = *I.hptr, &hind = *I.hind, &inv_node = *I.inv_node;
+    const std::vector<uint8_t> &inv_fc = *I.inv_fc;
+    const int64_t n = I.n;
     T = ReuseTables{};
     T.ncolp = ncolp;
-    T.elem.reserve((size_t)F0.E * nen + (stokes ? F0.E * 3 : 0));
-    T.info.reserve(T.elem.capacity());
-    T.pos.reserve(T.elem.capacity() * ncolp);
     int4 cur = {0, 0, 0, 0}, cr = {0, 0, 0, 0};
     bool open = false;
-    // Within a tile the tasks are laid out rank by rank (stable), so the
-    // threads that add in one rank step sit in as few waves as possible and
-    // every other wave skips that step.
-    const bool by_rank = rank_order_enabled();
-    // scalar problems: the kernel computes each distinct element of a tile once
-    // (element-staged tiles, reuse_kernel.h); for elasticity the staging was
-    // measured slower (C2 83 against 68 us) and tasks compute their own geometry
-    const bool staged = !stokes && nc == 1;
-    // staged tiles hold at most ucap distinct elements, one per thread of the
-    // workgroup: a second element on one thread would add a whole dependent
-    // load chain (element -> nodes -> coordinates) to the tile's critical path
-    std::vector<int32_t> stamp(staged ? F0.E : 0, -1);
-    int tile_id = 0, nu_open = 0;
+    int nu_open = 0;
     std::vector<int32_t> te;
     std::vector<uint32_t> ti;
     std::vector<uint8_t> tp;
     auto close = [&]() {
         if (!open) return;
+        // Within a tile the tasks are laid out rank by rank (stable), so the
+        // threads that add in one rank step sit in as few waves as possible and
+        // every other wave skips that step.
         if (by_rank && cur.y > 1) {
             const size_t t0 = (size_t)cur.x, nt = (size_t)cur.y;
             te.assign(T.elem.begin() + t0, T.elem.end());
@@ -175,10 +165,10 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
         tile_id++;
         nu_open = 0;
     };
-    int64_t r = 0;
+    int64_t r = r0;
     int32_t colbuf[16];
     std::vector<uint8_t> seen;
-    while (r < n) {
+    while (r < r1) {
         const int32_t node = inv_node[r];
         const int field = inv_fc[r] >> 4, comp = inv_fc[r] & 15;
         int nr = 1;
@@ -186,16 +176,17 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
             inv_fc[r + 1] == 1)
             nr = 2;
         const int L = hptr[r + 1] - hptr[r];
-        if (nr == 2 && hptr[r + 2] - hptr[r + 1] != L)
-            return set_err(c, ELFEL_ENOTSUP, "rows of one node differ in pattern");
+        if (nr == 2 && hptr[r + 2] - hptr[r + 1] != L) { err = "rows of one node differ in pattern"; return ELFEL_ENOTSUP; }
         int deg = 0;
         const Adj *A = nullptr;
         if (node >= 0) {
-            A = field == 0 ? &A0 : &A1;
+            A = field == 0 ? I.A0 : I.A1;
             deg = A->ptr[node + 1] - A->ptr[node];
         }
-        if (deg > cap || nr * L > vcap || deg > 16 || L > 128 || (staged && deg > ucap))
-            return set_err(c, ELFEL_ENOTSUP, "node too highly connected for the reuse tiles");
+        if (deg > cap || nr * L > vcap || deg > 16 || L > 128 || (staged && deg > ucap)) {
+            err = "node too highly connected for the reuse tiles";
+            return ELFEL_ENOTSUP;
+        }
         int newu = 0;
         if (staged)
             for (int s = 0; s < deg; s++) newu += stamp[A->elem[A->ptr[node] + s]] != tile_id;
@@ -227,8 +218,10 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
                 uint8_t pv = 0;
                 if (j < nj) {
                     const int32_t *it = std::lower_bound(list, list + L, colbuf[j]);
-                    if (it == list + L || *it != colbuf[j])
-                        return set_err(c, ELFEL_EINVAL, "internal: element column missing from the pattern");
+                    if (it == list + L || *it != colbuf[j]) {
+                        err = "internal: element column missing from the pattern";
+                        return ELFEL_EINVAL;
+                    }
                     pv = (uint8_t)(it - list);
                     // bit 7: this task is the first (lowest rank) to add into the entry
                     if (!seen[pv]) { seen[pv] = 1; pv |= 0x80; }
@@ -249,6 +242,107 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
         r += nr;
     }
     close();
+    return 0;
+}
+
+}  // namespace
+
+// Row groups in dof order packed into tiles of <= cap tasks and <= 24 cap
+// values.  The rows are cut into ranges of about 2^18 rows at row-group
+// boundaries -- a fixed cut, whatever the number of host threads, so the
+// tiles are the same on every machine -- and the ranges are packed in
+// parallel; a tile never spans two ranges.
+int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1, int64_t n,
+                const std::vector<int32_t> &hptr, const std::vector<int32_t> &hind, int cap, int ucap,
+                ReuseTables &T)
+{
+    const int nen = F0.nen, nc = F0.ncomp, nd = nen * nc;
+    const int ncol = nd + (stokes ? 3 : 0);
+    TileInput I;
+    I.stokes = stokes;
+    // scalar problems: the kernel computes each distinct element of a tile once
+    // (element-staged tiles, reuse_kernel.h); for elasticity the staging was
+    // measured slower (C2 83 against 68 us) and tasks compute their own geometry.
+    // Staged tiles hold at most ucap distinct elements, one per thread of the
+    // workgroup: a second element on one thread would add a whole dependent
+    // load chain (element -> nodes -> coordinates) to the tile's critical path.
+    I.staged = !stokes && nc == 1;
+    I.by_rank = rank_order_enabled();
+    I.nen = nen; I.nc = nc; I.ncol = ncol;
+    I.ncolp = (ncol + 3) & ~3;  // one byte per column: position (bits 0-6), first contribution (bit 7)
+    I.cap = cap; I.ucap = ucap;
+    I.vcap = std::min(std::max(env_int("ELFEL_HIP_VCAP", REUSE_VCAP_PER_THREAD), 8), 24) * cap;
+    I.n = n;
+    I.F0 = &F0; I.F1 = F1;
+    Adj A0, A1;
+    adjacency(F0, A0);
+    if (stokes) adjacency(*F1, A1);
+    I.A0 = &A0; I.A1 = &A1;
+    std::vector<int32_t> inv_node(n, -1);
+    std::vector<uint8_t> inv_fc(n, 0);  // field << 4 | comp
+    for (int64_t nd0 = 0; nd0 < F0.Nn; nd0++)
+        for (int cc = 0; cc < nc; cc++) {
+            const int32_t d = F0.h_dof[nd0 * nc + cc];
+            inv_node[d] = (int32_t)nd0;
+            inv_fc[d] = (uint8_t)cc;
+        }
+    if (stokes)
+        for (int64_t m = 0; m < F1->Nn; m++) {
+            const int32_t d = F1->h_dof[m];
+            inv_node[d] = (int32_t)m;
+            inv_fc[d] = (uint8_t)(1 << 4);
+        }
+    I.hptr = &hptr; I.hind = &hind; I.inv_node = &inv_node; I.inv_fc = &inv_fc;
+    // range cuts, moved forward off the second row of a two-row group
+    const int64_t nranges = std::max<int64_t>(1, std::min<int64_t>(1024, n >> 18));
+    std::vector<int64_t> cut(nranges + 1);
+    for (int64_t k = 0; k <= nranges; k++) {
+        int64_t r = n * k / nranges;
+        while (r > 0 && r < n && inv_node[r] >= 0 && inv_node[r] == inv_node[r - 1] && (inv_fc[r] >> 4) == 0 &&
+               (inv_fc[r - 1] >> 4) == 0 && nc == 2 && (inv_fc[r] & 15) == 1 && (inv_fc[r - 1] & 15) == 0)
+            r++;
+        cut[k] = std::max(r, k ? cut[k - 1] : 0);
+    }
+    std::vector<ReuseTables> part(nranges);
+    std::vector<int> prc(nranges, 0);
+    std::vector<std::string> perr(nranges);
+    const int nthreads = (int)std::max<int64_t>(1, std::min<int64_t>(
+        nranges, std::min<int64_t>(16, std::max(1u, std::thread::hardware_concurrency()))));
+    std::atomic<int64_t> next{0};
+    auto worker = [&]() {
+        std::vector<int32_t> stamp(I.staged ? F0.E : 0, -1);
+        int tile_id = 0;
+        for (;;) {
+            const int64_t k = next.fetch_add(1);
+            if (k >= nranges) break;
+            prc[k] = tile_range(I, cut[k], cut[k + 1], part[k], stamp, tile_id, perr[k]);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthreads; t++) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+    for (int64_t k = 0; k < nranges; k++)
+        if (prc[k]) return set_err(c, prc[k], perr[k]);
+    // concatenation: tile task offsets become global
+    T = ReuseTables{};
+    T.ncolp = I.ncolp;
+    size_t ntiles = 0, nslots = 0, ntel = 0;
+    for (const ReuseTables &p : part) { ntiles += p.tiles.size(); nslots += p.elem.size(); ntel += p.telem.size(); }
+    T.tiles.reserve(ntiles); T.trows.reserve(ntiles);
+    T.elem.reserve(nslots); T.info.reserve(nslots); T.pos.reserve(nslots * I.ncolp); T.telem.reserve(ntel);
+    for (const ReuseTables &p : part) {
+        const int off = (int)T.elem.size();
+        for (int4 td : p.tiles) { td.x += off; T.tiles.push_back(td); }
+        T.trows.insert(T.trows.end(), p.trows.begin(), p.trows.end());
+        T.elem.insert(T.elem.end(), p.elem.begin(), p.elem.end());
+        T.info.insert(T.info.end(), p.info.begin(), p.info.end());
+        T.pos.insert(T.pos.end(), p.pos.begin(), p.pos.end());
+        T.telem.insert(T.telem.end(), p.telem.begin(), p.telem.end());
+        T.maxv = std::max(T.maxv, p.maxv);
+        T.maxrank = std::max(T.maxrank, p.maxrank);
+        T.maxu = std::max(T.maxu, p.maxu);
+    }
     return 0;
 }
 
