@@ -271,7 +271,12 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     I.nen = nen; I.nc = nc; I.ncol = ncol;
     I.ncolp = (ncol + 3) & ~3;  // one byte per column: position (bits 0-6), first contribution (bit 7)
     I.cap = cap; I.ucap = ucap;
-    I.vcap = std::min(std::max(env_int("ELFEL_HIP_VCAP", REUSE_VCAP_PER_THREAD), 8), 24) * cap;
+    // LDS doubles for a tile's values per task slot.  Stokes tiles take 20:
+    // the largest tile sets every workgroup's LDS, and at 24 (45 KiB) only 3
+    // Stokes workgroups fit on a CU against 4 at 20 (C3 157 against 137 us);
+    // the elasticity kernel is held at 4 by its registers and keeps 24 (at 16:
+    // 66 against 64 us).
+    I.vcap = std::min(std::max(env_int("ELFEL_HIP_VCAP", stokes ? 20 : REUSE_VCAP_PER_THREAD), 8), 24) * cap;
     I.n = n;
     I.F0 = &F0; I.F1 = F1;
     Adj A0, A1;
