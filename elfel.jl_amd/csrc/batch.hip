@@ -361,9 +361,9 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
             // grid of a multiple of 8 workgroups striding the list): runs of 64
             // neighbouring tiles, which read the same elements, go to one XCD's
             // L2, and each XCD gets runs from every part of the list.  This cuts
-            // the fetched bytes from 3.1 to 1.3 GB per launch at the same time
-            // (1.31 ms; the launch is not fetch-bound); whole chunks per XCD
-            // left the XCDs unevenly loaded (1.35 ms)
+            // the fetched bytes from 3.2 to 2.1 GB per launch at about the same
+            // time (the launch is not fetch-bound); whole chunks per XCD left
+            // the XCDs unevenly loaded (1.35 against 1.31 ms)
             order_tiles(cat, 8, batch_slots(kind, bi ? 256 : 128), env_int("ELFEL_HIP_XCD_RUN_BATCH", 64));
             if ((rc = upload_tables(c, cat, K.tab[bi]))) return rc;
         }
