@@ -193,7 +193,7 @@ extern "C" int elfel_hip_free(elfel_hip_ctx *c)
     if (!c) return ELFEL_EINVAL;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
-    for (auto &f : c->f) { release(f.conn); release(f.xy); release(f.dof); }
+    for (auto &f : c->f) { release(f.conn); release(f.xy); release(f.dof); release(f.edof); }
     Buf *bufs[] = {&c->kappa_e, &c->Q_e, &c->scale_e, &c->geom, &c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1],
                    &c->hist, &c->scan_tmp, &c->flags, &c->scal, &c->fkeys[0], &c->fkeys[1], &c->fvals[0],
                    &c->fvals[1], &c->stage64, &c->ptr, &c->ind, &c->val, &c->F, &c->dscal, &c->pcg, &c->pcg_sell,
@@ -372,31 +372,60 @@ extern "C" int elfel_hip_vector(elfel_hip_ctx *c, int64_t ndofs, int64_t n, cons
 // ---------------------------------------------------------------------------
 // device-resident problem
 // ---------------------------------------------------------------------------
-extern "C" int elfel_hip_set_mesh(elfel_hip_ctx *c, int field, int32_t nen, int64_t E, const int64_t *conn, int64_t Nn,
-                                  const double *xy)
+extern "C" int elfel_hip_set_mesh_sdim(elfel_hip_ctx *c, int field, int32_t nen, int64_t E, const int64_t *conn,
+                                       int64_t Nn, int32_t sdim, const double *x)
 {
     CHECK_CTX(c);
-    if (field < 0 || field > 1 || nen < 1 || nen > ELFEL_MAXBF || E < 0 || Nn < 0 || (E > 0 && !conn) ||
-        (Nn > 0 && !xy))
+    if (field < 0 || field > 1 || nen < 1 || nen > ELFEL_MAXBF || E < 0 || Nn < 0 || sdim < 1 || sdim > 3 ||
+        (E > 0 && !conn) || (Nn > 0 && !x))
         return set_err(c, ELFEL_EINVAL, "elfel_hip_set_mesh: bad arguments");
     if (E * nen >= INT32_MAX || Nn >= INT32_MAX) return set_err(c, ELFEL_ENOTSUP, "mesh beyond int32 indexing");
     Field &f = c->f[field];
     int rc;
-    if ((rc = ensure(c, f.conn, 4 * E * nen)) || (rc = ensure(c, f.xy, 16 * Nn)) || (rc = ensure(c, c->stage64, 8 * E * nen)))
+    if ((rc = ensure(c, f.conn, 4 * E * nen)) || (rc = ensure(c, f.xy, 8 * sdim * Nn)) ||
+        (rc = ensure(c, c->stage64, 8 * E * nen)))
         return rc;
     f.has_mesh = false;
     if (E > 0) HIPCHK(c, hipMemcpyAsync(c->stage64.p, conn, 8 * E * nen, hipMemcpyHostToDevice, c->stream));
-    if (Nn > 0) HIPCHK(c, hipMemcpyAsync(f.xy.p, xy, 16 * Nn, hipMemcpyHostToDevice, c->stream));
+    if (Nn > 0) HIPCHK(c, hipMemcpyAsync(f.xy.p, x, 8 * sdim * Nn, hipMemcpyHostToDevice, c->stream));
     if ((rc = convert_indices(c, (const int64_t *)c->stage64.p, (int32_t *)f.conn.p, E * nen, 1, Nn))) return rc;
     int flags = 0;
     if ((rc = take_flags(c, &flags))) return rc;
     if (flags & DERR_RANGE) return set_err(c, ELFEL_ERANGE, "connectivity refers to a node outside 1..Nn");
-    f.nen = nen; f.E = E; f.Nn = Nn;
+    f.nen = nen; f.E = E; f.Nn = Nn; f.sdim = sdim;
     f.h_conn.resize((size_t)(E * nen));
     for (int64_t i = 0; i < E * nen; i++) f.h_conn[i] = (int32_t)(conn[i] - 1);
     f.has_mesh = true;
     f.mesh_version++;
     c->pat.valid = false;
+    return ELFEL_OK;
+}
+
+extern "C" int elfel_hip_set_mesh(elfel_hip_ctx *c, int field, int32_t nen, int64_t E, const int64_t *conn, int64_t Nn,
+                                  const double *xy)
+{
+    return elfel_hip_set_mesh_sdim(c, field, nen, E, conn, Nn, 2, xy);
+}
+
+extern "C" int elfel_hip_set_element_dofs(elfel_hip_ctx *c, int field, int32_t ncomp, int64_t E, const int64_t *dofnums,
+                                          int64_t ndofs)
+{
+    CHECK_CTX(c);
+    if (field < 0 || field > 1 || ncomp < 1 || ncomp > 2 || E < 0 || ndofs < 0 || (E > 0 && !dofnums))
+        return set_err(c, ELFEL_EINVAL, "elfel_hip_set_element_dofs: bad arguments");
+    if (ndofs >= INT32_MAX / 2 || E * ncomp >= INT32_MAX) return set_err(c, ELFEL_ENOTSUP, "beyond int32 indexing");
+    Field &f = c->f[field];
+    const int64_t n = E * ncomp;
+    int rc;
+    if ((rc = ensure(c, f.edof, 4 * n)) || (rc = ensure(c, c->stage64, 8 * n))) return rc;
+    f.has_edofs = false;
+    if (n > 0) HIPCHK(c, hipMemcpyAsync(c->stage64.p, dofnums, 8 * n, hipMemcpyHostToDevice, c->stream));
+    if ((rc = convert_indices(c, (const int64_t *)c->stage64.p, (int32_t *)f.edof.p, n, 1, ndofs))) return rc;
+    int flags = 0;
+    if ((rc = take_flags(c, &flags))) return rc;
+    if (flags & DERR_RANGE) return set_err(c, ELFEL_ERANGE, "cell dof number outside 1..ndofs (unnumbered dof 0?)");
+    f.edof_ncomp = ncomp; f.edof_E = E; f.edof_ndofs = ndofs;
+    f.has_edofs = true;
     return ELFEL_OK;
 }
 
@@ -459,6 +488,7 @@ static int validate_problem(elfel_hip_ctx *c, const elfel_hip_problem *pb, QTabl
     if (!pb) return set_err(c, ELFEL_EINVAL, "null problem");
     const Field &F = c->f[0];
     if (!F.has_mesh || !F.has_dofs) return set_err(c, ELFEL_ESTATE, "set_mesh/set_dofs of field 0 first");
+    if (F.sdim != 2) return set_err(c, ELFEL_EINVAL, "this weak form needs a mesh in two space dimensions");
     if (pb->kind < ELFEL_HEAT || pb->kind > ELFEL_STOKES_VECLAP) return set_err(c, ELFEL_EINVAL, "bad kind");
     const int want_nen = pb->fe == ELFEL_T3 ? 3 : pb->fe == ELFEL_T6 ? 6 : pb->fe == ELFEL_Q4 ? 4 : -1;
     if (want_nen < 0 || want_nen != F.nen) return set_err(c, ELFEL_EINVAL, "element type does not match the mesh");
@@ -468,7 +498,7 @@ static int validate_problem(elfel_hip_ctx *c, const elfel_hip_problem *pb, QTabl
     if (stokes) {
         const Field &P = c->f[1];
         if (pb->fe != ELFEL_T6) return set_err(c, ELFEL_ENOTSUP, "Taylor-Hood Stokes needs T6 velocity");
-        if (!P.has_mesh || !P.has_dofs || P.nen != 3 || P.E != F.E || P.ncomp != 1)
+        if (!P.has_mesh || !P.has_dofs || P.nen != 3 || P.E != F.E || P.ncomp != 1 || P.sdim != 2)
             return set_err(c, ELFEL_ESTATE, "Stokes needs the T3 pressure field (field 1) with the same elements");
     }
     if (pb->precision != ELFEL_FP64 && pb->precision != ELFEL_MIXED) return set_err(c, ELFEL_EINVAL, "bad precision");
@@ -496,12 +526,16 @@ extern "C" int elfel_hip_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb,
     CHECK_CTX(c);
     QTable tab;
     Coef cf;
-    int rc = validate_problem(c, pb, &tab, &cf);
+    const bool fam = pb && family_problem(pb);
+    int rc = fam ? family_validate(c, pb) : validate_problem(c, pb, &tab, &cf);
     if (rc) return rc;
     c->has_result = false;
     hipEvent_t ev = nullptr;
     tm_begin(c, ELFEL_PH_TOTAL, &ev);
-    if (pb->method == ELFEL_SORT) {
+    if (fam) {
+        c->nnz = -1;
+        rc = family_assemble(c, pb);
+    } else if (pb->method == ELFEL_SORT) {
         c->nnz = -1;
         rc = plain_assemble(c, pb, tab, cf);
     } else {

@@ -57,9 +57,15 @@ struct Buf {
 
 struct Field {
     int nen = 0, ncomp = 0;
+    int sdim = 2;                // space dimension of the coordinates (xy holds sdim doubles per node)
     int64_t E = 0, Nn = 0, ndofs = 0;
     bool has_mesh = false, has_dofs = false;
     Buf conn, xy, dof;
+    // cell dofs [E][edof_ncomp] (bubble, FEL2_Q4), 0-based
+    int edof_ncomp = 0;
+    int64_t edof_E = 0, edof_ndofs = 0;
+    bool has_edofs = false;
+    Buf edof;
     std::vector<int32_t> h_conn, h_dof;  // host copies (0-based) for the symbolic phase
     uint64_t mesh_version = 0, dof_version = 0;
 };
@@ -166,6 +172,16 @@ int build_dense_from_sorted(elfel_hip_ctx *c, const uint64_t *keys, const double
 
 // ---- plain.hip (FP64, unfused arithmetic) -------------------------------------
 int plain_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf);
+// COO buffers for n matrix entries and nv vector entries (plain method), and
+// the sort + reduce that turns the keys written into keys[0]/vals[0] (and
+// fkeys[0]/fvals[0]) into the CSR/CSC result and the load vector.
+int plain_buffers(elfel_hip_ctx *c, int64_t n, int64_t nv, int64_t ndofs);
+int plain_finish(elfel_hip_ctx *c, int layout, int64_t n, int64_t nv, int64_t ndofs, int minor_bits);
+
+// ---- families.hip (element families, plain FP64 method) -------------------------
+bool family_problem(const elfel_hip_problem *pb);
+int family_validate(elfel_hip_ctx *c, const elfel_hip_problem *pb);  // sets c->nrow, c->ncol
+int family_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb);
 int convert_indices(elfel_hip_ctx *c, const int64_t *src_dev, int32_t *dst, int64_t n, int64_t lo,
                     int64_t hi);
 

@@ -1,0 +1,581 @@
+// families.hip -- the element families beyond T3/T6/Q4 and the weak forms
+// that use them, on the plain FP64 method (COO in Elfel's stream order, then
+// the stable radix sort and in-order segmented reduce of plain.hip).
+// Compiled with -ffp-contract=off: every product and sum rounds as the
+// reference's unfused Julia arithmetic does.
+//
+//   FEH1_L2 (src/FElements.jl:203-225)         heat, mass        1D, Gauss 1-3
+//   FEH1_T4 (:356-384)                         heat, mass        3D, 1 or 4 points
+//   FEH1_T3_BUBBLE (:322-354)                  Reddy Stokes velocity (P1b/P1)
+//   FEL2_Q4 (:386-413), Q4 geometry carrier    Reddy Stokes pressure (Q1/Q0)
+//   FEH1_T3 / T6 / Q4                          mass; T6/T3 Reddy Stokes
+//
+// One work-item per element.  Geometry (src/FElements.jl:113-168): J = sum_k
+// x_k (outer) dN_k/dxi over the geometry nodes (first term assigned, then
+// added); det J = |J11| in 1D, J11 J22 - J21 J12 in 2D, the unrolled 3x3
+// expansion in 3D.  Spatial gradients (src/QPIterators.jl:132-140, g / J):
+// g / J11 in 1D, else the row vector g times the adjugate of J over det J,
+// each component summed in index order.
+#include <algorithm>
+
+#include "common.h"
+
+namespace {
+
+constexpr int FQ = 4;   // quadrature points
+constexpr int FB = 6;   // basis functions of the scalar element
+
+// Tables of one (element, rule) pair, reaching the kernels by value (every
+// index is wave-uniform: scalar loads from the argument segment).
+struct FamTab {
+    int nq;
+    double w[FQ];
+    double Nu[FQ][FB];       // primary (velocity / scalar) basis
+    double Gu[FQ][FB][3];    // its parametric gradients
+    double Np[FQ][3];        // pressure basis (Reddy)
+    double Gg[FQ][FB][3];    // parametric gradients of the geometry element
+};
+
+__host__ __device__ constexpr int fam_nbf(int fe)
+{
+    return fe == ELFEL_T3 ? 3 : fe == ELFEL_T6 ? 6 : fe == ELFEL_Q4 ? 4 : fe == ELFEL_T3B ? 4
+         : fe == ELFEL_Q4L2 ? 1 : fe == ELFEL_T4 ? 4 : fe == ELFEL_L2 ? 2 : -1;
+}
+__host__ __device__ constexpr int fam_ngeo(int fe)
+{
+    return fe == ELFEL_T3 || fe == ELFEL_T3B ? 3 : fe == ELFEL_T6 ? 6 : fe == ELFEL_Q4 || fe == ELFEL_Q4L2 ? 4
+         : fe == ELFEL_T4 ? 4 : fe == ELFEL_L2 ? 2 : -1;
+}
+__host__ __device__ constexpr int fam_mdim(int fe) { return fe == ELFEL_L2 ? 1 : fe == ELFEL_T4 ? 3 : 2; }
+
+// Rules: interval Gauss (src/RefShapes.jl:85-110, literal constants),
+// triangle 1/3 points (:113-119), square Gauss tensor i outer / j inner
+// (:333-366), tetrahedron 1/4 points (:232-245).
+int fam_rule(int fe, int rule, double (*pc)[3], double *w)
+{
+    for (int q = 0; q < FQ; q++) pc[q][0] = pc[q][1] = pc[q][2] = 0.0;
+    auto gauss = [](int order, double *p, double *wt) -> int {
+        if (order == 1) { p[0] = 0.0; wt[0] = 2.0; return 1; }
+        if (order == 2) { p[0] = -0.577350269189626; p[1] = 0.577350269189626; wt[0] = 1.0; wt[1] = 1.0; return 2; }
+        if (order == 3) {
+            p[0] = -0.774596669241483; p[1] = 0.0; p[2] = 0.774596669241483;
+            wt[0] = 0.5555555555555556; wt[1] = 0.8888888888888889; wt[2] = 0.5555555555555556;
+            return 3;
+        }
+        return -1;
+    };
+    if (fe == ELFEL_L2) {
+        double p[3], wt[3];
+        const int n = gauss(rule, p, wt);
+        for (int q = 0; q < n; q++) { pc[q][0] = p[q]; w[q] = wt[q]; }
+        return n;
+    }
+    if (fe == ELFEL_T4) {
+        if (rule == 1) { pc[0][0] = pc[0][1] = pc[0][2] = 0.25; w[0] = 1.0 / 6.0; return 1; }
+        if (rule == 4) {
+            const double a = 0.13819660, b = 0.58541020;
+            const double P[4][3] = {{a, a, a}, {b, a, a}, {a, b, a}, {a, a, b}};
+            for (int q = 0; q < 4; q++) {
+                for (int d = 0; d < 3; d++) pc[q][d] = P[q][d];
+                w[q] = 0.041666666666666666667;
+            }
+            return 4;
+        }
+        return -1;
+    }
+    if (fe == ELFEL_Q4 || fe == ELFEL_Q4L2) {
+        double p[3], wt[3];
+        const int n1 = rule <= 2 ? gauss(rule, p, wt) : -1;
+        if (n1 < 0) return -1;
+        int n = 0;
+        for (int i = 0; i < n1; i++)
+            for (int j = 0; j < n1; j++) { pc[n][0] = p[i]; pc[n][1] = p[j]; w[n] = wt[i] * wt[j]; n++; }
+        return n;
+    }
+    if (rule == 1) { pc[0][0] = 1.0 / 3.; pc[0][1] = 1.0 / 3.; w[0] = 1.0 / 2.0; return 1; }
+    if (rule == 3) {
+        pc[0][0] = 2.0 / 3; pc[0][1] = 1.0 / 6;
+        pc[1][0] = 1.0 / 6; pc[1][1] = 2.0 / 3;
+        pc[2][0] = 1.0 / 6; pc[2][1] = 1.0 / 6;
+        for (int q = 0; q < 3; q++) w[q] = (1.0 / 3) / 2;
+        return 3;
+    }
+    return -1;
+}
+
+// Basis values and parametric gradients g[k][d] (src/FElements.jl; the
+// nodal T3 / T6 / Q4 are qtable.h's).
+void fam_basis(int fe, const double *pc, double *N, double (*g)[3])
+{
+    const double r = pc[0], s = pc[1], t = pc[2];
+    for (int k = 0; k < FB; k++) g[k][0] = g[k][1] = g[k][2] = 0.0;
+    switch (fe) {
+    case ELFEL_T3B:
+        N[0] = (1 - r - s); N[1] = r; N[2] = s;
+        N[3] = (1 - r - s) * r * s;
+        g[0][0] = -1.; g[0][1] = -1.; g[1][0] = +1.; g[1][1] = 0.; g[2][0] = 0.; g[2][1] = +1.;
+        g[3][0] = (-r * s + (1 - r - s) * s);
+        g[3][1] = (-r * s + (1 - r - s) * r);
+        return;
+    case ELFEL_Q4L2:
+        N[0] = 1.0;
+        return;
+    case ELFEL_T4:
+        N[0] = (1 - r - s - t); N[1] = r; N[2] = s; N[3] = t;
+        g[0][0] = -1.0; g[0][1] = -1.0; g[0][2] = -1.0;
+        g[1][0] = +1.0; g[2][1] = +1.0; g[3][2] = +1.0;
+        return;
+    case ELFEL_L2:
+        N[0] = (1. - r) / 2.0; N[1] = (1. + r) / 2.0;
+        g[0][0] = -1.0 / 2.0; g[1][0] = +1.0 / 2.0;
+        return;
+    default: {
+        double g2[ELFEL_MAXBF][2];
+        qt_bfun(fe, r, s, N);
+        qt_bgrad(fe, r, s, g2);
+        for (int k = 0; k < fam_nbf(fe); k++) { g[k][0] = g2[k][0]; g[k][1] = g2[k][1]; }
+    }
+    }
+}
+
+// ufe: the primary element; pfe: the pressure element (Reddy) or -1; gfe:
+// the element whose nodes carry the geometry.
+int fam_table(int ufe, int pfe, int gfe, int rule, FamTab *t)
+{
+    double pc[FQ][3], w[FQ];
+    const int nq = fam_rule(ufe, rule, pc, w);
+    if (nq < 1) return -1;
+    *t = FamTab{};
+    t->nq = nq;
+    for (int q = 0; q < nq; q++) {
+        double N[FB] = {}, g[FB][3];
+        t->w[q] = w[q];
+        fam_basis(ufe, pc[q], t->Nu[q], t->Gu[q]);
+        if (pfe >= 0) {
+            fam_basis(pfe, pc[q], N, g);
+            for (int k = 0; k < 3; k++) t->Np[q][k] = N[k];
+        }
+        fam_basis(gfe, pc[q], N, t->Gg[q]);
+    }
+    return 0;
+}
+
+// ---- geometry on the device ---------------------------------------------------
+template <int M, int NG>
+struct Jac {
+    double a[M][M];
+    double det;
+
+    __device__ __forceinline__ void make(const double *X, const double (*G)[3])
+    {
+#pragma unroll
+        for (int s = 0; s < M; s++)
+#pragma unroll
+            for (int d = 0; d < M; d++) a[s][d] = X[s] * G[0][d];
+#pragma unroll
+        for (int k = 1; k < NG; k++)
+#pragma unroll
+            for (int s = 0; s < M; s++)
+#pragma unroll
+                for (int d = 0; d < M; d++) a[s][d] += X[k * M + s] * G[k][d];
+        if constexpr (M == 1) det = fabs(a[0][0]);
+        else if constexpr (M == 2) det = a[0][0] * a[1][1] - a[1][0] * a[0][1];
+        else
+            det = (+a[0][0] * (a[1][1] * a[2][2] - a[2][1] * a[1][2])
+                   - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0])
+                   + a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]));
+    }
+
+    // gx[s] = (g / J)_s for one basis function with parametric gradient g
+    __device__ __forceinline__ void grad(const double *g, double *gx) const
+    {
+        if constexpr (M == 1) {
+            gx[0] = g[0] / a[0][0];
+        } else if constexpr (M == 2) {
+            gx[0] = (a[1][1] * g[0] - a[1][0] * g[1]) / det;
+            gx[1] = (a[0][0] * g[1] - a[0][1] * g[0]) / det;
+        } else {
+            double C[3][3];
+            C[0][0] = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+            C[0][1] = a[1][2] * a[2][0] - a[1][0] * a[2][2];
+            C[0][2] = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+            C[1][0] = a[0][2] * a[2][1] - a[0][1] * a[2][2];
+            C[1][1] = a[0][0] * a[2][2] - a[0][2] * a[2][0];
+            C[1][2] = a[0][1] * a[2][0] - a[0][0] * a[2][1];
+            C[2][0] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
+            C[2][1] = a[0][2] * a[1][0] - a[0][0] * a[1][2];
+            C[2][2] = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+#pragma unroll
+            for (int s = 0; s < 3; s++) gx[s] = (g[0] * C[s][0] + g[1] * C[s][1] + g[2] * C[s][2]) / det;
+        }
+    }
+};
+
+__device__ __forceinline__ uint64_t coo_key(int32_t r, int32_t col, int minor_bits, int csc)
+{
+    const uint64_t mj = (uint64_t)(csc ? col : r), mn = (uint64_t)(csc ? r : col);
+    return (mj << minor_bits) | mn;
+}
+
+// ---- heat / mass on a nodal family --------------------------------------------
+// heat: ke[i,j] += dot(gradN_i, gradN_j) * (kappa JxW), fe[j] += N_j Q JxW
+//       (examples/heat/poisson/t4.jl:33-55, the loop of test/test_heat.jl:36-60);
+// mass: me[i,j] += (N_i N_j) * (kappa JxW) (examples/klein_gordon/klein_gordon.jl:34-51).
+template <int KIND, int M, int NB, int NQ>
+__global__ __launch_bounds__(256) void k_fam_scalar(int64_t E, const int32_t *__restrict__ conn,
+                                                    const double *__restrict__ x, const int32_t *__restrict__ dof,
+                                                    FamTab tab, double kappa, double Q, int minor_bits, int csc,
+                                                    uint64_t *__restrict__ keys, double *__restrict__ vals,
+                                                    uint64_t *__restrict__ fkeys, double *__restrict__ fvals)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double X[NB * M];
+    int32_t d[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) {
+        const int32_t n = conn[e * NB + k];
+#pragma unroll
+        for (int s = 0; s < M; s++) X[k * M + s] = x[(int64_t)n * M + s];
+        d[k] = dof[n];
+    }
+    double ke[NB * NB], fe[NB];
+#pragma unroll
+    for (int i = 0; i < NB * NB; i++) ke[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < NB; i++) fe[i] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        Jac<M, NB> J;
+        J.make(X, tab.Gg[q]);
+        const double JxW = J.det * tab.w[q];
+        double g[NB][M];
+        if constexpr (KIND == ELFEL_HEAT) {
+#pragma unroll
+            for (int k = 0; k < NB; k++) J.grad(tab.Gu[q][k], g[k]);
+        }
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+#pragma unroll
+            for (int i = 0; i < NB; i++) {
+                if constexpr (KIND == ELFEL_HEAT) {
+                    double s = g[i][0] * g[j][0];
+#pragma unroll
+                    for (int c = 1; c < M; c++) s += g[i][c] * g[j][c];
+                    ke[i + NB * j] += s * (kappa * JxW);
+                } else {
+                    ke[i + NB * j] += (tab.Nu[q][i] * tab.Nu[q][j]) * (kappa * JxW);
+                }
+            }
+            if constexpr (KIND == ELFEL_HEAT) fe[j] += tab.Nu[q][j] * Q * JxW;
+        }
+    }
+    int64_t o = e * NB * NB;
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            keys[o] = coo_key(d[i], d[j], minor_bits, csc);
+            vals[o] = ke[i + NB * j];
+            o++;
+        }
+    if constexpr (KIND == ELFEL_HEAT) {
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            fkeys[e * NB + j] = (uint64_t)d[j];
+            fvals[e * NB + j] = fe[j];
+        }
+    }
+}
+
+// ---- Reddy-form Stokes over three scalar spaces ---------------------------------
+// test/test_stokes.jl:373-431 (T6, T3), examples/stokes/colliding_flow/
+// p1b_p1.jl:50-108 (T3 + bubble, T3), q1_q0.jl:52-110 (Q4, FEL2_Q4):
+//   kxx += (mu JxW)(2 gx_i gx_j + gy_i gy_j), kyy += (mu JxW)(gx_i gx_j + 2 gy_i gy_j),
+//   kxy += (mu JxW)(gx_i gy_j), kxp += (-JxW)(gx_i Np_j), kyp += (-JxW)(gy_i Np_j).
+// The Jacobian comes from the T3 pressure element (jacjac(pel, pqp)) or, for
+// the cell-constant pressure, from the velocity Q4 (jacjac(uxel, uxqp)).
+// Appended kxx, kxy, kxy', kyy, kxp, kxp', kyp, kyp': column-major, a
+// transpose row-major of its parent.  The spatial gradients and JxW of every
+// point are formed once; each block is then summed over the points in order
+// and written before the next, so that one block at a time is live.
+template <int VFE, int PFE, int NQ>
+__global__ __launch_bounds__(256) void k_fam_reddy(int64_t E, const int32_t *__restrict__ conn_v,
+                                                   const double2 *__restrict__ xy_v,
+                                                   const int32_t *__restrict__ dof_v,
+                                                   const int32_t *__restrict__ edof_v,
+                                                   const int32_t *__restrict__ conn_p,
+                                                   const double2 *__restrict__ xy_p,
+                                                   const int32_t *__restrict__ dof_p,
+                                                   const int32_t *__restrict__ edof_p, FamTab tab, double mu,
+                                                   int minor_bits, int csc, uint64_t *__restrict__ keys,
+                                                   double *__restrict__ vals)
+{
+    constexpr int NU = fam_nbf(VFE), NV = fam_ngeo(VFE), NP = fam_nbf(PFE);
+    constexpr bool PNODAL = PFE == ELFEL_T3;
+    constexpr int NG = PNODAL ? 3 : NV;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    double X[2 * NG];
+#pragma unroll
+    for (int k = 0; k < NG; k++) {
+        const double2 p = PNODAL ? xy_p[conn_p[e * 3 + k]] : xy_v[conn_v[e * NV + k]];
+        X[2 * k] = p.x; X[2 * k + 1] = p.y;
+    }
+    double gx[NQ][NU], gy[NQ][NU], JxW[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        Jac<2, NG> J;
+        J.make(X, tab.Gg[q]);
+        JxW[q] = J.det * tab.w[q];
+#pragma unroll
+        for (int k = 0; k < NU; k++) {
+            double g[2];
+            J.grad(tab.Gu[q][k], g);
+            gx[q][k] = g[0]; gy[q][k] = g[1];
+        }
+    }
+    int32_t dx[NU], dy[NU], dp[NP];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const int32_t n = conn_v[e * NV + k];
+        dx[k] = dof_v[2 * (int64_t)n];
+        dy[k] = dof_v[2 * (int64_t)n + 1];
+    }
+    if constexpr (NU > NV) {  // the bubble's cell dofs
+        dx[NV] = edof_v[2 * e];
+        dy[NV] = edof_v[2 * e + 1];
+    }
+    if constexpr (PNODAL) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) dp[k] = dof_p[conn_p[e * 3 + k]];
+    } else {
+        dp[0] = edof_p[e];
+    }
+    int64_t o = e * (4 * NU * NU + 4 * NU * NP);
+    auto put = [&](int32_t r, int32_t col, double v) {
+        keys[o] = coo_key(r, col, minor_bits, csc);
+        vals[o] = v;
+        o++;
+    };
+    double k[NU * NU];
+    // kxx
+#pragma unroll
+    for (int i = 0; i < NU * NU; i++) k[i] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+#pragma unroll
+        for (int j = 0; j < NU; j++)
+#pragma unroll
+            for (int i = 0; i < NU; i++)
+                k[i + NU * j] += (mu * JxW[q]) * (2 * gx[q][i] * gx[q][j] + gy[q][i] * gy[q][j]);
+#pragma unroll
+    for (int j = 0; j < NU; j++)
+#pragma unroll
+        for (int i = 0; i < NU; i++) put(dx[i], dx[j], k[i + NU * j]);
+    // kxy and its transpose
+#pragma unroll
+    for (int i = 0; i < NU * NU; i++) k[i] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+#pragma unroll
+        for (int j = 0; j < NU; j++)
+#pragma unroll
+            for (int i = 0; i < NU; i++) k[i + NU * j] += (mu * JxW[q]) * (gx[q][i] * gy[q][j]);
+#pragma unroll
+    for (int j = 0; j < NU; j++)
+#pragma unroll
+        for (int i = 0; i < NU; i++) put(dx[i], dy[j], k[i + NU * j]);
+#pragma unroll
+    for (int b = 0; b < NU; b++)
+#pragma unroll
+        for (int a = 0; a < NU; a++) put(dy[a], dx[b], k[b + NU * a]);
+    // kyy
+#pragma unroll
+    for (int i = 0; i < NU * NU; i++) k[i] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+#pragma unroll
+        for (int j = 0; j < NU; j++)
+#pragma unroll
+            for (int i = 0; i < NU; i++)
+                k[i + NU * j] += (mu * JxW[q]) * (gx[q][i] * gx[q][j] + 2 * gy[q][i] * gy[q][j]);
+#pragma unroll
+    for (int j = 0; j < NU; j++)
+#pragma unroll
+        for (int i = 0; i < NU; i++) put(dy[i], dy[j], k[i + NU * j]);
+    // kxp, kyp and their transposes
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+#pragma unroll
+        for (int i = 0; i < NU * NP; i++) k[i] = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+#pragma unroll
+            for (int j = 0; j < NP; j++)
+#pragma unroll
+                for (int i = 0; i < NU; i++)
+                    k[i + NU * j] += (-JxW[q]) * ((c == 0 ? gx[q][i] : gy[q][i]) * tab.Np[q][j]);
+#pragma unroll
+        for (int j = 0; j < NP; j++)
+#pragma unroll
+            for (int i = 0; i < NU; i++) put(c == 0 ? dx[i] : dy[i], dp[j], k[i + NU * j]);
+#pragma unroll
+        for (int b = 0; b < NU; b++)
+#pragma unroll
+            for (int a = 0; a < NP; a++) put(dp[a], c == 0 ? dx[b] : dy[b], k[b + NU * a]);
+    }
+}
+
+bool reddy_pair(int vfe, int pfe)
+{
+    return (vfe == ELFEL_T6 && pfe == ELFEL_T3) || (vfe == ELFEL_T3B && pfe == ELFEL_T3) ||
+           (vfe == ELFEL_Q4 && pfe == ELFEL_Q4L2);
+}
+
+int64_t field_ndofs(const Field &f)
+{
+    return std::max(f.has_dofs ? f.ndofs : 0, f.has_edofs ? f.edof_ndofs : 0);
+}
+
+template <int KIND, int M, int NB, int NQ>
+int launch_scalar(elfel_hip_ctx *c, const FamTab &tab, double kappa, double Q, int minor_bits, int csc)
+{
+    const Field &F = c->f[0];
+    const bool heat = KIND == ELFEL_HEAT;
+    hipEvent_t ev = nullptr;
+    tm_begin(c, ELFEL_PH_ELEMENT, &ev);
+    if (F.E > 0)
+        hipLaunchKernelGGL((k_fam_scalar<KIND, M, NB, NQ>), dim3(ceil_div(F.E, 256)), dim3(256), 0, c->stream, F.E,
+                           (const int32_t *)F.conn.p, (const double *)F.xy.p, (const int32_t *)F.dof.p, tab, kappa, Q,
+                           minor_bits, csc, (uint64_t *)c->keys[0].p, (double *)c->vals[0].p,
+                           heat ? (uint64_t *)c->fkeys[0].p : nullptr, heat ? (double *)c->fvals[0].p : nullptr);
+    HIPCHK(c, hipGetLastError());
+    tm_end(c, ELFEL_PH_ELEMENT, ev);
+    return 0;
+}
+
+template <int VFE, int PFE, int NQ>
+int launch_reddy(elfel_hip_ctx *c, const FamTab &tab, double mu, int minor_bits, int csc)
+{
+    const Field &V = c->f[0], &P = c->f[1];
+    hipEvent_t ev = nullptr;
+    tm_begin(c, ELFEL_PH_ELEMENT, &ev);
+    if (V.E > 0)
+        hipLaunchKernelGGL((k_fam_reddy<VFE, PFE, NQ>), dim3(ceil_div(V.E, 256)), dim3(256), 0, c->stream, V.E,
+                           (const int32_t *)V.conn.p, (const double2 *)V.xy.p, (const int32_t *)V.dof.p,
+                           VFE == ELFEL_T3B ? (const int32_t *)V.edof.p : nullptr,
+                           PFE == ELFEL_T3 ? (const int32_t *)P.conn.p : nullptr,
+                           PFE == ELFEL_T3 ? (const double2 *)P.xy.p : nullptr,
+                           PFE == ELFEL_T3 ? (const int32_t *)P.dof.p : nullptr,
+                           PFE == ELFEL_Q4L2 ? (const int32_t *)P.edof.p : nullptr, tab, mu, minor_bits, csc,
+                           (uint64_t *)c->keys[0].p, (double *)c->vals[0].p);
+    HIPCHK(c, hipGetLastError());
+    tm_end(c, ELFEL_PH_ELEMENT, ev);
+    return 0;
+}
+
+}  // namespace
+
+bool family_problem(const elfel_hip_problem *pb)
+{
+    return pb->kind == ELFEL_STOKES_REDDY || pb->kind == ELFEL_MASS || pb->fe == ELFEL_T3B || pb->fe == ELFEL_Q4L2 ||
+           pb->fe == ELFEL_T4 || pb->fe == ELFEL_L2;
+}
+
+int family_validate(elfel_hip_ctx *c, const elfel_hip_problem *pb)
+{
+    if (pb->method != ELFEL_SORT || pb->precision != ELFEL_FP64)
+        return set_err(c, ELFEL_ENOTSUP, "element families and the Reddy / mass forms run on the plain FP64 method");
+    if (pb->layout != ELFEL_CSR && pb->layout != ELFEL_CSC) return set_err(c, ELFEL_EINVAL, "bad layout");
+    if (c->has_kappa_e || c->has_Q_e || c->has_scale_e)
+        return set_err(c, ELFEL_ENOTSUP, "element families take uniform coefficients (clear the per-element ones)");
+    const Field &F = c->f[0];
+    const int fe = pb->fe;
+    if (fam_nbf(fe) < 0) return set_err(c, ELFEL_EINVAL, "bad element");
+    if (!F.has_mesh) return set_err(c, ELFEL_ESTATE, "set the mesh of field 0 first");
+    if (F.nen != fam_ngeo(fe) || F.sdim != fam_mdim(fe))
+        return set_err(c, ELFEL_EINVAL, "element type does not match the mesh (nodes per element, space dimension)");
+    int64_t ndofs = 0;
+    if (pb->kind == ELFEL_HEAT || pb->kind == ELFEL_MASS) {
+        if (fe == ELFEL_T3B || fe == ELFEL_Q4L2)
+            return set_err(c, ELFEL_ENOTSUP, "heat and mass on the nodal families (T3, T6, Q4, T4, L2)");
+        if (!F.has_dofs || F.ncomp != 1) return set_err(c, ELFEL_ESTATE, "a scalar dof map of field 0 first");
+        ndofs = F.ndofs;
+    } else if (pb->kind == ELFEL_STOKES_REDDY) {
+        const Field &P = c->f[1];
+        if (!reddy_pair(fe, pb->fe_p))
+            return set_err(c, ELFEL_ENOTSUP, "Reddy Stokes pairs: (T6, T3), (T3B, T3), (Q4, Q4L2)");
+        if (!F.has_dofs || F.ncomp != 2)
+            return set_err(c, ELFEL_ESTATE, "field 0 needs node dofs with 2 components (the ux and uy spaces)");
+        if (fe == ELFEL_T3B && (!F.has_edofs || F.edof_ncomp != 2 || F.edof_E != F.E))
+            return set_err(c, ELFEL_ESTATE, "the bubble needs cell dofs of field 0 with 2 components");
+        if (pb->fe_p == ELFEL_T3) {
+            if (!P.has_mesh || P.nen != 3 || P.sdim != 2 || P.E != F.E || !P.has_dofs || P.ncomp != 1)
+                return set_err(c, ELFEL_ESTATE, "the T3 pressure needs field 1's T3 mesh (same elements) and node dofs");
+        } else if (!P.has_edofs || P.edof_ncomp != 1 || P.edof_E != F.E) {
+            return set_err(c, ELFEL_ESTATE, "the FEL2_Q4 pressure needs cell dofs of field 1");
+        }
+        ndofs = std::max(field_ndofs(F), field_ndofs(P));
+    } else {
+        return set_err(c, ELFEL_ENOTSUP, "weak form not available for this element");
+    }
+    FamTab tab;
+    const int pfe = pb->kind == ELFEL_STOKES_REDDY ? pb->fe_p : -1;
+    const int gfe = pfe == ELFEL_T3 ? ELFEL_T3 : fe;
+    if (fam_table(fe, pfe, gfe, pb->rule, &tab)) return set_err(c, ELFEL_ENOTSUP, "unsupported quadrature rule");
+    c->nrow = c->ncol = ndofs;
+    return 0;
+}
+
+int family_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb)
+{
+    const Field &F = c->f[0];
+    const int fe = pb->fe;
+    const bool reddy = pb->kind == ELFEL_STOKES_REDDY;
+    const int pfe = reddy ? pb->fe_p : -1;
+    FamTab tab;
+    if (fam_table(fe, pfe, pfe == ELFEL_T3 ? ELFEL_T3 : fe, pb->rule, &tab))
+        return set_err(c, ELFEL_ENOTSUP, "unsupported quadrature rule");
+    const int nu = fam_nbf(fe), np = reddy ? fam_nbf(pfe) : 0;
+    const int64_t n = F.E * (reddy ? 4 * nu * nu + 4 * nu * np : nu * nu);
+    const bool heat = pb->kind == ELFEL_HEAT;
+    const int64_t nv = heat ? F.E * nu : 0;
+    const int64_t ndofs = c->nrow;
+    const int minor_bits = bits_for(ndofs);
+    const int csc = pb->layout == ELFEL_CSC ? 1 : 0;
+    int rc = plain_buffers(c, n, nv, ndofs);
+    if (rc) return rc;
+    const int nq = tab.nq;
+    rc = 1;  // stays 1 when no kernel instance fits
+#define S(K_, M_, NB_, NQ_) rc = launch_scalar<K_, M_, NB_, NQ_>(c, tab, pb->kappa, pb->Q, minor_bits, csc)
+#define R(V_, P_, NQ_) rc = launch_reddy<V_, P_, NQ_>(c, tab, pb->mu, minor_bits, csc)
+    if (heat || pb->kind == ELFEL_MASS) {
+        const bool h = heat;
+        if (fe == ELFEL_L2) {
+            if (nq == 1) { if (h) S(ELFEL_HEAT, 1, 2, 1); else S(ELFEL_MASS, 1, 2, 1); }
+            else if (nq == 2) { if (h) S(ELFEL_HEAT, 1, 2, 2); else S(ELFEL_MASS, 1, 2, 2); }
+            else if (nq == 3) { if (h) S(ELFEL_HEAT, 1, 2, 3); else S(ELFEL_MASS, 1, 2, 3); }
+        } else if (fe == ELFEL_T4) {
+            if (nq == 1) { if (h) S(ELFEL_HEAT, 3, 4, 1); else S(ELFEL_MASS, 3, 4, 1); }
+            else if (nq == 4) { if (h) S(ELFEL_HEAT, 3, 4, 4); else S(ELFEL_MASS, 3, 4, 4); }
+        } else if (!h && fe == ELFEL_T3) {
+            if (nq == 1) S(ELFEL_MASS, 2, 3, 1); else if (nq == 3) S(ELFEL_MASS, 2, 3, 3);
+        } else if (!h && fe == ELFEL_T6) {
+            if (nq == 1) S(ELFEL_MASS, 2, 6, 1); else if (nq == 3) S(ELFEL_MASS, 2, 6, 3);
+        } else if (!h && fe == ELFEL_Q4) {
+            if (nq == 1) S(ELFEL_MASS, 2, 4, 1); else if (nq == 4) S(ELFEL_MASS, 2, 4, 4);
+        }
+    } else if (reddy) {
+        if (fe == ELFEL_T6 && nq == 3) R(ELFEL_T6, ELFEL_T3, 3);
+        else if (fe == ELFEL_T3B && nq == 3) R(ELFEL_T3B, ELFEL_T3, 3);
+        else if (fe == ELFEL_T3B && nq == 1) R(ELFEL_T3B, ELFEL_T3, 1);
+        else if (fe == ELFEL_Q4 && nq == 4) R(ELFEL_Q4, ELFEL_Q4L2, 4);
+        else if (fe == ELFEL_Q4 && nq == 1) R(ELFEL_Q4, ELFEL_Q4L2, 1);
+    }
+#undef S
+#undef R
+    if (rc == 1) return set_err(c, ELFEL_ENOTSUP, "unsupported (weak form, element, rule)");
+    if (rc) return rc;
+    return plain_finish(c, pb->layout, n, nv, ndofs, minor_bits);
+}

@@ -155,6 +155,7 @@ __global__ void k_mark_boundary(int64_t n, int nen, int ncorner, const int32_t *
 int finish_field_mesh(elfel_hip_ctx *c, Field &f, int nen, int64_t E, int64_t Nn)
 {
     f.nen = nen;
+    f.sdim = 2;
     f.E = E;
     f.Nn = Nn;
     f.h_conn.resize((size_t)(E * nen));
@@ -211,7 +212,7 @@ extern "C" int elfel_hip_t6_to_t3(elfel_hip_ctx *c, int src, int dst, int64_t *N
     if (src < 0 || src > 1 || dst < 0 || dst > 1 || src == dst)
         return set_err(c, ELFEL_EINVAL, "elfel_hip_t6_to_t3: bad fields");
     Field &s = c->f[src], &d = c->f[dst];
-    if (!s.has_mesh || s.nen != 6) return set_err(c, ELFEL_ESTATE, "elfel_hip_t6_to_t3: source field has no T6 mesh");
+    if (!s.has_mesh || s.nen != 6 || s.sdim != 2) return set_err(c, ELFEL_ESTATE, "elfel_hip_t6_to_t3: source field has no T6 mesh");
     const int64_t E = s.E, Nn6 = s.Nn;
     int rc;
     // flags (scanned in place), a copy of the flags, the total
@@ -249,6 +250,7 @@ extern "C" int elfel_hip_boundary_nodes(elfel_hip_ctx *c, int field, int64_t Nn,
     if (field < 0 || field > 1) return set_err(c, ELFEL_EINVAL, "elfel_hip_boundary_nodes: bad field");
     const Field &f = c->f[field];
     if (!f.has_mesh) return set_err(c, ELFEL_ESTATE, "elfel_hip_boundary_nodes: no mesh");
+    if (f.sdim != 2) return set_err(c, ELFEL_ENOTSUP, "elfel_hip_boundary_nodes: 2D meshes only");
     if (!flags || Nn != f.Nn) return set_err(c, ELFEL_EINVAL, "elfel_hip_boundary_nodes: flags must hold Nn entries");
     const int ncorner = f.nen == 6 ? 3 : f.nen;  // T3/T6: 3 corner edges, Q4: 4
     if (f.nen != 3 && f.nen != 4 && f.nen != 6) return set_err(c, ELFEL_ENOTSUP, "boundary of this element shape");
@@ -291,7 +293,8 @@ extern "C" int elfel_hip_copy_mesh(elfel_hip_ctx *c, int field, int64_t E, int64
     const Field &f = c->f[field];
     if (!f.has_mesh) return set_err(c, ELFEL_ESTATE, "elfel_hip_copy_mesh: no mesh");
     if (E != f.E || Nn != f.Nn) return set_err(c, ELFEL_EINVAL, "elfel_hip_copy_mesh: sizes differ from the mesh");
-    if (xy && Nn > 0) HIPCHK(c, hipMemcpyAsync(xy, f.xy.p, 16 * (size_t)Nn, hipMemcpyDeviceToHost, c->stream));
+    if (xy && Nn > 0)
+        HIPCHK(c, hipMemcpyAsync(xy, f.xy.p, 8 * (size_t)f.sdim * Nn, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (conn)
         for (int64_t i = 0; i < E * f.nen; i++) conn[i] = (int64_t)f.h_conn[i] + 1;

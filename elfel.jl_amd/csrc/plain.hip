@@ -241,6 +241,58 @@ static plain_fn pick_plain(int kind, int fe, int nq)
     return nullptr;
 }
 
+int plain_buffers(elfel_hip_ctx *c, int64_t n, int64_t nv, int64_t ndofs)
+{
+    int rc;
+    if ((rc = ensure(c, c->keys[0], sizeof(uint64_t) * n)) || (rc = ensure(c, c->keys[1], sizeof(uint64_t) * n)) ||
+        (rc = ensure(c, c->vals[0], sizeof(double) * n)) || (rc = ensure(c, c->vals[1], sizeof(double) * n)))
+        return rc;
+    if (nv > 0) {
+        if ((rc = ensure(c, c->fkeys[0], sizeof(uint64_t) * nv)) || (rc = ensure(c, c->fkeys[1], sizeof(uint64_t) * nv)) ||
+            (rc = ensure(c, c->fvals[0], sizeof(double) * nv)) || (rc = ensure(c, c->fvals[1], sizeof(double) * nv)) ||
+            (rc = ensure(c, c->F, sizeof(double) * ndofs)))
+            return rc;
+    }
+    if ((rc = ensure(c, c->ptr, sizeof(int32_t) * (ndofs + 1))) || (rc = ensure(c, c->ind, sizeof(int32_t) * n)) ||
+        (rc = ensure(c, c->val, sizeof(double) * n)))
+        return rc;
+    return 0;
+}
+
+int plain_finish(elfel_hip_ctx *c, int layout, int64_t n, int64_t nv, int64_t ndofs, int minor_bits)
+{
+    uint64_t *k0 = (uint64_t *)c->keys[0].p, *k1 = (uint64_t *)c->keys[1].p;
+    uint64_t *v0 = (uint64_t *)c->vals[0].p, *v1 = (uint64_t *)c->vals[1].p;
+    const bool vec = nv > 0;
+    uint64_t *fk0 = vec ? (uint64_t *)c->fkeys[0].p : nullptr, *fk1 = vec ? (uint64_t *)c->fkeys[1].p : nullptr;
+    uint64_t *fv0 = vec ? (uint64_t *)c->fvals[0].p : nullptr, *fv1 = vec ? (uint64_t *)c->fvals[1].p : nullptr;
+    hipEvent_t ev = nullptr;
+    tm_begin(c, ELFEL_PH_SORT, &ev);
+    int rc = radix_sort_pairs(c, k0, v0, k1, v1, n, bits_for(ndofs) + minor_bits);
+    if (rc) return rc;
+    if (vec) {
+        rc = radix_sort_pairs(c, fk0, fv0, fk1, fv1, nv, bits_for(ndofs));
+        if (rc) return rc;
+    }
+    tm_end(c, ELFEL_PH_SORT, ev);
+    tm_begin(c, ELFEL_PH_REDUCE, &ev);
+    rc = build_csr_from_sorted(c, k0, (const double *)v0, n, minor_bits, ndofs, (int32_t *)c->ptr.p,
+                               (int32_t *)c->ind.p, (double *)c->val.p, (int32_t *)c->dscal.p, true);
+    if (rc) return rc;
+    if (vec) {
+        rc = build_dense_from_sorted(c, fk0, (const double *)fv0, nv, ndofs, (double *)c->F.p);
+        if (rc) return rc;
+    }
+    tm_end(c, ELFEL_PH_REDUCE, ev);
+    c->has_F = vec;
+    c->nF = vec ? ndofs : 0;
+    c->nmaj = ndofs;
+    c->nmin = ndofs;
+    c->layout = layout;
+    c->result_is_pattern = false;
+    return 0;
+}
+
 int plain_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf)
 {
     const Field &F = c->f[0];
@@ -253,51 +305,13 @@ int plain_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
     const int64_t n = F.E * nblk;
     const int64_t ndofs = c->nrow;
     const int csc = pb->layout == ELFEL_CSC ? 1 : 0;
-    const int minor_bits = bits_for(ndofs), major_bits = bits_for(ndofs);
-    int rc;
-    if ((rc = ensure(c, c->keys[0], sizeof(uint64_t) * n)) || (rc = ensure(c, c->keys[1], sizeof(uint64_t) * n)) ||
-        (rc = ensure(c, c->vals[0], sizeof(double) * n)) || (rc = ensure(c, c->vals[1], sizeof(double) * n)))
-        return rc;
+    const int minor_bits = bits_for(ndofs);
     const bool heat = pb->kind == ELFEL_HEAT;
     const int64_t nv = heat ? F.E * nd : 0;
-    if (heat) {
-        if ((rc = ensure(c, c->fkeys[0], sizeof(uint64_t) * nv)) || (rc = ensure(c, c->fkeys[1], sizeof(uint64_t) * nv)) ||
-            (rc = ensure(c, c->fvals[0], sizeof(double) * nv)) || (rc = ensure(c, c->fvals[1], sizeof(double) * nv)) ||
-            (rc = ensure(c, c->F, sizeof(double) * ndofs)))
-            return rc;
-    }
-    if ((rc = ensure(c, c->ptr, sizeof(int32_t) * (ndofs + 1))) || (rc = ensure(c, c->ind, sizeof(int32_t) * n)) ||
-        (rc = ensure(c, c->val, sizeof(double) * n)))
-        return rc;
-    uint64_t *k0 = (uint64_t *)c->keys[0].p, *k1 = (uint64_t *)c->keys[1].p;
-    uint64_t *v0 = (uint64_t *)c->vals[0].p, *v1 = (uint64_t *)c->vals[1].p;
-    uint64_t *fk0 = heat ? (uint64_t *)c->fkeys[0].p : nullptr, *fk1 = heat ? (uint64_t *)c->fkeys[1].p : nullptr;
-    uint64_t *fv0 = heat ? (uint64_t *)c->fvals[0].p : nullptr, *fv1 = heat ? (uint64_t *)c->fvals[1].p : nullptr;
-    rc = fn(c, tab, cf, minor_bits, csc, k0, (double *)v0, fk0, (double *)fv0);
+    int rc = plain_buffers(c, n, nv, ndofs);
     if (rc) return rc;
-    hipEvent_t ev = nullptr;
-    tm_begin(c, ELFEL_PH_SORT, &ev);
-    rc = radix_sort_pairs(c, k0, v0, k1, v1, n, major_bits + minor_bits);
+    rc = fn(c, tab, cf, minor_bits, csc, (uint64_t *)c->keys[0].p, (double *)c->vals[0].p,
+            heat ? (uint64_t *)c->fkeys[0].p : nullptr, heat ? (double *)c->fvals[0].p : nullptr);
     if (rc) return rc;
-    if (heat) {
-        rc = radix_sort_pairs(c, fk0, fv0, fk1, fv1, nv, bits_for(ndofs));
-        if (rc) return rc;
-    }
-    tm_end(c, ELFEL_PH_SORT, ev);
-    tm_begin(c, ELFEL_PH_REDUCE, &ev);
-    rc = build_csr_from_sorted(c, k0, (const double *)v0, n, minor_bits, ndofs, (int32_t *)c->ptr.p,
-                               (int32_t *)c->ind.p, (double *)c->val.p, (int32_t *)c->dscal.p, true);
-    if (rc) return rc;
-    if (heat) {
-        rc = build_dense_from_sorted(c, fk0, (const double *)fv0, nv, ndofs, (double *)c->F.p);
-        if (rc) return rc;
-    }
-    tm_end(c, ELFEL_PH_REDUCE, ev);
-    c->has_F = heat;
-    c->nF = heat ? ndofs : 0;
-    c->nmaj = ndofs;
-    c->nmin = ndofs;
-    c->layout = pb->layout;
-    c->result_is_pattern = false;
-    return 0;
+    return plain_finish(c, pb->layout, n, nv, ndofs, minor_bits);
 }

@@ -129,7 +129,7 @@ extern "C" int elfel_hip_colliding_errors(elfel_hip_ctx *c, int64_t ndofs, const
     const Field &V = c->f[0], &P = c->f[1];
     if (!V.has_mesh || !V.has_dofs || !P.has_mesh || !P.has_dofs)
         return set_err(c, ELFEL_ESTATE, "colliding_errors: set_mesh/set_dofs of both fields first");
-    if (V.nen != 6 || V.ncomp != 2 || P.nen != 3 || P.ncomp != 1)
+    if (V.nen != 6 || V.ncomp != 2 || P.nen != 3 || P.ncomp != 1 || V.sdim != 2 || P.sdim != 2)
         return set_err(c, ELFEL_EINVAL, "colliding_errors: field 0 must be vector T6, field 1 scalar T3");
     if (!U || ndofs < std::max(V.ndofs, P.ndofs)) return set_err(c, ELFEL_EINVAL, "colliding_errors: bad vector");
     QTable t3, t6;

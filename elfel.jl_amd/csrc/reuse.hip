@@ -605,6 +605,8 @@ int geometry_stats_run(elfel_hip_ctx *c, double *min_det, int64_t *nbad)
 {
     const Field &F = c->f[0];
     if (!F.has_mesh) return set_err(c, ELFEL_ESTATE, "no mesh");
+    if (F.sdim != 2 || (F.nen != 3 && F.nen != 4 && F.nen != 6))
+        return set_err(c, ELFEL_ENOTSUP, "geometry statistics of T3, Q4 and T6 meshes only");
     if (!c->has_result && c->pat.fe < 0) return set_err(c, ELFEL_ESTATE, "assemble first");
     const int fe = F.nen == 3 ? ELFEL_T3 : (F.nen == 6 ? ELFEL_T6 : ELFEL_Q4);
     // Use the rule of the last pattern if any, else the element's common rule.

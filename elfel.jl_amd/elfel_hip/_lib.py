@@ -14,8 +14,8 @@ LIBPATH = os.environ.get("ELFEL_HIP_LIB", os.path.join(os.path.dirname(_PKG), "l
 # status codes
 OK, EINVAL, ERANGE, ENOMEM, EHIP, ESTATE, EGEOM, ENOTSUP = 0, -1, -2, -3, -4, -5, -6, -7
 # enums
-HEAT, ELAST, STOKES_GEN, STOKES_VECLAP = 0, 1, 2, 3
-T3, T6, Q4 = 0, 1, 2
+HEAT, ELAST, STOKES_GEN, STOKES_VECLAP, STOKES_REDDY, MASS = 0, 1, 2, 3, 4, 5
+T3, T6, Q4, T3B, Q4L2, T4, L2 = 0, 1, 2, 3, 4, 5, 6
 FP64, MIXED = 0, 1
 SORT, REUSE = 0, 1
 CSR, CSC = 0, 1
@@ -34,7 +34,7 @@ EXPORTED = [
     "elfel_hip_number_dofs", "elfel_hip_gather_sysvec", "elfel_hip_scatter_sysvec",
     "elfel_hip_colliding_errors",
     "elfel_hip_block_mesh", "elfel_hip_t6_to_t3", "elfel_hip_boundary_nodes", "elfel_hip_copy_mesh",
-    "elfel_hip_load_mesh",
+    "elfel_hip_load_mesh", "elfel_hip_set_mesh_sdim", "elfel_hip_set_element_dofs",
 ]
 
 
@@ -53,7 +53,7 @@ class Problem(ctypes.Structure):
         ("kind", ctypes.c_int32), ("fe", ctypes.c_int32), ("rule", ctypes.c_int32),
         ("precision", ctypes.c_int32), ("method", ctypes.c_int32), ("layout", ctypes.c_int32),
         ("kappa", ctypes.c_double), ("Q", ctypes.c_double), ("mu", ctypes.c_double),
-        ("D", ctypes.c_double * 9),
+        ("D", ctypes.c_double * 9), ("fe_p", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -83,6 +83,10 @@ def load():
     lib.elfel_hip_vector.argtypes = [P, ctypes.c_int64, ctypes.c_int64, i64p, f64p, f64p]
     lib.elfel_hip_set_mesh.argtypes = [P, ctypes.c_int, ctypes.c_int32, ctypes.c_int64, i64p, ctypes.c_int64, f64p]
     lib.elfel_hip_set_dofs.argtypes = [P, ctypes.c_int, ctypes.c_int32, ctypes.c_int64, i64p, ctypes.c_int64]
+    lib.elfel_hip_set_mesh_sdim.argtypes = [P, ctypes.c_int, ctypes.c_int32, ctypes.c_int64, i64p, ctypes.c_int64,
+                                            ctypes.c_int32, f64p]
+    lib.elfel_hip_set_element_dofs.argtypes = [P, ctypes.c_int, ctypes.c_int32, ctypes.c_int64, i64p,
+                                               ctypes.c_int64]
     lib.elfel_hip_set_coefficients.argtypes = [P, ctypes.c_int64, f64p, f64p, f64p]
     lib.elfel_hip_assemble.argtypes = [P, ctypes.POINTER(Problem), i64p]
     lib.elfel_hip_copy_vector.argtypes = [P, f64p, ctypes.c_int64]
@@ -203,10 +207,32 @@ class Context:
 
     # ---- device-resident problem ----------------------------------------------
     def set_mesh(self, field, conn, xy):
+        """conn: E x nen (1-based); xy: Nn x sdim coordinates (sdim 1, 2 or 3)."""
         conn, xy = _i64(conn), _f64(xy)
+        if xy.ndim == 1:
+            xy = xy.reshape(-1, 1)
         E, nen = conn.shape
-        self._chk(self.lib.elfel_hip_set_mesh(self.h, field, nen, E, _ptr(conn, ctypes.c_int64), xy.shape[0],
-                                              _ptr(xy, ctypes.c_double)), "set_mesh")
+        Nn, sdim = xy.shape
+        if sdim == 2:
+            rc = self.lib.elfel_hip_set_mesh(self.h, field, nen, E, _ptr(conn, ctypes.c_int64), Nn,
+                                             _ptr(xy, ctypes.c_double))
+        else:
+            rc = self.lib.elfel_hip_set_mesh_sdim(self.h, field, nen, E, _ptr(conn, ctypes.c_int64), Nn, sdim,
+                                                  _ptr(xy, ctypes.c_double))
+        self._chk(rc, "set_mesh")
+        self._set_sdim(field, sdim)
+
+    def _set_sdim(self, field, sdim):
+        self._sdim = getattr(self, "_sdim", {})
+        self._sdim[field] = sdim
+
+    def set_element_dofs(self, field, dofnums, ndofs):
+        """Cell dofs (bubble, FEL2_Q4): E x ncomp 1-based numbers."""
+        d = _i64(dofnums)
+        if d.ndim == 1:
+            d = d.reshape(-1, 1)
+        self._chk(self.lib.elfel_hip_set_element_dofs(self.h, field, d.shape[1], d.shape[0],
+                                                      _ptr(d, ctypes.c_int64), ndofs), "set_element_dofs")
 
     def set_dofs(self, field, dofnums, ndofs):
         d = _i64(dofnums)
@@ -225,9 +251,9 @@ class Context:
                   "set_coefficients")
 
     def assemble(self, kind, fe, rule, precision=FP64, method=SORT, layout=CSR, kappa=1.0, Q=0.0, mu=1.0,
-                 D=None, wait=True):
+                 D=None, wait=True, fe_p=T3):
         pb = Problem()
-        pb.kind, pb.fe, pb.rule = kind, fe, rule
+        pb.kind, pb.fe, pb.rule, pb.fe_p = kind, fe, rule, fe_p
         pb.precision, pb.method, pb.layout = precision, method, layout
         pb.kappa, pb.Q, pb.mu = kappa, Q, mu
         if D is not None:
@@ -293,12 +319,14 @@ class Context:
         E, Nn = ctypes.c_int64(), ctypes.c_int64()
         self._chk(self.lib.elfel_hip_block_mesh(self.h, field, fe, A, B, nA, nB, ctypes.byref(E), ctypes.byref(Nn)),
                   "block_mesh")
+        self._set_sdim(field, 2)
         return E.value, Nn.value
 
     def t6_to_t3(self, src, dst):
         """T6toT3 of field src's mesh into field dst; returns the node count."""
         n = ctypes.c_int64()
         self._chk(self.lib.elfel_hip_t6_to_t3(self.h, src, dst, ctypes.byref(n)), "t6_to_t3")
+        self._set_sdim(dst, 2)
         return n.value
 
     def boundary_nodes(self, field, Nn):
@@ -316,12 +344,15 @@ class Context:
         nen, E, Nn = ctypes.c_int32(), ctypes.c_int64(), ctypes.c_int64()
         self._chk(self.lib.elfel_hip_load_mesh(self.h, field, os.fsencode(path), ctypes.byref(nen), ctypes.byref(E),
                                                ctypes.byref(Nn)), "load_mesh")
+        self._set_sdim(field, 2)
         return nen.value, E.value, Nn.value
 
-    def copy_mesh(self, field, E, nen, Nn):
-        """(conn E x nen 1-based int64, xy Nn x 2) of `field`'s mesh."""
+    def copy_mesh(self, field, E, nen, Nn, sdim=None):
+        """(conn E x nen 1-based int64, xy Nn x sdim) of `field`'s mesh."""
         conn = np.zeros((E, nen), np.int64)
-        xy = np.zeros((Nn, 2))
+        if sdim is None:
+            sdim = getattr(self, "_sdim", {}).get(field, 2)
+        xy = np.zeros((Nn, sdim))
         self._chk(self.lib.elfel_hip_copy_mesh(self.h, field, E, _ptr(conn, ctypes.c_int64), Nn,
                                                _ptr(xy, ctypes.c_double)), "copy_mesh")
         return conn, xy

@@ -36,10 +36,18 @@ enum {
     ELFEL_ENOTSUP = -7   /* combination not supported */
 };
 
-/* weak forms */
-enum { ELFEL_HEAT = 0, ELFEL_ELAST = 1, ELFEL_STOKES_GEN = 2, ELFEL_STOKES_VECLAP = 3 };
-/* scalar finite elements */
-enum { ELFEL_T3 = 0, ELFEL_T6 = 1, ELFEL_Q4 = 2 };
+/* weak forms: heat (test/test_heat.jl:36-60, examples/heat/poisson/t4.jl:33-55),
+ * plane elasticity, Taylor-Hood Stokes over one vector space (generic B'DB or
+ * vector Laplacian), Reddy-form Stokes over three scalar spaces ux, uy, p
+ * (test/test_stokes.jl:373-431, examples/stokes/colliding_flow/p1b_p1.jl:50-108,
+ * q1_q0.jl:52-110), and the mass matrix N_i N_j (kappa JxW)
+ * (examples/klein_gordon/klein_gordon.jl:34-51,94-112) */
+enum { ELFEL_HEAT = 0, ELFEL_ELAST = 1, ELFEL_STOKES_GEN = 2, ELFEL_STOKES_VECLAP = 3, ELFEL_STOKES_REDDY = 4,
+       ELFEL_MASS = 5 };
+/* scalar finite elements: FEH1_T3, FEH1_T6, FEH1_Q4, FEH1_T3_BUBBLE (T3 + a
+ * cubic bubble with one dof on the cell), FEL2_Q4 (one dof on the cell, the
+ * Q4 its geometry carrier), FEH1_T4 (3D), FEH1_L2 (1D); src/FElements.jl */
+enum { ELFEL_T3 = 0, ELFEL_T6 = 1, ELFEL_Q4 = 2, ELFEL_T3B = 3, ELFEL_Q4L2 = 4, ELFEL_T4 = 5, ELFEL_L2 = 6 };
 /* arithmetic of the element kernels */
 enum { ELFEL_FP64 = 0, ELFEL_MIXED = 1 };
 /* global build method */
@@ -93,6 +101,18 @@ int elfel_hip_set_mesh(elfel_hip_ctx *ctx, int field, int32_t nen, int64_t E, co
                        int64_t Nn, const double *xy);
 int elfel_hip_set_dofs(elfel_hip_ctx *ctx, int field, int32_t ncomp, int64_t Nn,
                        const int64_t *dofnums, int64_t ndofs);
+/* A mesh in sdim = 1, 2 or 3 space dimensions (x[sdim*n + d], the memory of
+ * Vector{SVector{sdim,Float64}}): L2 meshes (sdim 1) and T4 meshes (sdim 3)
+ * for the element families; elfel_hip_set_mesh is sdim = 2. */
+int elfel_hip_set_mesh_sdim(elfel_hip_ctx *ctx, int field, int32_t nen, int64_t E, const int64_t *conn,
+                            int64_t Nn, int32_t sdim, const double *x);
+/* Dof numbers of the cells (the incidence relation of the element's own
+ * dimension, FESpace._irsfields[2], src/FESpaces.jl:76-86): dofnums[ncomp*e +
+ * c], 1-based.  The bubble of FEH1_T3_BUBBLE and the FEL2_Q4 pressure; an
+ * element's dofs are its nodes' dofs followed by its cell dofs, the order of
+ * eldofs (src/FEIterators.jl:161-171).  A field may hold cell dofs only. */
+int elfel_hip_set_element_dofs(elfel_hip_ctx *ctx, int field, int32_t ncomp, int64_t E, const int64_t *dofnums,
+                               int64_t ndofs);
 /* Per-element coefficients (any pointer may be NULL = uniform value of the
  * problem): conductivity, heat source, scale of the material matrix D. */
 int elfel_hip_set_coefficients(elfel_hip_ctx *ctx, int64_t E, const double *kappa_e,
@@ -108,6 +128,9 @@ typedef struct {
     double kappa, Q;    /* heat: uniform conductivity and source */
     double mu;          /* Stokes vector-Laplacian viscosity */
     double D[9];        /* elasticity / Stokes-gen material matrix, column-major */
+    int32_t fe_p;       /* Reddy Stokes: pressure element, ELFEL_T3 (field 1's nodes) or ELFEL_Q4L2
+                           (field 1's cell dofs); pairs (T6, T3), (T3B, T3), (Q4, Q4L2) */
+    int32_t reserved;
 } elfel_hip_problem;
 
 /* Integrates every element and builds the global matrix (and the load vector
@@ -115,7 +138,15 @@ typedef struct {
  * LocalMatrixAssembler + SysmatAssemblerSparse (test/test_heat.jl:36-71,
  * test/test_elasticity.jl:149-181, examples/stokes/colliding_flow/
  * ht_p2_p1_gen.jl:46-90, test/test_stokes.jl:629-668).  nnz may be NULL:
- * then the call does not wait for the device. */
+ * then the call does not wait for the device.
+ * Element families (T3B, Q4L2, T4, L2) and the Reddy and mass forms run on
+ * the plain FP64 method (ELFEL_SORT, ELFEL_FP64): heat on T4 (rule = tetrahedron
+ * points 1 or 4) and L2 (Gauss order 1-3, sdim 1); mass on T3, T6, Q4, T4, L2;
+ * Reddy Stokes with field 0 the velocity mesh, its node dofs ncomp = 2
+ * (component 1 the ux space's numbers, 2 the uy space's) and for T3B its cell
+ * dofs ncomp = 2, field 1 the pressure (T3 mesh + node dofs, or Q4L2 cell
+ * dofs).  Blocks are appended as the drivers do: kxx, kxy, kxy', kyy, kxp,
+ * kxp', kyp, kyp'. */
 int elfel_hip_assemble(elfel_hip_ctx *ctx, const elfel_hip_problem *pb, int64_t *nnz);
 /* Load vector of the last heat assembly (finish!(SysvecAssembler), :228-230). */
 int elfel_hip_copy_vector(elfel_hip_ctx *ctx, double *F, int64_t n);
@@ -185,7 +216,7 @@ int elfel_hip_boundary_nodes(elfel_hip_ctx *ctx, int field, int64_t Nn, uint8_t 
  * manifest) read on the host and set as field `field`'s mesh. */
 int elfel_hip_load_mesh(elfel_hip_ctx *ctx, int field, const char *path, int32_t *nen, int64_t *E, int64_t *Nn);
 /* Copies field `field`'s mesh to the host: conn (E x nen, 1-based, row per
- * element) and xy (Nn x 2); either may be null. */
+ * element) and xy (Nn x sdim, sdim of the mesh); either may be null. */
 int elfel_hip_copy_mesh(elfel_hip_ctx *ctx, int field, int64_t E, int64_t *conn, int64_t Nn, double *xy);
 
 /* ---- post-processing -------------------------------------------------------

@@ -22,10 +22,12 @@
 extern "C" {
 #endif
 
-/* finite element codes */
-enum { ORC_T3 = 0, ORC_T6 = 1, ORC_Q4 = 2 };
+/* finite element codes: FEH1_T3, FEH1_T6, FEH1_Q4, FEH1_T3_BUBBLE, FEL2_Q4,
+ * FEH1_T4, FEH1_L2 */
+enum { ORC_T3 = 0, ORC_T6 = 1, ORC_Q4 = 2, ORC_T3B = 3, ORC_Q4L2 = 4, ORC_T4 = 5, ORC_L2 = 6 };
 /* weak forms */
-enum { ORC_HEAT = 0, ORC_ELAST = 1, ORC_STOKES_GEN = 2, ORC_STOKES_VECLAP = 3 };
+enum { ORC_HEAT = 0, ORC_ELAST = 1, ORC_STOKES_GEN = 2, ORC_STOKES_VECLAP = 3, ORC_STOKES_REDDY = 4,
+       ORC_MASS = 5 };
 
 typedef struct {
     int kind;          /* ORC_HEAT / ORC_ELAST / ORC_STOKES_* */
@@ -83,6 +85,28 @@ int orc_stokes_reddy_coo(int64_t E, const int64_t *conn6, const double *xy6,
                          const int64_t *conn3, const double *xy3,
                          const int64_t *dn_ux, const int64_t *dn_uy, const int64_t *dn_p,
                          double mu, int64_t *I, int64_t *J, double *V);
+
+/* ---- element families (elfel_oracle_fam.c) ---- */
+int orc_fam_nbf(int fe);
+int orc_fam_ngeo(int fe);
+int orc_fam_mdim(int fe);
+int orc_fam_quad(int fe, int rule, double *pc /* 3 per point */, double *w);
+void orc_fam_bfun(int fe, const double *pc, double *N);
+void orc_fam_bfungradpar(int fe, const double *pc, double *g /* nbf*mdim */);
+/* heat / mass on a nodal family (T3, T6, Q4, T4, L2); x holds mdim
+ * coordinates per node; dofnums one per node */
+int64_t orc_fam_scalar_coo_len(int fe, int64_t E, int64_t *nvec);
+int orc_fam_scalar_coo(int kind, int fe, int rule, int64_t E, const int64_t *conn, const double *x,
+                       const int64_t *dofnums, double kappa, double Q, int64_t *I, int64_t *J, double *V,
+                       int64_t *FI, double *FV);
+/* Reddy-form Stokes: (T6, T3), (T3B, T3), (Q4, Q4L2) */
+int64_t orc_fam_reddy_coo_len(int vfe, int pfe, int64_t E);
+int orc_fam_reddy_coo(int vfe, int pfe, int rule, int64_t E, const int64_t *conn_v, const double *xy_v,
+                      const int64_t *conn_p, const double *xy_p, const int64_t *dn_ux, const int64_t *dn_uy,
+                      const int64_t *de_ux, const int64_t *de_uy, const int64_t *dn_p, const int64_t *de_p,
+                      double mu, int64_t *I, int64_t *J, double *V);
+int64_t orc_l2block(double L, int n, int64_t *conn, double *x);
+int64_t orc_t4block(double A, double B, double C, int nA, int nB, int nC, int64_t *conn, double *xyz);
 
 /* ---- SparseArrays.sparse equivalent (src/Assemblers.jl:121-123) ---- */
 int64_t orc_sparse(int64_t n, const int64_t *I, const int64_t *J, const double *V,

@@ -15,8 +15,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIBPATH = os.path.join(_HERE, "liborc.so")
 
-T3, T6, Q4 = 0, 1, 2
-HEAT, ELAST, STOKES_GEN, STOKES_VECLAP = 0, 1, 2, 3
+T3, T6, Q4, T3B, Q4L2, T4, L2 = 0, 1, 2, 3, 4, 5, 6
+HEAT, ELAST, STOKES_GEN, STOKES_VECLAP, STOKES_REDDY, MASS = 0, 1, 2, 3, 4, 5
 
 _i64p = ctypes.POINTER(ctypes.c_int64)
 _f64p = ctypes.POINTER(ctypes.c_double)
@@ -46,6 +46,8 @@ def _load():
     lib.orc_t6block.restype = ctypes.c_int64
     lib.orc_t6tot3.restype = ctypes.c_int64
     lib.orc_stokes_reddy_coo_len.restype = ctypes.c_int64
+    for f in ("orc_fam_scalar_coo_len", "orc_fam_reddy_coo_len", "orc_l2block", "orc_t4block"):
+        getattr(lib, f).restype = ctypes.c_int64
     for f in ("orc_colliding_p_error", "orc_colliding_v_error_scalar", "orc_colliding_v_error_vector"):
         getattr(lib, f).restype = ctypes.c_double
     return lib
@@ -201,6 +203,93 @@ def stokes_reddy_coo(conn6, xy6, conn3, xy3, dn_ux, dn_uy, dn_p, mu):
                              _p(dn_ux, _i64p), _p(dn_uy, _i64p), _p(dn_p, _i64p), ctypes.c_double(mu),
                              _p(I, _i64p), _p(J, _i64p), _p(V, _f64p))
     return I, J, V
+
+
+# ---- element families (elfel_oracle_fam.c) --------------------------------
+def fam_nbf(fe):
+    return lib.orc_fam_nbf(fe)
+
+
+def fam_mdim(fe):
+    return lib.orc_fam_mdim(fe)
+
+
+def fam_quad(fe, rule):
+    pc, w = np.zeros(48), np.zeros(16)
+    n = lib.orc_fam_quad(fe, rule, _p(pc, _f64p), _p(w, _f64p))
+    if n < 0:
+        raise ValueError(f"no rule {rule} for element {fe}")
+    return pc[: 3 * n].reshape(n, 3)[:, : fam_mdim(fe)].copy(), w[:n].copy()
+
+
+def fam_bfun(fe, pc):
+    pcx = np.zeros(3)
+    pcx[: len(pc)] = pc
+    N = np.zeros(8)
+    lib.orc_fam_bfun(fe, _p(pcx, _f64p), _p(N, _f64p))
+    return N[: fam_nbf(fe)].copy()
+
+
+def fam_bfungradpar(fe, pc):
+    pcx = np.zeros(3)
+    pcx[: len(pc)] = pc
+    g = np.zeros(24)
+    lib.orc_fam_bfungradpar(fe, _p(pcx, _f64p), _p(g, _f64p))
+    m = fam_mdim(fe)
+    return g[: m * fam_nbf(fe)].reshape(-1, m).copy()
+
+
+def fam_scalar_coo(kind, fe, rule, conn, x, dofnums, kappa=1.0, Q=0.0):
+    """Heat (grad-grad + source) or mass on a nodal family; x is (Nn, mdim)."""
+    conn, x, dofnums = _i64(conn), _f64(x), _i64(dofnums).ravel()
+    E = conn.shape[0]
+    nv = ctypes.c_int64(0)
+    n = lib.orc_fam_scalar_coo_len(fe, E, ctypes.byref(nv))
+    I, J, V = np.zeros(n, np.int64), np.zeros(n, np.int64), np.zeros(n)
+    FI, FV = np.zeros(nv.value, np.int64), np.zeros(nv.value)
+    rc = lib.orc_fam_scalar_coo(kind, fe, rule, E, _p(conn, _i64p), _p(x, _f64p), _p(dofnums, _i64p),
+                                ctypes.c_double(kappa), ctypes.c_double(Q), _p(I, _i64p), _p(J, _i64p),
+                                _p(V, _f64p), _p(FI, _i64p), _p(FV, _f64p))
+    if rc != 0:
+        raise ValueError(f"orc_fam_scalar_coo failed ({rc})")
+    if kind != HEAT:
+        FI, FV = FI[:0], FV[:0]
+    return I, J, V, FI, FV
+
+
+def fam_reddy_coo(vfe, pfe, rule, conn_v, xy_v, dn_ux, dn_uy, mu=1.0, de_ux=None, de_uy=None,
+                  conn_p=None, xy_p=None, dn_p=None, de_p=None):
+    """Reddy-form Stokes over the spaces ux, uy (node dofs dn_*, cell dofs de_*) and p."""
+    k = {"conn_v": _i64(conn_v), "xy_v": _f64(xy_v), "dn_ux": _i64(dn_ux), "dn_uy": _i64(dn_uy)}
+    for name, v in (("de_ux", de_ux), ("de_uy", de_uy), ("dn_p", dn_p), ("de_p", de_p)):
+        k[name] = None if v is None else _i64(v)
+    k["conn_p"] = None if conn_p is None else _i64(conn_p)
+    k["xy_p"] = None if xy_p is None else _f64(xy_p)
+    E = k["conn_v"].shape[0]
+    n = lib.orc_fam_reddy_coo_len(vfe, pfe, E)
+    I, J, V = np.zeros(n, np.int64), np.zeros(n, np.int64), np.zeros(n)
+    rc = lib.orc_fam_reddy_coo(vfe, pfe, rule, E, _p(k["conn_v"], _i64p), _p(k["xy_v"], _f64p),
+                               _p(k["conn_p"], _i64p), _p(k["xy_p"], _f64p), _p(k["dn_ux"], _i64p),
+                               _p(k["dn_uy"], _i64p), _p(k["de_ux"], _i64p), _p(k["de_uy"], _i64p),
+                               _p(k["dn_p"], _i64p), _p(k["de_p"], _i64p), ctypes.c_double(mu),
+                               _p(I, _i64p), _p(J, _i64p), _p(V, _f64p))
+    if rc != 0:
+        raise ValueError(f"orc_fam_reddy_coo failed ({rc})")
+    return I, J, V
+
+
+def l2block(L, n):
+    conn, x = np.zeros((n, 2), np.int64), np.zeros((n + 1, 1))
+    lib.orc_l2block(ctypes.c_double(L), n, _p(conn, _i64p), _p(x, _f64p))
+    return conn, x
+
+
+def t4block(A, B, C, nA, nB, nC):
+    E, Nn = 6 * nA * nB * nC, (nA + 1) * (nB + 1) * (nC + 1)
+    conn, xyz = np.zeros((E, 4), np.int64), np.zeros((Nn, 3))
+    lib.orc_t4block(ctypes.c_double(A), ctypes.c_double(B), ctypes.c_double(C), nA, nB, nC,
+                    _p(conn, _i64p), _p(xyz, _f64p))
+    return conn, xyz
 
 
 def sparse(I, J, V, nrow, ncol, csc=True):
