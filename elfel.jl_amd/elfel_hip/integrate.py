@@ -8,6 +8,15 @@ calls, and leaves the global matrix on the device for finish(am):
 * integrate_elasticity  test/test_elasticity.jl:149-181
 * integrate_stokes      examples/stokes/colliding_flow/ht_p2_p1_gen.jl:46-90 (form="gen"),
                         test/test_stokes.jl:629-668 (form="veclap")
+* integrate_stokes_reddy  three scalar spaces ux, uy, p: test/test_stokes.jl:373-431
+                        (T6/T3), examples/stokes/colliding_flow/p1b_p1.jl:50-108
+                        (FEH1_T3_BUBBLE/T3), q1_q0.jl:52-110 (Q4/FEL2_Q4)
+* integrate_mass        examples/klein_gordon/klein_gordon.jl:34-51 (M), :94-112 (m^2 M)
+
+integrate_heat also takes FEH1_T4 (examples/heat/poisson/t4.jl:33-55) and
+FEH1_L2 spaces (klein_gordon.jl:64-81 is the heat form with kappa = 1, Q = 0).
+The element families, the Reddy form and the mass form run on the plain FP64
+method.
 
 precision: "fp64" (the plain method: every operation as the reference does it)
 or "mixed" (FP32 element arithmetic, FP64 sums into the matrix).
@@ -75,6 +84,40 @@ def integrate_stokes(am, ufesp, pfesp, qsettings, form="gen", D=None, mu=1.0, pr
         D = np.diag([2 * mu, 2 * mu, mu])
     return _run(am, [ufesp, pfesp], kind, ufesp.fe.code, rule_id("triangle", qsettings), precision, method, layout,
                 (None, None, None), D=np.asarray(D, np.float64), mu=float(mu))
+
+
+def integrate_mass(am, fesp, qsettings, coef=1.0, layout="csc"):
+    """me[i, j] += (N_i N_j) * (coef JxW) over every element."""
+    fe = fesp.fe
+    return _run(am, [fesp], _lib.MASS, fe.code, rule_id(fe.refshape, qsettings), "fp64", "sort", layout,
+                (None, None, None), kappa=float(coef))
+
+
+def integrate_stokes_reddy(am, uxfesp, uyfesp, pfesp, qsettings, mu=1.0, layout="csc"):
+    """The Reddy-form Stokes blocks kxx, kxy, kxy', kyy, kxp, kxp', kyp, kyp'
+    of the spaces ux, uy (one mesh, one element) and p."""
+    if uxfesp.mesh is not uyfesp.mesh or type(uxfesp.fe) is not type(uyfesp.fe):
+        raise ValueError("ux and uy must be spaces of one element type on one mesh")
+    ctx = am.context()
+    n = am.nrow
+    if n <= 0 or am.ncol != n:
+        raise ValueError("start(am, ndofs, ndofs) before integrating")
+    mesh = uxfesp.mesh
+    ctx.set_mesh(0, mesh.conn, mesh.xy)
+    ctx.set_dofs(0, np.stack([uxfesp.dofnums[:, 0], uyfesp.dofnums[:, 0]], 1), n)
+    if uxfesp.cell_dofnums.shape[0]:
+        ctx.set_element_dofs(0, np.stack([uxfesp.cell_dofnums[:, 0], uyfesp.cell_dofnums[:, 0]], 1), n)
+    if pfesp.dofnums.shape[0]:
+        ctx.set_mesh(1, pfesp.mesh.conn, pfesp.mesh.xy)
+        ctx.set_dofs(1, pfesp.dofnums, n)
+    else:
+        ctx.set_element_dofs(1, pfesp.cell_dofnums, n)
+    ctx._bound = {}  # fields 0 and 1 now hold these spaces
+    ctx.set_coefficients(mesh.nelements, None, None, None)
+    nnz = ctx.assemble(_lib.STOKES_REDDY, uxfesp.fe.code, rule_id(uxfesp.fe.refshape, qsettings),
+                       layout=_LAYOUT[layout], mu=float(mu), fe_p=pfesp.fe.code)
+    am._device_result = True
+    return nnz
 
 
 def plane_strain_D(E, nu):

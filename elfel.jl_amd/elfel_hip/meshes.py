@@ -14,6 +14,14 @@ incidence relation (connectivity, element-major, 1-based Int64) and the
   order (the pressure mesh of test/test_stokes.jl:32,37).
 * load: the reference's .mesh manifest (xyz file, shape name, conn file;
   test/qmesh.mesh:1-3).
+* L2block: nodes L i / n, segments (i, i+1) (the interval of
+  examples/klein_gordon/klein_gordon.jl:29).
+* T4block: nodes x-fastest, then y, then z; each cell (x-index fastest, then
+  y, then z) split into six positively oriented tetrahedra around its
+  diagonal (i, j, k)-(i+1, j+1, k+1) (the cube of
+  examples/heat/poisson/t4.jl:23).  MeshSteward's own L2block / T4block are
+  not in the reference tree: these orders are this package's, shared with
+  the oracle.
 """
 import os
 
@@ -40,6 +48,10 @@ class Mesh:
     def boundary_nodes(self):
         """connectedv(boundary(mesh)): nodes on edges that belong to one element."""
         return boundary_nodes(self)
+
+    @property
+    def sdim(self):
+        return self.xy.shape[1]
 
     def vselect(self, box, inflate=0.0):
         """Nodes inside box = [xmin, xmax, ymin, ymax] enlarged by inflate (MeshSteward.vselect)."""
@@ -116,6 +128,30 @@ def T6toT3(mesh):
     return Mesh(newnum[mesh.conn[:, :3]], mesh.xy[corners - 1], "T3")
 
 
+def L2block(L, n):
+    conn = np.stack([np.arange(1, n + 1), np.arange(2, n + 2)], 1)
+    return Mesh(conn, _lin(L, n).reshape(-1, 1), "L2")
+
+
+# corners of the unit cube (bit 0 = x, bit 1 = y, bit 2 = z) of the six
+# tetrahedra (0, a, a|b, 7) along the monotone paths from corner 0 to 7; the
+# last two are swapped where that makes the orientation positive
+_KUHN = [(0, 1, 3, 7), (0, 1, 7, 5), (0, 2, 7, 3), (0, 2, 6, 7), (0, 4, 5, 7), (0, 4, 7, 6)]
+
+
+def T4block(A, B, C, nA, nB, nC):
+    xs, ys, zs = _lin(A, nA), _lin(B, nB), _lin(C, nC)
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")  # x fastest when raveled
+    xyz = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    W, S = nA + 1, (nA + 1) * (nB + 1)
+    k, j, i = np.meshgrid(np.arange(nC), np.arange(nB), np.arange(nA), indexing="ij")  # x-index fastest
+    base = (k.ravel() * S + j.ravel() * W + i.ravel() + 1).astype(np.int64)
+    corner = [((b >> 2) & 1) * S + ((b >> 1) & 1) * W + (b & 1) for b in range(8)]
+    conn = np.stack([np.stack([base + corner[t[0]], base + corner[t[1]], base + corner[t[2]], base + corner[t[3]]], 1)
+                     for t in _KUHN], 1).reshape(-1, 4)
+    return Mesh(conn, xyz, "T4")
+
+
 def transform(mesh, f):
     mesh.xy = np.ascontiguousarray(f(mesh.xy), dtype=np.float64)
     return mesh
@@ -123,6 +159,12 @@ def transform(mesh, f):
 
 def boundary_nodes(mesh):
     c = mesh.conn
+    if mesh.shape in ("L2", "T4"):
+        # facets (end points, triangles) that belong to one element
+        faces = [(0,), (1,)] if mesh.shape == "L2" else [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]
+        f = np.sort(np.concatenate([c[:, list(t)] for t in faces]), axis=1)
+        _, inv, cnt = np.unique(f, axis=0, return_inverse=True, return_counts=True)
+        return np.unique(f[cnt[inv.ravel()] == 1].ravel())
     if mesh.shape == "T3" or mesh.shape == "T6":
         ends = [(0, 1, 3), (1, 2, 4), (2, 0, 5)]
     elif mesh.shape == "Q4":

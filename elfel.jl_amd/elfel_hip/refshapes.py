@@ -2,7 +2,8 @@
 
 Rules and literal constants follow src/RefShapes.jl: Gauss rules `_gauss1`
 (:85-110), triangle rules `_triangle` (:113-230), the square tensor rule with
-i outer / j inner (:333-366).  Settings mirror the reference's NamedTuples:
+i outer / j inner (:333-366), tetrahedron rules `_tetrahedron` (:232-258).
+Settings mirror the reference's NamedTuples:
 quadrature("triangle", kind="default", npts=3), quadrature("square",
 kind="Gauss", order=2).
 """
@@ -52,6 +53,18 @@ def _triangle(npts):
     raise ValueError(f"Unknown number of integration points {npts}")
 
 
+def _tetrahedron(npts):
+    if npts == 1:
+        return np.array([[0.25, 0.25, 0.25]]), np.array([1.0 / 6.0])
+    if npts == 4:
+        a, b = 0.13819660, 0.58541020
+        return np.array([[a, a, a], [b, a, a], [a, b, a], [a, a, b]]), np.full(4, 0.041666666666666666667)
+    if npts == 5:
+        a, b, c, d, e = 1.0 / 6.0, 0.25, 0.5, -0.8, 0.45
+        return np.array([[b, b, b], [c, a, a], [a, c, a], [a, a, c], [a, a, a]]), np.array([d, e, e, e, e]) / 6
+    raise ValueError(f"Unknown number of integration points {npts}")
+
+
 def quadrature(refshape, kind="default", npts=None, order=None):
     if refshape == "interval":
         if kind not in ("default", "Gauss"):
@@ -71,11 +84,17 @@ def quadrature(refshape, kind="default", npts=None, order=None):
         pc = np.array([[p1[i], p1[j]] for i in range(o) for j in range(o)])
         w = np.array([w1[i] * w1[j] for i in range(o) for j in range(o)])
         return IntegRule(pc, w)
+    if refshape == "tetrahedron":
+        if kind != "default":
+            raise ValueError(f"Integration rule {kind} not available")
+        pc, w = _tetrahedron(npts or 1)
+        return IntegRule(pc, w)
     raise ValueError(f"unknown reference shape {refshape}")
 
 
 def rule_id(refshape, settings):
-    """Device rule id: triangle -> number of points, square -> Gauss order."""
-    if refshape == "triangle":
+    """Device rule id: triangle, tetrahedron -> number of points; square,
+    interval -> Gauss order."""
+    if refshape in ("triangle", "tetrahedron"):
         return int(settings.get("npts", 1))
     return int(settings.get("order", 1))
