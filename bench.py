@@ -22,6 +22,10 @@ Other workloads, each printing its own line:
       problem by problem;
   c5  heat Q4 4096x4096, 2x2 Gauss, lognormal conductivity, then Jacobi-PCG
       (FP32 inner iterations + FP64 refinement, and FP64) to 1e-10.
+Element families (the plain FP64 method, the only one they run on):
+  t4  heat on a 64^3 block of T4 tetrahedra (examples/heat/poisson/t4.jl);
+  p1b Reddy-form Stokes, T3 + bubble velocity / T3 pressure, 512 x 512
+      (examples/stokes/colliding_flow/p1b_p1.jl).
 
 Prints ONE JSON line.  --gpus N > 1 is not supported (one GPU per box).
 """
@@ -45,7 +49,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", choices=["c1", "c2", "c3", "c4", "c5"], default="c2")
+    ap.add_argument("--config", choices=["c1", "c2", "c3", "c4", "c5", "t4", "p1b"], default="c2")
     ap.add_argument("--N", type=int, default=0, help="grid size (default: the configuration's)")
     ap.add_argument("--fp64-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -567,6 +571,140 @@ def cpu_baseline_batch(probs, sample):
                       f"({reps * (te + ts):.1f} s): element loop {te:.4f} s + sparse() {ts:.4f} s each, single thread"}
 
 
+# ---------------------------------------------------------------------------
+# element families (plain FP64 method)
+# ---------------------------------------------------------------------------
+def family_case(cfgname, N):
+    """Mesh and numbering of a family workload (host arrays)."""
+    from elfel_hip import meshes as M
+    if cfgname == "t4":
+        N = N or 64
+        m = M.T4block(1.0, 1.0, 1.0, N, N, N)
+        E, Nn = m.nelements, m.nnodes
+        return dict(N=N, E=E, Nn=Nn, n=Nn, conn=m.conn, x=m.xy, dn=np.arange(1, Nn + 1),
+                    workload=f"heat T4 {N}^3 block (six tetrahedra per cell), 1-pt rule, kappa 1, Q -6, CSR + "
+                             f"load vector",
+                    kernel="k_fam_scalar<HEAT, 3D, T4, 1 point>",
+                    kernel_in=E * 16 + Nn * 24 + Nn * 4, per_el_out=16 * 16 + 4 * 16,
+                    mesh_bytes=E * 16 + Nn * 24 + Nn * 4, heat=True)
+    N = N or 512
+    m = M.transform(M.T3block(2.0, 2.0, N, N), lambda x: x - 1.0)
+    E, Nn = m.nelements, m.nnodes
+    dux, duy = np.arange(1, Nn + 1), np.arange(Nn + 1, 2 * Nn + 1)
+    eux, euy = np.arange(2 * Nn + 1, 2 * Nn + E + 1), np.arange(2 * Nn + E + 1, 2 * Nn + 2 * E + 1)
+    dp = np.arange(2 * Nn + 2 * E + 1, 3 * Nn + 2 * E + 1)
+    return dict(N=N, E=E, Nn=Nn, n=3 * Nn + 2 * E, conn=m.conn, x=m.xy, dux=dux, duy=duy, eux=eux, euy=euy, dp=dp,
+                workload=f"Reddy Stokes P1b/P1 {N}x{N} (T3 + bubble velocity, T3 pressure), 3-pt rule, CSR",
+                kernel="k_fam_reddy<T3B, T3, 3 points>",
+                kernel_in=E * 24 + Nn * 16 + Nn * 12 + E * 8, per_el_out=(4 * 16 + 4 * 12) * 16,
+                mesh_bytes=E * 24 + Nn * 16 + Nn * 12 + E * 8, heat=False)
+
+
+def family_upload(ctx, L, cs):
+    if "dn" in cs:
+        ctx.set_mesh(0, cs["conn"], cs["x"])
+        ctx.set_dofs(0, cs["dn"], cs["n"])
+        return lambda wait=True: ctx.assemble(L.HEAT, L.T4, 1, layout=L.CSR, kappa=1.0, Q=-6.0, wait=wait)
+    ctx.set_mesh(0, cs["conn"], cs["x"])
+    ctx.set_dofs(0, np.stack([cs["dux"], cs["duy"]], 1), cs["n"])
+    ctx.set_element_dofs(0, np.stack([cs["eux"], cs["euy"]], 1), cs["n"])
+    ctx.set_mesh(1, cs["conn"], cs["x"])
+    ctx.set_dofs(1, cs["dp"], cs["n"])
+    return lambda wait=True: ctx.assemble(L.STOKES_REDDY, L.T3B, 3, layout=L.CSR, mu=1.0, fe_p=L.T3, wait=wait)
+
+
+def family_oracle(O, cs):
+    if "dn" in cs:
+        I, J, V, FI, FV = O.fam_scalar_coo(O.HEAT, O.T4, 1, cs["conn"], cs["x"], cs["dn"], kappa=1.0, Q=-6.0)
+        O.sparse(I, J, V, cs["n"], cs["n"], csc=False)
+        O.vec_assemble(FI, FV, cs["n"])
+        return
+    I, J, V = O.fam_reddy_coo(O.T3B, O.T3, 3, cs["conn"], cs["x"], cs["dux"], cs["duy"], mu=1.0, de_ux=cs["eux"],
+                              de_uy=cs["euy"], conn_p=cs["conn"], xy_p=cs["x"], dn_p=cs["dp"])
+    O.sparse(I, J, V, cs["n"], cs["n"], csc=False)
+
+
+def bench_family(args, L):
+    ctx = L.Context(0)
+    devname, ncu = ctx.device_name()
+    t0 = time.time()
+    cs = family_case(args.config, args.N)
+    assemble = family_upload(ctx, L, cs)
+    t_setup = time.time() - t0
+    progress(f"{args.config}: mesh, numbering and upload ready ({t_setup:.1f} s)")
+    E, Nn, n = cs["E"], cs["Nn"], cs["n"]
+    nnz = assemble()
+    ctx.synchronize()
+    ramp = clock_ramp(lambda: assemble(False), ctx.synchronize)
+    for _ in range(args.warmup):
+        assemble(False)
+    ctx.synchronize()
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        assemble(False)
+    ctx.synchronize()
+    ms = (time.perf_counter() - w0) * 1e3 / args.steps
+    ctx.timing(True)
+    for _ in range(args.steps):
+        assemble(False)
+    ctx.synchronize()
+    ph = ctx.timing_read()
+    ctx.timing(False)
+    progress("fp64 plain timed")
+
+    def per(name):
+        return (ph[name][0] / ph[name][1]) if ph[name][1] else None
+    elem_ms = per("element")
+    kernel_bytes = cs["kernel_in"] + E * cs["per_el_out"]
+    achieved = kernel_bytes / (elem_ms * 1e-3) / 1e9
+    full_bytes = cs["mesh_bytes"] + nnz * 12 + (n + 1) * 4 + (n * 8 if cs["heat"] else 0)
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            O = _oracle()
+            scs = family_case(args.config, 32 if args.config == "t4" else 128)
+            t1 = time.perf_counter()
+            family_oracle(O, scs)
+            one = time.perf_counter() - t1
+            reps = max(1, min(1000, int(round(CPU_SAMPLE_S / max(one, 1e-6)))))
+            t1 = time.perf_counter()
+            for _ in range(reps):
+                family_oracle(O, scs)
+            each = (time.perf_counter() - t1) / reps
+            cpu = {"value": scs["E"] / each, "unit": "elements/s", "cores": 1, "kind": "port",
+                   "sample": f"{scs['workload']} ({scs['E']} elements), {reps} repetitions ({reps * each:.1f} s): "
+                             f"element loop + sparse() {each:.4f} s each, single thread"}
+        except Exception as e:  # the checker is optional for the GPU numbers
+            cpu = {"error": f"oracle unavailable: {e}"}
+    line = {
+        "metric": METRIC,
+        "value": E / (ms * 1e-3),
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic structured mesh, inputs resident in HBM",
+        "config": {"workload": cs["workload"], "elements": E, "nodes": Nn, "dofs": n, "nnz": nnz, "layout": "CSR",
+                   "method": "plain FP64 (element kernel -> COO in stream order -> radix sort -> segmented reduce)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": cs["kernel"],
+                     "kernel_ms": elem_ms, "algorithmic_bytes": kernel_bytes},
+        "cpu_baseline": cpu,
+        "phases_ms": {k: per(k) for k in ("element", "sort", "reduce", "total")},
+        "full_assembly_bytes": full_bytes,
+        "full_assembly_gbs": full_bytes / (ms * 1e-3) / 1e9,
+        "clock_ramp_steps": ramp,
+        "setup": {"host_mesh_numbering_upload_s": t_setup},
+        "device": {"name": devname, "cus": ncu},
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -579,6 +717,8 @@ def main():
     from elfel_hip import workloads as W
     if args.config == "c4":
         bench_batch(args, L, W)
+    elif args.config in ("t4", "p1b"):
+        bench_family(args, L)
     else:
         bench_single(args, L, W)
 
