@@ -198,7 +198,8 @@ extern "C" int elfel_hip_free(elfel_hip_ctx *c)
                    &c->hist, &c->scan_tmp, &c->flags, &c->scal, &c->fkeys[0], &c->fkeys[1], &c->fvals[0],
                    &c->fvals[1], &c->stage64, &c->ptr, &c->ind, &c->val, &c->F, &c->dscal, &c->pcg, &c->pcg_sell,
                    &c->pat.ptr, &c->pat.ind, &c->pat.tile, &c->pat.tile_rows, &c->pat.task_elem,
-                   &c->pat.task_info, &c->pat.task_pos, &c->pat.tile_elem, &c->pat.conn_pad, &c->pat.xy_pad};
+                   &c->pat.task_info, &c->pat.task_pos, &c->pat.tile_elem, &c->pat.conn_pad, &c->pat.xy_pad,
+                   &c->pat.truns, &c->pat.runs};
     for (Buf *b : bufs) release(*b);
     batch_free(c);
     for (auto &p : c->tm.pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }

@@ -90,6 +90,8 @@ struct Pattern {
     Buf task_pos;                // uint8 [ntasks][ncolmax]
     Buf tile_elem;               // int32 [ntiles][slots]: distinct elements of a tile (heat)
     Buf conn_pad, xy_pad;        // field 0's connectivity and coordinates, padded (CONN_GAP, XY_GAP)
+    Buf truns, runs;             // patch tiles: per-tile run ranges and the runs (empty: contiguous tiles)
+    bool has_runs = false;
     int maxu = 0;                // most distinct elements in one tile
 };
 

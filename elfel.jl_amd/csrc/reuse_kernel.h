@@ -56,6 +56,12 @@ struct ReuseArgs {
     int ngap, xgap;
     double *vals;
     double *F;
+    // Patch tiles (element-staged heat): a tile's rows are not one dof range
+    // but runs of consecutive rows, listed per tile (tile_runs: first run,
+    // run count).  A run is {first value, first row, LDS value start | values
+    // << 16, tile-local row start | rows << 16}.  Null: contiguous tiles.
+    const int2 *tile_runs = nullptr;
+    const int4 *runs = nullptr;
 };
 
 // The owner's entries of one element block.  The owner node a and its
@@ -569,6 +575,33 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
     // range goes out alone.  Non-temporal stores stream the values past the
     // caches -- nothing in this launch reads them again -- and cut the kernel
     // time by a quarter on MI355X.
+    if (A.runs) {
+        // patch tile: the LDS values are the tile's rows in dof order; each run
+        // of consecutive rows goes out as one contiguous range, a wave per run
+        const int2 rp = A.tile_runs[tile];
+        const int wave = tid >> 6, lane = tid & 63;
+        for (int k = wave; k < rp.y; k += BLOCK / 64) {
+            const int4 run = A.runs[rp.x + k];
+            const int lv0 = run.z & 0xFFFF, nv = run.z >> 16;
+            double *rd = A.vals + run.x;
+            const int rhead = run.x & 1;
+            if (lane == 0) {
+                if (rhead && nv > 0) __builtin_nontemporal_store(acc[lds_slot(lv0)], &rd[0]);
+                if (((nv - rhead) & 1) && nv > rhead) __builtin_nontemporal_store(acc[lds_slot(lv0 + nv - 1)], &rd[nv - 1]);
+            }
+            const int np = (nv - rhead) >> 1;
+            for (int p = lane; p < np; p += 64) {
+                const int v = rhead + 2 * p;
+                __builtin_nontemporal_store(acc[lds_slot(lv0 + v)], &rd[v]);
+                __builtin_nontemporal_store(acc[lds_slot(lv0 + v + 1)], &rd[v + 1]);
+            }
+            if constexpr (KIND == ELFEL_HEAT) {
+                const int lr0 = run.w & 0xFFFF, nr = run.w >> 16;
+                for (int i = lane; i < nr; i += 64) A.F[run.y + i] = accF[lr0 + i];
+            }
+        }
+        return;
+    }
     double *dst = A.vals + td.z;
     const int head = td.z & 1;
     const int npair = (td.w - head) >> 1;
@@ -640,6 +673,8 @@ __global__ __launch_bounds__(256) void k_gather_geom(int64_t E, const int32_t *_
 // Host-side task tables of the reuse method (reuse.hip).
 struct ReuseTables {
     std::vector<int4> tiles, trows;
+    std::vector<int2> truns;     // patch tiles: per tile {first run, run count}
+    std::vector<int4> runs;      // patch tiles: runs of consecutive rows (ReuseArgs::runs)
     std::vector<int32_t> elem;
     std::vector<uint32_t> info;
     std::vector<uint8_t> pos;
@@ -650,7 +685,7 @@ int pattern_to_host(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable 
                     std::vector<int32_t> &hptr, std::vector<int32_t> &hind);
 int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1, int64_t n,
                 const std::vector<int32_t> &hptr, const std::vector<int32_t> &hind, int cap, int ucap,
-                ReuseTables &T);
+                ReuseTables &T, bool patch = false);
 int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P);
 // Sorts the tiles by their first element and lays them out so that workgroup b
 // (dispatched to XCD b mod nxcd) walks a contiguous chunk of the sorted list
