@@ -279,10 +279,13 @@ def clock_ramp(launch, sync):
 
 CPU_SAMPLE_S = 10.0  # seconds of single-thread CPU work in the timed CPU baseline sample
 
-TRAFFIC_FILES = {"c2": "k_reuse_elasticity_t6_400x400_traffic.json",
+TRAFFIC_FILES = {"c1": "k_reuse_heat_t3_100x100_traffic.json",
+                 "c2": "k_reuse_elasticity_t6_400x400_traffic.json",
                  "c3": "k_reuse_stokes_t6t3_500x500_traffic.json",
                  "c4": "k_reuse_batch_256_problems_traffic.json",
-                 "c5": "k_reuse_heat_q4_4096x4096_and_pcg_traffic.json"}
+                 "c5": "k_reuse_heat_q4_4096x4096_and_pcg_traffic.json",
+                 "t4": "k_fam_heat_t4_64cubed_traffic.json",
+                 "p1b": "k_fam_reddy_p1b_p1_512x512_traffic.json"}
 
 
 def measured_traffic(cfgname, N_arg, N, block=None):
@@ -691,9 +694,10 @@ def bench_family(args, L):
         "data": "synthetic structured mesh, inputs resident in HBM",
         "config": {"workload": cs["workload"], "elements": E, "nodes": Nn, "dofs": n, "nnz": nnz, "layout": "CSR",
                    "method": "plain FP64 (element kernel -> COO in stream order -> radix sort -> segmented reduce)"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": cs["kernel"],
-                     "kernel_ms": elem_ms, "algorithmic_bytes": kernel_bytes},
+        "roofline": dict({"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": cs["kernel"],
+                          "kernel_ms": elem_ms, "algorithmic_bytes": kernel_bytes},
+                         **measured_traffic(args.config, args.N, cs["N"])),
         "cpu_baseline": cpu,
         "phases_ms": {k: per(k) for k in ("element", "sort", "reduce", "total")},
         "full_assembly_bytes": full_bytes,
