@@ -196,7 +196,7 @@ extern "C" int elfel_hip_free(elfel_hip_ctx *c)
     for (auto &f : c->f) { release(f.conn); release(f.xy); release(f.dof); release(f.edof); }
     Buf *bufs[] = {&c->kappa_e, &c->Q_e, &c->scale_e, &c->geom, &c->keys[0], &c->keys[1], &c->vals[0], &c->vals[1],
                    &c->hist, &c->scan_tmp, &c->flags, &c->scal, &c->fkeys[0], &c->fkeys[1], &c->fvals[0],
-                   &c->fvals[1], &c->stage64, &c->ptr, &c->ind, &c->val, &c->F, &c->dscal, &c->pcg, &c->pcg_sell,
+                   &c->fvals[1], &c->stage64, &c->segpos, &c->ptr, &c->ind, &c->val, &c->F, &c->dscal, &c->pcg, &c->pcg_sell,
                    &c->pat.ptr, &c->pat.ind, &c->pat.tile, &c->pat.tile_rows, &c->pat.task_elem,
                    &c->pat.task_info, &c->pat.task_pos, &c->pat.tile_elem, &c->pat.conn_pad, &c->pat.xy_pad,
                    &c->pat.truns, &c->pat.runs};

@@ -118,6 +118,7 @@ struct elfel_hip_ctx {
     Buf hist, scan_tmp, flags, scal;
     Buf fkeys[2], fvals[2];
     Buf stage64;                  // int64 staging for copy-out
+    Buf segpos;                   // segment head positions of the sorted COO stream (+ the count)
     // result
     Buf ptr, ind, val, F;
     int64_t nmaj = 0, nmin = 0, nnz = -1, nrow = 0, ncol = 0, nF = 0;

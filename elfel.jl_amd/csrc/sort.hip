@@ -12,7 +12,11 @@
 //   scan         exclusive scan of the digit-major [256][ntiles] table
 //   k_rs_scatter stable rank: per round of 256 keys, lanes holding the same
 //                digit are matched with 8 ballots; ranks = popcount of lower
-//                peers + counts of earlier waves + earlier rounds of the tile.
+//                peers + counts of earlier waves + earlier rounds of the tile;
+//                the tile is staged in LDS in digit order and written back
+//                one digit range after another.
+#include <algorithm>
+
 #include "common.h"
 
 #define RS_BLOCK 256
@@ -137,26 +141,44 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint64_t *__restrict
     hist[(int64_t)tid * ntiles + blockIdx.x] = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
 }
 
+// Stable scatter of one tile.  Ranks come round by round as before (wave64
+// ballots within a round, earlier waves and rounds of the tile through LDS
+// counts); the tile is then laid out in LDS in digit order and written back
+// bucket by bucket, so consecutive lanes store consecutive addresses of one
+// digit's range instead of 8 bytes into each of up to 256 buckets per round.
+// Every key lands where the unstaged scatter put it: the order is the same.
 __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint64_t *__restrict__ kin,
                                                          const uint64_t *__restrict__ vin,
                                                          uint64_t *__restrict__ kout,
                                                          uint64_t *__restrict__ vout, int64_t n, int shift,
                                                          const uint32_t *__restrict__ hist, int ntiles)
 {
-    __shared__ uint32_t base[RS_RADIX];
+    extern __shared__ uint64_t stage[];  // [RS_TILE] keys, then [RS_TILE] values
+    uint64_t *sk = stage, *sv = stage + RS_TILE;
+    __shared__ uint32_t gbase[RS_RADIX];  // this tile's first global slot of each digit
+    __shared__ uint32_t run[RS_RADIX];    // keys of each digit met so far, then the tile-local starts
     __shared__ uint32_t wc[4][RS_RADIX];
+    __shared__ uint32_t wsum[RS_BLOCK / 64];
     const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-    base[tid] = hist[(int64_t)tid * ntiles + blockIdx.x];
+    gbase[tid] = hist[(int64_t)tid * ntiles + blockIdx.x];
+    run[tid] = 0;
     for (int ww = 0; ww < 4; ww++) wc[ww][tid] = 0;
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int64_t t0 = (int64_t)blockIdx.x * RS_TILE;
+    uint64_t kr[RS_ITEMS], vr[RS_ITEMS];
+    uint32_t lr[RS_ITEMS];
+#pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
         const int64_t idx = t0 + r * RS_BLOCK + tid;
         const bool valid = idx < n;
-        uint64_t k = 0, v = 0;
-        if (valid) { k = kin[idx]; v = vin[idx]; }
-        const uint32_t d = (uint32_t)(k >> shift) & (RS_RADIX - 1);
+        kr[r] = 0; vr[r] = 0;
+        if (valid) { kr[r] = kin[idx]; vr[r] = vin[idx]; }
+    }
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const bool valid = t0 + r * RS_BLOCK + tid < n;
+        const uint32_t d = (uint32_t)(kr[r] >> shift) & (RS_RADIX - 1);
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; b++) {
@@ -169,20 +191,43 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint64_t *__restr
         __syncthreads();
         uint32_t acc = 0;
         for (int ww = 0; ww < 4; ww++) {
-            uint32_t cnt = wc[ww][tid];
+            const uint32_t cnt = wc[ww][tid];
             wc[ww][tid] = acc;
             acc += cnt;
         }
         __syncthreads();
-        if (valid) {
-            uint32_t dst = base[d] + wc[w][d] + rank;
-            kout[dst] = k;
-            vout[dst] = v;
-        }
+        lr[r] = valid ? run[d] + wc[w][d] + rank : 0u;
         __syncthreads();
-        base[tid] += acc;
+        run[tid] += acc;
         for (int ww = 0; ww < 4; ww++) wc[ww][tid] = 0;
         __syncthreads();
+    }
+    // tile-local start of each digit: exclusive scan of the counts
+    const uint32_t total = run[tid];
+    const uint32_t inc = wave_incl_scan(total, lane);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (int k = 0; k < w; k++) woff += wsum[k];
+    run[tid] = woff + inc - total;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        if (t0 + r * RS_BLOCK + tid < n) {
+            const uint32_t d = (uint32_t)(kr[r] >> shift) & (RS_RADIX - 1);
+            const uint32_t p = run[d] + lr[r];
+            sk[p] = kr[r];
+            sv[p] = vr[r];
+        }
+    }
+    __syncthreads();
+    const int m = (int)((n - t0) < RS_TILE ? (n - t0) : RS_TILE);
+    for (int i = tid; i < m; i += RS_BLOCK) {
+        const uint64_t k = sk[i];
+        const uint32_t d = (uint32_t)(k >> shift) & (RS_RADIX - 1);
+        const uint32_t dst = gbase[d] + (uint32_t)i - run[d];
+        kout[dst] = k;
+        vout[dst] = sv[i];
     }
 }
 
@@ -201,8 +246,8 @@ int radix_sort_pairs(elfel_hip_ctx *c, uint64_t *&keys, uint64_t *&vals, uint64_
         HIPCHK(c, hipGetLastError());
         rc = exclusive_scan_u32(c, hist, nh, nullptr);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_BLOCK), 0, c->stream, keys, vals, keys_alt,
-                           vals_alt, n, shift, hist, ntiles);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_BLOCK), 2 * RS_TILE * sizeof(uint64_t), c->stream,
+                           keys, vals, keys_alt, vals_alt, n, shift, hist, ntiles);
         HIPCHK(c, hipGetLastError());
         uint64_t *t = keys; keys = keys_alt; keys_alt = t;
         t = vals; vals = vals_alt; vals_alt = t;
@@ -219,35 +264,46 @@ __global__ void k_mark_heads(const uint64_t *__restrict__ keys, int64_t n, uint3
     if (i < n) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
 }
 
-// For each segment head i (segment index s = seg[i]): sum the segment left to
-// right, write column/value, and fill ptr for the majors ending at this head.
-__global__ void k_seg_reduce(const uint64_t *__restrict__ keys, const double *__restrict__ vals, int64_t n,
-                             const uint32_t *__restrict__ seg, int minor_bits, int64_t nmaj,
-                             int32_t *__restrict__ ptr, int32_t *__restrict__ ind, double *__restrict__ val,
-                             int32_t *__restrict__ nnz_dev, int with_values)
+// Head positions: hpos[seg[i]] = i for every segment head i, and the segment
+// count (seg is the exclusive scan of the head flags).
+__global__ void k_head_pos(const uint64_t *__restrict__ keys, int64_t n, const uint32_t *__restrict__ seg,
+                           uint32_t *__restrict__ hpos, uint32_t *__restrict__ nseg)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint64_t k = keys[i];
-    const bool head = (i == 0) || (keys[i - 1] != k);
+    const bool head = (i == 0) || (keys[i - 1] != keys[i]);
+    if (head) hpos[seg[i]] = (uint32_t)i;
+    if (i == n - 1) *nseg = seg[i] + (head ? 1u : 0u);
+}
+
+// One work-item per segment s (grid-stride): the segment's entries summed left
+// to right from its head, its column, and ptr for the majors that end at it.
+__global__ __launch_bounds__(256) void k_seg_sum(const uint64_t *__restrict__ keys, const double *__restrict__ vals,
+                                                 int64_t n, const uint32_t *__restrict__ hpos,
+                                                 const uint32_t *__restrict__ nseg_dev, int minor_bits,
+                                                 int64_t nmaj, int32_t *__restrict__ ptr, int32_t *__restrict__ ind,
+                                                 double *__restrict__ val, int32_t *__restrict__ nnz_dev,
+                                                 int with_values)
+{
+    const int64_t nseg = *nseg_dev;
     const uint64_t mmask = (minor_bits >= 64) ? ~0ull : ((1ull << minor_bits) - 1);
-    if (head) {
-        const uint32_t s = seg[i];
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i0 = hpos[s];
+        const int64_t i1 = s + 1 < nseg ? (int64_t)hpos[s + 1] : n;
+        const uint64_t k = keys[i0];
         if (with_values) {
-            double acc = vals[i];
-            for (int64_t j = i + 1; j < n && keys[j] == k; j++) acc += vals[j];
+            double acc = vals[i0];
+            for (int64_t j = i0 + 1; j < i1; j++) acc += vals[j];
             val[s] = acc;
         }
         ind[s] = (int32_t)(k & mmask);
         const int64_t m = (int64_t)(k >> minor_bits);
-        const int64_t mp = (i == 0) ? -1 : (int64_t)(keys[i - 1] >> minor_bits);
+        const int64_t mp = (i0 == 0) ? -1 : (int64_t)(keys[i0 - 1] >> minor_bits);
         for (int64_t r = mp + 1; r <= m; r++) ptr[r] = (int32_t)s;
-    }
-    if (i == n - 1) {
-        const int32_t nnz = (int32_t)(seg[i] + (head ? 1u : 0u));
-        const int64_t m = (int64_t)(k >> minor_bits);
-        for (int64_t r = m + 1; r <= nmaj; r++) ptr[r] = nnz;
-        *nnz_dev = nnz;
+        if (s == nseg - 1) {
+            for (int64_t r = m + 1; r <= nmaj; r++) ptr[r] = (int32_t)nseg;
+            *nnz_dev = (int32_t)nseg;
+        }
     }
 }
 
@@ -281,8 +337,16 @@ int build_csr_from_sorted(elfel_hip_ctx *c, const uint64_t *keys, const double *
     HIPCHK(c, hipGetLastError());
     rc = exclusive_scan_u32(c, seg, n, nullptr);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_seg_reduce, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, keys, vals, n, seg,
-                       minor_bits, nmaj, ptr, ind, val, nnz_dev, values ? 1 : 0);
+    // one work-item per segment over the compacted head positions (the
+    // per-entry form idled the lanes of every entry that does not start a
+    // segment while the head lanes summed)
+    if ((rc = ensure(c, c->segpos, sizeof(uint32_t) * (n + 1)))) return rc;
+    uint32_t *hpos = (uint32_t *)c->segpos.p, *nseg = hpos + n;
+    hipLaunchKernelGGL(k_head_pos, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, keys, n, seg, hpos, nseg);
+    HIPCHK(c, hipGetLastError());
+    const int grid = (int)std::min<int64_t>(ceil_div(n, 256), 8192);
+    hipLaunchKernelGGL(k_seg_sum, dim3(grid), dim3(256), 0, c->stream, keys, vals, n, hpos, nseg, minor_bits,
+                       nmaj, ptr, ind, val, nnz_dev, values ? 1 : 0);
     HIPCHK(c, hipGetLastError());
     return 0;
 }
