@@ -165,3 +165,68 @@ def test_workloads_deterministic_and_in_range():
     specs = W.batch_specs()
     E = np.array([2 * n * n for _, n in specs])
     assert len(specs) == 256 and E.min() >= 800 and E.max() <= 2.1e5
+
+
+# ---- element families in the mirror ---------------------------------------------------
+def test_family_generators_match_oracle():
+    for args in ((1.0, 2.0, 0.5, 3, 2, 4), (1.0, 1.0, 1.0, 5, 5, 5)):
+        m = eh.T4block(*args)
+        c, x = O.t4block(*args)
+        assert np.array_equal(m.conn, c) and np.array_equal(m.xy, x)
+    m = eh.L2block(2.0, 9)
+    c, x = O.l2block(2.0, 9)
+    assert np.array_equal(m.conn, c) and np.array_equal(m.xy, x)
+    # boundary facets: the cube's surface nodes, the segment's end points
+    b = eh.T4block(1.0, 1.0, 1.0, 4, 3, 2).boundary_nodes()
+    assert b.size == 5 * 4 * 3 - 3 * 2 * 1
+    assert np.array_equal(eh.L2block(1.0, 7).boundary_nodes(), [1, 8])
+
+
+def test_family_basis_mirror_matches_oracle():
+    rng = np.random.default_rng(4)
+    for fe, code in ((eh.FEH1_T3_BUBBLE(), O.T3B), (eh.FEH1_T4(), O.T4), (eh.FEH1_L2(), O.L2), (eh.FEL2_Q4(), O.Q4L2)):
+        for _ in range(4):
+            pc = rng.uniform(0.05, 0.3, fe.mdim)
+            assert np.array_equal(fe.bfun(pc), O.fam_bfun(code, pc))
+            assert np.array_equal(fe.bfungradpar(pc), O.fam_bfungradpar(code, pc))
+    for m in range(4):  # FEData of the constructors (src/FElements.jl:339,372,400)
+        assert eh.FEH1_T3_BUBBLE().ndofperfeat(m) == (1, 0, 1, 0)[m]
+        assert eh.FEL2_Q4().ndofperfeat(m) == (0, 0, 1, 0)[m]
+        assert eh.FEH1_T4().nfeatofdim(m) == (4, 6, 4, 1)[m]
+
+
+def test_cell_dof_numbering():
+    """numberdofs!([Ux, Uy, P]) with bubble and cell-constant fields: free dofs
+    of each space (vertices, then cells) from one past the previous space's
+    highest free number, then the data dofs likewise (src/FESpaces.jl:141-173,
+    250-259); element dofs are the nodes' then the cell's."""
+    mesh = eh.T3block(1.0, 1.0, 3, 2)
+    ux, uy = eh.FESpace(mesh, eh.FEH1_T3_BUBBLE()), eh.FESpace(mesh, eh.FEH1_T3_BUBBLE())
+    p = eh.FESpace(mesh, eh.FEH1_T3())
+    ux.setebc(0, [1, 2], 1, 0.5)
+    uy.setebc(2, [3], 1, -1.0)
+    p.setebc(0, 1, 1, 0.0)
+    eh.numberdofs([ux, uy, p])
+    Nn, E = mesh.nnodes, mesh.nelements
+    nf_ux, nf_uy, nf_p = Nn - 2 + E, Nn + E - 1, Nn - 1
+    assert ux.nunknowns() == nf_ux and uy.nunknowns() == nf_uy and p.nunknowns() == nf_p
+    # free numbers: ux nodes, ux cells, uy nodes, uy cells, p nodes; then the data dofs
+    assert ux.dofnums[2, 0] == 1 and ux.cell_dofnums[0, 0] == Nn - 2 + 1
+    assert uy.dofnums[0, 0] == nf_ux + 1 and uy.cell_dofnums[0, 0] == nf_ux + Nn + 1
+    assert p.dofnums[1, 0] == nf_ux + nf_uy + 1
+    nu = nf_ux + nf_uy + nf_p
+    assert ux.dofnums[0, 0] == nu + 1 and ux.dofnums[1, 0] == nu + 2
+    assert uy.cell_dofnums[2, 0] == nu + 3 and p.dofnums[0, 0] == nu + 4
+    assert sorted(np.concatenate([s.element_dofs().ravel() for s in (ux, uy, p)]).tolist())[-1] == eh.ndofs([ux, uy, p])
+    ed = ux.element_dofs()
+    assert ed.shape == (E, 4) and np.array_equal(ed[:, :3], ux.dofnums[mesh.conn - 1, 0])
+    assert np.array_equal(ed[:, 3], ux.cell_dofnums[:, 0])
+    U = np.zeros(eh.ndofs([ux, uy, p]))
+    eh.gathersysvec(U, [ux, uy, p])
+    assert U[nu] == 0.5 and U[nu + 2] == -1.0
+    q = eh.FESpace(eh.Q4block(1.0, 1.0, 2, 2), eh.FEL2_Q4())
+    q.setebc(2, 4, 1, 0.0)
+    eh.numberdofs(q)
+    assert q.ndofs() == 4 and q.dofnums.shape[0] == 0 and q.cell_dofnums[3, 0] == 4
+    with pytest.raises(ValueError):
+        q.setebc(0, 1, 1, 0.0)
