@@ -167,7 +167,8 @@ def finish(a):
     return a.finish()
 
 
-# Names used by the north-star API description; same functions.
+# Aliases of start / finish under the names assemble_init, makematrix and
+# makevector; the same functions.
 assemble_init = start
 makematrix = finish
 makevector = finish

@@ -3,7 +3,7 @@
 FP64 reuse repeats the reference's arithmetic and summation order: its result
 must equal the oracle's bit for bit (checked with tolerance 0).  The mixed
 method (FP32 element arithmetic, FP64 sums) must stay within 1e-6 relative in
-the Frobenius norm (the north-star gate), and its largest entrywise error is
+the Frobenius norm (the accuracy gate of the mixed path), and its largest entrywise error is
 reported.
 """
 import numpy as np
