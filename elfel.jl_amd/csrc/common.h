@@ -178,8 +178,9 @@ int plain_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
 // COO buffers for n matrix entries and nv vector entries (plain method), and
 // the sort + reduce that turns the keys written into keys[0]/vals[0] (and
 // fkeys[0]/fvals[0]) into the CSR/CSC result and the load vector.
-int plain_buffers(elfel_hip_ctx *c, int64_t n, int64_t nv, int64_t ndofs);
-int plain_finish(elfel_hip_ctx *c, int layout, int64_t n, int64_t nv, int64_t ndofs, int minor_bits);
+// vec: the problem has a load vector (nv pairs, possibly none).
+int plain_buffers(elfel_hip_ctx *c, int64_t n, int64_t nv, int64_t ndofs, bool vec);
+int plain_finish(elfel_hip_ctx *c, int layout, int64_t n, int64_t nv, int64_t ndofs, int minor_bits, bool vec);
 
 // ---- families.hip (element families, plain FP64 method) -------------------------
 bool family_problem(const elfel_hip_problem *pb);

@@ -544,7 +544,7 @@ int family_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb)
     const int64_t ndofs = c->nrow;
     const int minor_bits = bits_for(ndofs);
     const int csc = pb->layout == ELFEL_CSC ? 1 : 0;
-    int rc = plain_buffers(c, n, nv, ndofs);
+    int rc = plain_buffers(c, n, nv, ndofs, heat);
     if (rc) return rc;
     const int nq = tab.nq;
     rc = 1;  // stays 1 when no kernel instance fits
@@ -577,5 +577,5 @@ int family_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb)
 #undef R
     if (rc == 1) return set_err(c, ELFEL_ENOTSUP, "unsupported (weak form, element, rule)");
     if (rc) return rc;
-    return plain_finish(c, pb->layout, n, nv, ndofs, minor_bits);
+    return plain_finish(c, pb->layout, n, nv, ndofs, minor_bits, heat);
 }

@@ -199,6 +199,7 @@ static int launch_plain(elfel_hip_ctx *c, const QTable &tab, const Coef &cf, int
     if (rc) return rc;
     double2 *gxy = (double2 *)c->geom.p;
     int32_t *gd = (int32_t *)(gxy + (size_t)NEN * E);
+    if (E == 0) return 0;  // no elements: an empty stream
     hipEvent_t ev = nullptr;
     tm_begin(c, ELFEL_PH_GATHER, &ev);
     hipLaunchKernelGGL((k_gather<NEN, NC>), dim3(ceil_div(E, 256)), dim3(256), 0, c->stream, E,
@@ -241,13 +242,13 @@ static plain_fn pick_plain(int kind, int fe, int nq)
     return nullptr;
 }
 
-int plain_buffers(elfel_hip_ctx *c, int64_t n, int64_t nv, int64_t ndofs)
+int plain_buffers(elfel_hip_ctx *c, int64_t n, int64_t nv, int64_t ndofs, bool vec)
 {
     int rc;
     if ((rc = ensure(c, c->keys[0], sizeof(uint64_t) * n)) || (rc = ensure(c, c->keys[1], sizeof(uint64_t) * n)) ||
         (rc = ensure(c, c->vals[0], sizeof(double) * n)) || (rc = ensure(c, c->vals[1], sizeof(double) * n)))
         return rc;
-    if (nv > 0) {
+    if (vec) {
         if ((rc = ensure(c, c->fkeys[0], sizeof(uint64_t) * nv)) || (rc = ensure(c, c->fkeys[1], sizeof(uint64_t) * nv)) ||
             (rc = ensure(c, c->fvals[0], sizeof(double) * nv)) || (rc = ensure(c, c->fvals[1], sizeof(double) * nv)) ||
             (rc = ensure(c, c->F, sizeof(double) * ndofs)))
@@ -259,11 +260,10 @@ int plain_buffers(elfel_hip_ctx *c, int64_t n, int64_t nv, int64_t ndofs)
     return 0;
 }
 
-int plain_finish(elfel_hip_ctx *c, int layout, int64_t n, int64_t nv, int64_t ndofs, int minor_bits)
+int plain_finish(elfel_hip_ctx *c, int layout, int64_t n, int64_t nv, int64_t ndofs, int minor_bits, bool vec)
 {
     uint64_t *k0 = (uint64_t *)c->keys[0].p, *k1 = (uint64_t *)c->keys[1].p;
     uint64_t *v0 = (uint64_t *)c->vals[0].p, *v1 = (uint64_t *)c->vals[1].p;
-    const bool vec = nv > 0;
     uint64_t *fk0 = vec ? (uint64_t *)c->fkeys[0].p : nullptr, *fk1 = vec ? (uint64_t *)c->fkeys[1].p : nullptr;
     uint64_t *fv0 = vec ? (uint64_t *)c->fvals[0].p : nullptr, *fv1 = vec ? (uint64_t *)c->fvals[1].p : nullptr;
     hipEvent_t ev = nullptr;
@@ -308,10 +308,10 @@ int plain_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
     const int minor_bits = bits_for(ndofs);
     const bool heat = pb->kind == ELFEL_HEAT;
     const int64_t nv = heat ? F.E * nd : 0;
-    int rc = plain_buffers(c, n, nv, ndofs);
+    int rc = plain_buffers(c, n, nv, ndofs, heat);
     if (rc) return rc;
     rc = fn(c, tab, cf, minor_bits, csc, (uint64_t *)c->keys[0].p, (double *)c->vals[0].p,
             heat ? (uint64_t *)c->fkeys[0].p : nullptr, heat ? (double *)c->fvals[0].p : nullptr);
     if (rc) return rc;
-    return plain_finish(c, pb->layout, n, nv, ndofs, minor_bits);
+    return plain_finish(c, pb->layout, n, nv, ndofs, minor_bits, heat);
 }

@@ -224,3 +224,16 @@ def test_family_errors(fctx):
         fctx.assemble(L.STOKES_REDDY, L.T3B, 3, fe_p=L.Q4L2)
     assert e.value.code == L.ENOTSUP
     fctx.assemble(L.STOKES_REDDY, L.T3B, 3, fe_p=L.T3)
+
+
+def test_empty_meshes_give_empty_matrix_and_zero_load(fctx):
+    """A mesh without elements: no entries, every pointer 0, and the load
+    vector of a heat assembly all zeros (start! zeros it, src/Assemblers.jl:194-198)."""
+    for fe, conn, x in ((L.L2, np.zeros((0, 2), np.int64), np.linspace(0, 1, 5).reshape(-1, 1)),
+                        (L.Q4, np.zeros((0, 4), np.int64), np.zeros((5, 2)))):
+        fctx.set_mesh(0, conn, x)
+        fctx.set_dofs(0, np.arange(1, 6), 5)
+        nnz = fctx.assemble(L.HEAT, fe, 2, kappa=1.0, Q=3.0)
+        ptr, ind, val = fctx.copy_sparse(base=0)[:3]
+        assert nnz == 0 and ind.size == 0 and not ptr.any()
+        assert np.array_equal(fctx.copy_vector(5), np.zeros(5))
