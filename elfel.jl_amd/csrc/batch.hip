@@ -167,6 +167,7 @@ static ReuseArgs kind_args(KindData &k, int bi, double *vals, double *F)
     A.tiles = (const int4 *)P.tile.p;
     A.tile_rows = (const int4 *)P.tile_rows.p;
     A.task_elem = (const int32_t *)P.task_elem.p;
+    A.task_elem8 = P.elem8 ? (const uint8_t *)P.task_elem.p : nullptr;
     A.tile_elem = (const int32_t *)P.tile_elem.p;
     A.maxu = P.maxu;
     A.task_info = (const uint32_t *)P.task_info.p;

@@ -85,7 +85,8 @@ struct Pattern {
     Buf ptr, ind;                // CSR pattern (int32)
     Buf tile;                    // int4 per tile: task0, ntasks, v0, nvals | (row0 in .w)
     Buf tile_rows;               // int2 per tile: row0, nrows
-    Buf task_elem;               // int32 per task
+    Buf task_elem;               // int32 per task (uint8 for element-staged tiles, elem8)
+    bool elem8 = false;
     Buf task_info;               // uint32 per task
     Buf task_pos;                // uint8 [ntasks][ncolmax]
     Buf tile_elem;               // int32 [ntiles][slots]: distinct elements of a tile (heat)

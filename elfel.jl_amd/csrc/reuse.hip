@@ -558,8 +558,20 @@ int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P)
         HIPCHK(c, hipMemcpy(P.tile.p, T.tiles.data(), sizeof(int4) * T.tiles.size(), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(P.tile_rows.p, T.trows.data(), sizeof(int4) * T.trows.size(), hipMemcpyHostToDevice));
     }
-    if (ntasks) {
+    // staged tiles name a task's element by its index in the tile's list:
+    // one byte per task (the builder keeps at most 256 elements per tile)
+    P.elem8 = !T.telem.empty();
+    if (ntasks && P.elem8) {
+        std::vector<uint8_t> e8((size_t)ntasks);
+        for (int64_t k = 0; k < ntasks; k++) {
+            if (T.elem[k] < 0 || T.elem[k] > 255) return set_err(c, ELFEL_EINVAL, "internal: tile element index above 255");
+            e8[k] = (uint8_t)T.elem[k];
+        }
+        HIPCHK(c, hipMemcpy(P.task_elem.p, e8.data(), (size_t)ntasks, hipMemcpyHostToDevice));
+    } else if (ntasks) {
         HIPCHK(c, hipMemcpy(P.task_elem.p, T.elem.data(), 4 * ntasks, hipMemcpyHostToDevice));
+    }
+    if (ntasks) {
         HIPCHK(c, hipMemcpy(P.task_info.p, T.info.data(), 4 * ntasks, hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(P.task_pos.p, T.pos.data(), T.pos.size(), hipMemcpyHostToDevice));
     }
@@ -660,6 +672,7 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
     A.tiles = (const int4 *)P.tile.p;
     A.tile_rows = (const int4 *)P.tile_rows.p;
     A.task_elem = (const int32_t *)P.task_elem.p;
+    A.task_elem8 = P.elem8 ? (const uint8_t *)P.task_elem.p : nullptr;
     A.tile_elem = (const int32_t *)P.tile_elem.p;
     A.maxu = P.maxu;
     A.task_info = (const uint32_t *)P.task_info.p;

@@ -43,7 +43,7 @@
 struct ReuseArgs {
     const int4 *tiles;        // task0, ntasks, v0, nvals
     const int4 *tile_rows;    // row0, nrows, nranks, 0
-    const int32_t *task_elem;  // element of the task (heat: its index in tile_elem)
+    const int32_t *task_elem;  // element of the task (not read for heat)
     const int32_t *tile_elem;  // heat: distinct elements of each tile, [ntiles][slots]
     int maxu;                  // heat: most distinct elements in a tile
     const uint32_t *task_info;
@@ -62,6 +62,9 @@ struct ReuseArgs {
     // << 16, tile-local row start | rows << 16}.  Null: contiguous tiles.
     const int2 *tile_runs = nullptr;
     const int4 *runs = nullptr;
+    // heat (element-staged tiles): the task's element as its index in the
+    // tile's list (< 256), one byte per task
+    const uint8_t *task_elem8 = nullptr;
 };
 
 // The owner's entries of one element block.  The owner node a and its
@@ -439,7 +442,8 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
 #pragma unroll
     for (int u = 0; u < TPT; u++) {
         task[u] = (int64_t)tile * SLOTS + u * BLOCK + tid;
-        e[u] = A.task_elem[task[u]];
+        if constexpr (KIND == ELFEL_HEAT) e[u] = A.task_elem8[task[u]];
+        else e[u] = A.task_elem[task[u]];
         info[u] = A.task_info[task[u]];
     }
     // column positions and first-contribution flags, prefetched as 32-bit
