@@ -624,8 +624,11 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
 }
 
 // Tasks per thread of the single-problem kernels: heat tasks are light (one
-// row of NEN columns), two per thread overlap their loads.
-__host__ __device__ constexpr int reuse_tpt(int kind) { return kind == ELFEL_HEAT ? 2 : 1; }
+// row of NEN columns), three per thread overlap their loads and make tiles of
+// up to 192 rows, whose elements fewer other tiles share (heat Q4 4096^2 in
+// band tiles: 2 / 3 / 4 tasks per thread 0.72 / 0.62 / 0.66 ms; heat T3 100^2
+// 4.7 / 4.9 / 5.1 us).
+__host__ __device__ constexpr int reuse_tpt(int kind) { return kind == ELFEL_HEAT ? 3 : 1; }
 #define REUSE_MAX_TILE_ROWS 256  // tile-local row numbers of one-row groups are 8-bit
 
 template <int KIND, typename R, int NEN, int NC, int NQ, bool CSC, int BLOCK>
