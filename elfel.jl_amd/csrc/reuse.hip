@@ -288,10 +288,11 @@ int tile_range(const TileInput &I, int64_t r0, int64_t r1, ReuseTables &T, std::
 // Packed greedily in this order, a tile's rows cover a few short stretches of
 // neighbouring grid lines instead of one stretch of one line: each element of
 // a heat Q4 4096^2 mesh is integrated by 1.3 tiles instead of 2, while a tile
-// still writes its values in a few long runs (band height in node rows 2 / 4 /
-// 8 measured 0.78 / 0.74 / 0.82 ms, strips 0.86 ms; Z-order patches, 16 runs
-// per tile, 0.85 ms; storing the coordinates in this order as well changed
-// nothing).  Small meshes keep strips: on heat T3 100^2 (10,201 rows, a
+// still writes its values in a few long runs (two tasks per thread: band
+// height in node rows 2 / 4 / 8 measured 0.78 / 0.74 / 0.82 ms, strips 0.86 ms,
+// Z-order patches with 16 runs per tile 0.85 ms, the coordinates stored in this
+// order as well no change; three tasks per thread: 3 / 4 / 5 / 6 rows 0.64 /
+// 0.62 / 0.62 / 0.64 ms).  Small meshes keep strips: on heat T3 100^2 (10,201 rows, a
 // launch-latency-bound case) bands measured 6.1 against 4.8 us.
 void band_order(const std::vector<double2> &xy, std::vector<int32_t> &order)
 {
