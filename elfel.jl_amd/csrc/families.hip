@@ -433,11 +433,6 @@ bool reddy_pair(int vfe, int pfe)
            (vfe == ELFEL_Q4 && pfe == ELFEL_Q4L2);
 }
 
-int64_t field_ndofs(const Field &f)
-{
-    return std::max(f.has_dofs ? f.ndofs : 0, f.has_edofs ? f.edof_ndofs : 0);
-}
-
 template <int KIND, int M, int NB, int NQ>
 int launch_scalar(elfel_hip_ctx *c, const FamTab &tab, double kappa, double Q, int minor_bits, int csc)
 {
@@ -516,7 +511,11 @@ int family_validate(elfel_hip_ctx *c, const elfel_hip_problem *pb)
         } else if (!P.has_edofs || P.edof_ncomp != 1 || P.edof_E != F.E) {
             return set_err(c, ELFEL_ESTATE, "the FEL2_Q4 pressure needs cell dofs of field 1");
         }
-        ndofs = std::max(field_ndofs(F), field_ndofs(P));
+        // the system size from the dof maps this pair reads (a map left from an
+        // earlier problem on the context does not count)
+        ndofs = F.ndofs;
+        if (fe == ELFEL_T3B) ndofs = std::max(ndofs, F.edof_ndofs);
+        ndofs = std::max(ndofs, pb->fe_p == ELFEL_T3 ? P.ndofs : P.edof_ndofs);
     } else {
         return set_err(c, ELFEL_ENOTSUP, "weak form not available for this element");
     }
