@@ -351,16 +351,20 @@ int build_csr_from_sorted(elfel_hip_ctx *c, const uint64_t *keys, const double *
     return 0;
 }
 
-__global__ void k_dense_reduce(const uint64_t *__restrict__ keys, const double *__restrict__ vals, int64_t n,
-                               double *__restrict__ out)
+// One work-item per segment of equal keys (compacted heads, as for the
+// matrix): the segment summed left to right into out[key].
+__global__ __launch_bounds__(256) void k_dense_sum(const uint64_t *__restrict__ keys, const double *__restrict__ vals,
+                                                   int64_t n, const uint32_t *__restrict__ hpos,
+                                                   const uint32_t *__restrict__ nseg_dev, double *__restrict__ out)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t k = keys[i];
-    if (i > 0 && keys[i - 1] == k) return;
-    double acc = vals[i];
-    for (int64_t j = i + 1; j < n && keys[j] == k; j++) acc += vals[j];
-    out[k] = 0.0 + acc;  // start! zeros the vector, assemble! adds (src/Assemblers.jl:194-221)
+    const int64_t nseg = *nseg_dev;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nseg; s += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i0 = hpos[s];
+        const int64_t i1 = s + 1 < nseg ? (int64_t)hpos[s + 1] : n;
+        double acc = vals[i0];
+        for (int64_t j = i0 + 1; j < i1; j++) acc += vals[j];
+        out[keys[i0]] = 0.0 + acc;  // start! zeros the vector, assemble! adds (src/Assemblers.jl:194-221)
+    }
 }
 
 int build_dense_from_sorted(elfel_hip_ctx *c, const uint64_t *keys, const double *vals, int64_t n,
@@ -369,7 +373,17 @@ int build_dense_from_sorted(elfel_hip_ctx *c, const uint64_t *keys, const double
     hipLaunchKernelGGL(k_fill_f64, dim3(ceil_div(ndofs, 256)), dim3(256), 0, c->stream, out, ndofs, 0.0);
     HIPCHK(c, hipGetLastError());
     if (n == 0) return 0;
-    hipLaunchKernelGGL(k_dense_reduce, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, keys, vals, n, out);
+    int rc;
+    if ((rc = ensure(c, c->flags, sizeof(uint32_t) * n)) || (rc = ensure(c, c->segpos, sizeof(uint32_t) * (n + 1))))
+        return rc;
+    uint32_t *seg = (uint32_t *)c->flags.p, *hpos = (uint32_t *)c->segpos.p, *nseg = hpos + n;
+    hipLaunchKernelGGL(k_mark_heads, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, keys, n, seg);
+    HIPCHK(c, hipGetLastError());
+    if ((rc = exclusive_scan_u32(c, seg, n, nullptr))) return rc;
+    hipLaunchKernelGGL(k_head_pos, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, keys, n, seg, hpos, nseg);
+    HIPCHK(c, hipGetLastError());
+    const int grid = (int)std::min<int64_t>(ceil_div(n, 256), 8192);
+    hipLaunchKernelGGL(k_dense_sum, dim3(grid), dim3(256), 0, c->stream, keys, vals, n, hpos, nseg, out);
     HIPCHK(c, hipGetLastError());
     return 0;
 }
