@@ -203,6 +203,8 @@ extern "C" int elfel_hip_free(elfel_hip_ctx *c)
     for (Buf *b : bufs) release(*b);
     batch_free(c);
     for (auto &p : c->tm.pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+    if (c->tm.r0) hipEventDestroy(c->tm.r0);
+    if (c->tm.r1) hipEventDestroy(c->tm.r1);
     for (auto e : c->tm.pool) hipEventDestroy(e);
     hipStreamDestroy(c->stream);
     delete c;
@@ -533,6 +535,7 @@ extern "C" int elfel_hip_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb,
     c->has_result = false;
     hipEvent_t ev = nullptr;
     tm_begin(c, ELFEL_PH_TOTAL, &ev);
+    if (c->tm.region) c->tm.region_n++;
     if (fam) {
         c->nnz = -1;
         rc = family_assemble(c, pb);
@@ -586,6 +589,18 @@ extern "C" int elfel_hip_timing(elfel_hip_ctx *c, int enable)
 {
     CHECK_CTX(c);
     tm_collect(c);
+    c->tm.region = false;
+    if (enable == 2) {
+        // region mode: no per-phase events, one pair around the whole region
+        c->tm.on = false;
+        for (int i = 0; i < ELFEL_NPHASES; i++) { c->tm.ms[i] = 0; c->tm.n[i] = 0; }
+        if (!c->tm.r0) HIPCHK(c, hipEventCreate(&c->tm.r0));
+        if (!c->tm.r1) HIPCHK(c, hipEventCreate(&c->tm.r1));
+        HIPCHK(c, hipEventRecord(c->tm.r0, c->stream));
+        c->tm.region = true;
+        c->tm.region_n = 0;
+        return ELFEL_OK;
+    }
     c->tm.on = enable != 0;
     for (int i = 0; i < ELFEL_NPHASES; i++) { c->tm.ms[i] = 0; c->tm.n[i] = 0; }
     // events are created here, not inside timed launches
@@ -600,6 +615,18 @@ extern "C" int elfel_hip_timing(elfel_hip_ctx *c, int enable)
 extern "C" int elfel_hip_timing_read(elfel_hip_ctx *c, double *ms, int64_t *counts)
 {
     CHECK_CTX(c);
+    if (c->tm.region) {
+        HIPCHK(c, hipEventRecord(c->tm.r1, c->stream));
+        HIPCHK(c, hipEventSynchronize(c->tm.r1));
+        float t = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&t, c->tm.r0, c->tm.r1));
+        c->tm.region = false;
+        for (int i = 0; i < ELFEL_NPHASES; i++) {
+            if (ms) ms[i] = i == ELFEL_PH_TOTAL ? (double)t : 0.0;
+            if (counts) counts[i] = i == ELFEL_PH_TOTAL ? c->tm.region_n : 0;
+        }
+        return ELFEL_OK;
+    }
     tm_collect(c);
     for (int i = 0; i < ELFEL_NPHASES; i++) {
         if (ms) ms[i] = c->tm.ms[i];

@@ -421,6 +421,7 @@ extern "C" int elfel_hip_batch_assemble(elfel_hip_ctx *c, int wait)
     if (!b || !b->built || !b->exec) return set_err(c, ELFEL_ESTATE, "build the batch first");
     hipEvent_t ev = nullptr;
     tm_begin(c, ELFEL_PH_TOTAL, &ev);
+    if (c->tm.region) c->tm.region_n++;
     HIPCHK(c, hipGraphLaunch(b->exec, c->stream));
     tm_end(c, ELFEL_PH_TOTAL, ev);
     if (wait) HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -98,6 +98,12 @@ struct Pattern {
 
 struct Timing {
     bool on = false;
+    // region mode: one event pair around everything submitted between
+    // elfel_hip_timing(ctx, 2) and elfel_hip_timing_read; region_n counts the
+    // assemblies (single or batched) in between
+    bool region = false;
+    hipEvent_t r0 = nullptr, r1 = nullptr;
+    int64_t region_n = 0;
     double ms[ELFEL_NPHASES] = {0};
     int64_t n[ELFEL_NPHASES] = {0};
     std::vector<hipEvent_t> pool;

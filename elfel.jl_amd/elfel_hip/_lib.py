@@ -427,6 +427,11 @@ class Context:
     def timing(self, enable=True):
         self._chk(self.lib.elfel_hip_timing(self.h, 1 if enable else 0), "timing")
 
+    def timing_region(self):
+        """One hipEvent pair around everything submitted until timing_read()
+        (whose "total" entry then holds the elapsed ms and the assembly count)."""
+        self._chk(self.lib.elfel_hip_timing(self.h, 2), "timing")
+
     def timing_read(self):
         ms = np.zeros(NPHASES)
         n = np.zeros(NPHASES, np.int64)

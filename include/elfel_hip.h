@@ -255,7 +255,11 @@ int elfel_hip_batch_stats(elfel_hip_ctx *ctx, int64_t *nprob, int64_t *nelem, in
                           double *tuned_ms);
 
 /* ---- timing ---------------------------------------------------------------
- * hipEvent timing of each phase on the context's stream, accumulated while
+ * enable = 2 (region mode): one hipEvent pair on the context's stream around
+ * everything submitted until elfel_hip_timing_read, which then returns the
+ * elapsed time in ms[ELFEL_PH_TOTAL] and the number of assemblies (single or
+ * batched) in counts[ELFEL_PH_TOTAL]; no per-phase events in between.
+ * enable = 1: hipEvent timing of each phase on the context's stream, accumulated while
  * enabled.  ms[ELFEL_NPHASES], counts[ELFEL_NPHASES] (launch counts). */
 int elfel_hip_timing(elfel_hip_ctx *ctx, int enable);
 int elfel_hip_timing_read(elfel_hip_ctx *ctx, double *ms, int64_t *counts);
