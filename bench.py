@@ -1,9 +1,13 @@
 #!/usr/bin/env python3
 """Benchmark: element integration + global assembly on one MI355X.
 
-Default workload (--config c2): plane-strain elasticity, 400 x 400 grid of T6
-triangles (320,000 elements, 1,283,202 dofs), 3-point rule, nodes jittered by
-U(-0.15, 0.15) h (seeded SplitMix64), inputs resident in HBM, CSC output.
+Default workload (--config c5), the largest configuration and the one that
+carries the solve: heat on a 4096 x 4096 grid of Q4 quadrilaterals
+(16,777,216 elements, 16,785,409 dofs), 2x2 Gauss, conductivity
+lognormal(0, 1) and source U[-1, 1] per element (seeded SplitMix64), u = 0 on
+the boundary, inputs resident in HBM, CSR output + load vector; then
+Jacobi-PCG on the 16,769,025 free dofs to ||b - A x|| / ||b|| <= 1e-10, FP32
+iterations with FP64 residual replacement against FP64 PCG (the "pcg" object).
 
 A step is one assembly of the whole mesh into the global sparse matrix:
   * mixed      : REUSE method, FP64 geometry, FP32 element products, FP64 sums
@@ -16,12 +20,12 @@ build time is reported separately.
 
 Other workloads, each printing its own line:
   c1  heat T3 100x100, 1-point rule, CSR + load vector;
+  c2  plane-strain elasticity T6 400x400 (320,000 elements, 1,283,202 dofs),
+      3-point rule, nodes jittered by U(-0.15, 0.15) h, CSC;
   c3  Stokes Taylor-Hood T6/T3 500x500, K and B blocks in one launch, CSC;
   c4  256 heat-T3 / elasticity-T6 problems of uneven size in one persistent
       launch replayed from a hipGraph (mixed), against the FP64 plain method
-      problem by problem;
-  c5  heat Q4 4096x4096, 2x2 Gauss, lognormal conductivity, then Jacobi-PCG
-      (FP32 inner iterations + FP64 refinement, and FP64) to 1e-10.
+      problem by problem.
 Element families (the plain FP64 method, the only one they run on):
   t4  heat on a 64^3 block of T4 tetrahedra (examples/heat/poisson/t4.jl);
   p1b Reddy-form Stokes, T3 + bubble velocity / T3 pressure, 512 x 512
@@ -49,7 +53,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", choices=["c1", "c2", "c3", "c4", "c5", "t4", "p1b"], default="c2")
+    ap.add_argument("--config", choices=["c1", "c2", "c3", "c4", "c5", "t4", "p1b"], default="c5")
     ap.add_argument("--N", type=int, default=0, help="grid size (default: the configuration's)")
     ap.add_argument("--fp64-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -299,11 +303,26 @@ TRAFFIC_FILES = {"c1": "k_reuse_heat_t3_100x100_traffic.json",
                  "p1b": "k_fam_reddy_p1b_p1_512x512_traffic.json"}
 
 
+def kernel_source_hash():
+    """SHA-256 of the library's device and host sources (csrc/*): a counter
+    run recorded under one hash says nothing about the kernels of another."""
+    import hashlib
+    d = os.path.join(ROOT, "elfel.jl_amd", "csrc")
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(d)):
+        if name.endswith((".hip", ".h", ".cpp")) or name == "Makefile":
+            with open(os.path.join(d, name), "rb") as fh:
+                h.update(name.encode() + b"\0" + fh.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(cfgname, N_arg, N, block=None):
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE runs of the same workload (scaled by the
-    device-copy calibration), when the workload is the profiled size.  The
-    batched kernel's file holds one entry per workgroup size."""
+    device-copy calibration), when the workload is the profiled size and the
+    counters were taken on the current sources (same source hash); otherwise
+    traffic stays null and the reason is given.  The batched kernel's file
+    holds one entry per workgroup size."""
     f = TRAFFIC_FILES.get(cfgname)
     if not f or N_arg:
         return {}
@@ -312,12 +331,17 @@ def measured_traffic(cfgname, N_arg, N, block=None):
         return {}
     with open(path) as fh:
         t = json.load(fh)
+    src = kernel_source_hash()
+    if t.get("source_hash") != src:
+        return {"traffic_note": f"counters in profiles/{f} were taken on other sources "
+                                f"({t.get('source_hash')} against {src}); not used"}
     if "by_block" in t:
         t = t["by_block"].get(str(block))
         if not t:
             return {}
     return {"traffic": t["traffic_bytes"], "traffic_fetch": t["fetch_bytes"], "traffic_write": t["write_bytes"],
-            "traffic_source": t["source"] + " (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, x2 / x1 copy calibration)"}
+            "traffic_source": t["source"] + " (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, x2 / x1 copy calibration)",
+            "source_hash": src}
 
 
 def device_setup_c5(L, cs):

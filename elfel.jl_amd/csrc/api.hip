@@ -13,6 +13,7 @@ int plain_coo_keys(elfel_hip_ctx *c, const int64_t *I, const int64_t *J, int64_t
                    int minor_bits, int csc, uint64_t *keys);
 int plain_vec_keys(elfel_hip_ctx *c, const int64_t *I, int64_t n, int64_t ndofs, uint64_t *keys);
 int geometry_stats_run(elfel_hip_ctx *c, double *min_det, int64_t *nbad);
+int geometry_check(elfel_hip_ctx *c, int rule, double *min_det, int64_t *nbad, int64_t *nneg);
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -280,6 +281,9 @@ extern "C" int elfel_hip_sparse(elfel_hip_ctx *c, int64_t nrow, int64_t ncol, in
     c->nrow = nrow; c->ncol = ncol; c->nmaj = nmaj; c->nmin = nmin; c->layout = layout;
     c->has_result = true;
     c->result_is_pattern = false;
+    c->used_method = ELFEL_SORT;
+    c->used_precision = ELFEL_FP64;
+    c->used_note.clear();
     if (nnz) *nnz = h;
     return ELFEL_OK;
 }
@@ -395,6 +399,7 @@ extern "C" int elfel_hip_set_mesh_sdim(elfel_hip_ctx *c, int field, int32_t nen,
     int flags = 0;
     if ((rc = take_flags(c, &flags))) return rc;
     if (flags & DERR_RANGE) return set_err(c, ELFEL_ERANGE, "connectivity refers to a node outside 1..Nn");
+    field_mesh_changed(f, E, Nn);
     f.nen = nen; f.E = E; f.Nn = Nn; f.sdim = sdim;
     f.h_conn.resize((size_t)(E * nen));
     for (int64_t i = 0; i < E * nen; i++) f.h_conn[i] = (int32_t)(conn[i] - 1);
@@ -450,8 +455,8 @@ extern "C" int elfel_hip_set_dofs(elfel_hip_ctx *c, int field, int32_t ncomp, in
     if ((rc = take_flags(c, &flags))) return rc;
     if (flags & DERR_RANGE)
         return set_err(c, ELFEL_ERANGE, "dof number outside 1..ndofs (unnumbered dof 0?)");
-    f.ncomp = ncomp; f.ndofs = ndofs;
     if (f.Nn != Nn && f.has_mesh) return set_err(c, ELFEL_EINVAL, "dof map size does not match the mesh");
+    f.ncomp = ncomp; f.ndofs = ndofs; f.dof_Nn = Nn;
     f.h_dof.resize((size_t)n);
     for (int64_t i = 0; i < n; i++) f.h_dof[i] = (int32_t)(dofnums[i] - 1);
     f.has_dofs = true;
@@ -532,7 +537,22 @@ extern "C" int elfel_hip_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb,
     const bool fam = pb && family_problem(pb);
     int rc = fam ? family_validate(c, pb) : validate_problem(c, pb, &tab, &cf);
     if (rc) return rc;
+    if (pb->flags & ~ELFEL_CHECK_GEOM) return set_err(c, ELFEL_EINVAL, "unknown problem flags");
+    if (pb->flags & ELFEL_CHECK_GEOM) {
+        double mn = 0;
+        int64_t nbad = 0, nneg = 0;
+        if ((rc = geometry_check(c, pb->rule, &mn, &nbad, &nneg))) return rc;
+        if (nneg > 0) {
+            char msg[160];
+            snprintf(msg, sizeof msg, "%lld element(s) with det J <= 0 at a quadrature point (smallest %.3e)",
+                     (long long)nneg, mn);
+            return set_err(c, ELFEL_EGEOM, msg);
+        }
+    }
     c->has_result = false;
+    c->used_method = pb->method;
+    c->used_precision = pb->precision;
+    c->used_note.clear();
     hipEvent_t ev = nullptr;
     tm_begin(c, ELFEL_PH_TOTAL, &ev);
     if (c->tm.region) c->tm.region_n++;
@@ -552,6 +572,16 @@ extern "C" int elfel_hip_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb,
         if ((rc = fetch_nnz(c))) return rc;
         *nnz = c->nnz;
     }
+    return ELFEL_OK;
+}
+
+extern "C" int elfel_hip_assembly_info(elfel_hip_ctx *c, int *method, int *precision, char *note, int len)
+{
+    CHECK_CTX(c);
+    if (!c->has_result || c->used_method < 0) return set_err(c, ELFEL_ESTATE, "no assembly has run");
+    if (method) *method = c->used_method;
+    if (precision) *precision = c->used_precision;
+    if (note && len > 0) snprintf(note, (size_t)len, "%s", c->used_note.c_str());
     return ELFEL_OK;
 }
 

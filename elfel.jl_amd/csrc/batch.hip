@@ -44,6 +44,7 @@ struct Batch {
     Buf ptr, ind, val, F;
     int64_t total_nnz = 0, total_ptr = 0, total_rows = 0;
     int block = 256, grid = 0, lds_vals = 0;
+    Buf args[2];  // BatchArgs in device memory, per workgroup size (128, 256)
     double tuned_ms = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -76,6 +77,7 @@ void batch_free(elfel_hip_ctx *c)
         }
     }
     release(b->ptr); release(b->ind); release(b->val); release(b->F);
+    release(b->args[0]); release(b->args[1]);
     delete b;
     c->batch = nullptr;
 }
@@ -183,26 +185,36 @@ static ReuseArgs kind_args(KindData &k, int bi, double *vals, double *F)
     return A;
 }
 
+// The kernel arguments of workgroup size block (index bi), written to device
+// memory once per build.
+static int batch_upload_args(elfel_hip_ctx *c, Batch *b, int bi)
+{
+    KindData &H = b->kd[0], &Ev = b->kd[1];
+    BatchArgs a{};
+    a.ch.kappa = b->pb_heat.kappa; a.ch.Q = b->pb_heat.Q; a.ch.mu = b->pb_heat.mu;
+    a.ch.kappa_e = H.coef.p ? (const double *)H.coef.p : nullptr;
+    a.ce.mu = b->pb_el.mu;
+    for (int i = 0; i < 9; i++) { a.ch.D[i] = b->pb_heat.D[i]; a.ce.D[i] = b->pb_el.D[i]; }
+    a.ce.scale_e = Ev.coef.p ? (const double *)Ev.coef.p : nullptr;
+    a.H = kind_args(H, bi, (double *)b->val.p, (double *)b->F.p);
+    a.Ev = kind_args(Ev, bi, (double *)b->val.p, nullptr);
+    if (!H.E) a.H.ntiles = 0;
+    if (!Ev.E) a.Ev.ntiles = 0;
+    int rc = ensure(c, b->args[bi], sizeof(BatchArgs));
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpy(b->args[bi].p, &a, sizeof(BatchArgs), hipMemcpyHostToDevice));
+    return 0;
+}
+
 // One batched assembly on the stream (no synchronisation).
 static int batch_launch(elfel_hip_ctx *c, Batch *b, int block, int grid)
 {
     const int bi = block == 128 ? 0 : 1;
     KindData &H = b->kd[0], &Ev = b->kd[1];
-    QTable th{}, te{};
-    make_qtable(ELFEL_T3, 1, &th);
-    make_qtable(ELFEL_T6, 3, &te);
-    Coef ch{}, ce{};
-    ch.kappa = b->pb_heat.kappa; ch.Q = b->pb_heat.Q; ch.mu = b->pb_heat.mu;
-    ch.kappa_e = H.coef.p ? (const double *)H.coef.p : nullptr;
-    ce.mu = b->pb_el.mu;
-    for (int i = 0; i < 9; i++) { ch.D[i] = b->pb_heat.D[i]; ce.D[i] = b->pb_el.D[i]; }
-    ce.scale_e = Ev.coef.p ? (const double *)Ev.coef.p : nullptr;
-    ReuseArgs AH = kind_args(H, bi, (double *)b->val.p, (double *)b->F.p);
-    ReuseArgs AE = kind_args(Ev, bi, (double *)b->val.p, nullptr);
-    if (!H.E) AH.ntiles = 0;
-    if (!Ev.E) AE.ntiles = 0;
-    const int nt = AH.ntiles + AE.ntiles;
+    const int nH = H.E ? (int)H.tab[bi].ntiles : 0, nE = Ev.E ? (int)Ev.tab[bi].ntiles : 0;
+    const int nt = nH + nE;
     if (nt == 0) return 0;
+    const BatchArgs *args = (const BatchArgs *)b->args[bi].p;
     int lds_vals = std::max(H.E ? lds_span((int)H.tab[bi].lds_vals) : 0, Ev.E ? lds_span((int)Ev.tab[bi].lds_vals) : 0);
     if (H.E)  // heat-T3 element records (floats) alias the value sums
         lds_vals = std::max(lds_vals, (H.tab[bi].maxu * heat_rec<3>() * 4 + 7) / 8);
@@ -212,11 +224,11 @@ static int batch_launch(elfel_hip_ctx *c, Batch *b, int block, int grid)
     const int g = grid > 0 ? std::min(grid, nt) : nt;
     const bool csc = b->layout == ELFEL_CSC;
     if (block == 128) {
-        if (csc) hipLaunchKernelGGL((k_reuse_batch<true, 128>), dim3(g), dim3(128), lds, c->stream, AH, AE, ch, ce, th, te, lds_vals);
-        else hipLaunchKernelGGL((k_reuse_batch<false, 128>), dim3(g), dim3(128), lds, c->stream, AH, AE, ch, ce, th, te, lds_vals);
+        if (csc) hipLaunchKernelGGL((k_reuse_batch<true, 128>), dim3(g), dim3(128), lds, c->stream, args, nH, nt, lds_vals);
+        else hipLaunchKernelGGL((k_reuse_batch<false, 128>), dim3(g), dim3(128), lds, c->stream, args, nH, nt, lds_vals);
     } else {
-        if (csc) hipLaunchKernelGGL((k_reuse_batch<true, 256>), dim3(g), dim3(256), lds, c->stream, AH, AE, ch, ce, th, te, lds_vals);
-        else hipLaunchKernelGGL((k_reuse_batch<false, 256>), dim3(g), dim3(256), lds, c->stream, AH, AE, ch, ce, th, te, lds_vals);
+        if (csc) hipLaunchKernelGGL((k_reuse_batch<true, 256>), dim3(g), dim3(256), lds, c->stream, args, nH, nt, lds_vals);
+        else hipLaunchKernelGGL((k_reuse_batch<false, 256>), dim3(g), dim3(256), lds, c->stream, args, nH, nt, lds_vals);
     }
     HIPCHK(c, hipGetLastError());
     return 0;
@@ -373,6 +385,8 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
     }
     if (b->total_nnz >= INT32_MAX || b->total_ptr >= INT32_MAX)
         return set_err(c, ELFEL_ENOTSUP, "batch beyond int32 value offsets");
+    for (int bi = 0; bi < 2; bi++)
+        if ((rc = batch_upload_args(c, b, bi))) return rc;
     b->built = true;
     // launch shape: cache or autotune
     hipDeviceProp_t prop;

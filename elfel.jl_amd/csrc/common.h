@@ -59,6 +59,7 @@ struct Field {
     int nen = 0, ncomp = 0;
     int sdim = 2;                // space dimension of the coordinates (xy holds sdim doubles per node)
     int64_t E = 0, Nn = 0, ndofs = 0;
+    int64_t dof_Nn = 0;          // node count of the dof map (set_dofs)
     bool has_mesh = false, has_dofs = false;
     Buf conn, xy, dof;
     // cell dofs [E][edof_ncomp] (bubble, FEL2_Q4), 0-based
@@ -69,6 +70,15 @@ struct Field {
     std::vector<int32_t> h_conn, h_dof;  // host copies (0-based) for the symbolic phase
     uint64_t mesh_version = 0, dof_version = 0;
 };
+
+// A new mesh of a field: a dof map of another node count, or cell dofs of
+// another element count, no longer describe it and must be set again (the
+// kernels would index them out of range).
+inline void field_mesh_changed(Field &f, int64_t E, int64_t Nn)
+{
+    if (f.has_dofs && f.dof_Nn != Nn) f.has_dofs = false;
+    if (f.has_edofs && f.edof_E != E) f.has_edofs = false;
+}
 
 // Pattern cached for the REUSE method (built once per mesh + dof map).
 struct Pattern {
@@ -94,6 +104,11 @@ struct Pattern {
     Buf truns, runs;             // patch tiles: per-tile run ranges and the runs (empty: contiguous tiles)
     bool has_runs = false;
     int maxu = 0;                // most distinct elements in one tile
+    // the mesh does not fit the tiles (a node in more than 16 elements, a row
+    // of more than 128 entries, a dof shared by two nodes, an element that
+    // repeats a dof): assemblies of this pattern run the plain FP64 method
+    bool fallback = false;
+    std::string fallback_note;
 };
 
 struct Timing {
@@ -132,8 +147,11 @@ struct elfel_hip_ctx {
     int layout = 0;
     bool has_result = false, has_F = false;
     bool result_is_pattern = false;   // ptr/ind live in the cached pattern
+    int used_method = -1, used_precision = -1;  // what the last assembly ran (elfel_hip_assembly_info)
+    std::string used_note;
     Pattern pat;
-    // device scalars: [0] nnz (int32), [1] error flags, [3] bad-shape count (int64, at byte 24)
+    // device scalars: [0] nnz (int32), [1] error flags; int64 counts of badly shaped elements (byte 24) and
+    // of elements with det J <= 0 (byte 32)
     Buf dscal;
     Timing tm;
     // solver scratch: vectors, then the SELL-64 copy of the free block

@@ -154,6 +154,7 @@ __global__ void k_mark_boundary(int64_t n, int nen, int ncorner, const int32_t *
 
 int finish_field_mesh(elfel_hip_ctx *c, Field &f, int nen, int64_t E, int64_t Nn)
 {
+    field_mesh_changed(f, E, Nn);
     f.nen = nen;
     f.sdim = 2;
     f.E = E;

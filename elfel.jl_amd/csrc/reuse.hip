@@ -251,6 +251,15 @@ int tile_range(const TileInput &I, int64_t r0, int64_t r1, ReuseTables &T, std::
                     colbuf[nj++] = F0.h_dof[(int64_t)F0.h_conn[(int64_t)e * nen + k] * nc + cc];
             if (stokes && field == 0)
                 for (int k = 0; k < 3; k++) colbuf[nj++] = F1->h_dof[F1->h_conn[(int64_t)e * 3 + k]];
+            // a task reads all of its partial sums before it writes any back,
+            // so its columns must be distinct dofs (an element that repeats a
+            // node or a dof goes to the sort method)
+            for (int j = 1; j < nj; j++)
+                for (int i = 0; i < j; i++)
+                    if (colbuf[i] == colbuf[j]) {
+                        err = "an element with a repeated dof";
+                        return ELFEL_ENOTSUP;
+                    }
             for (int j = 0; j < ncolp; j++) {
                 uint8_t pv = 0;
                 if (j < nj) {
@@ -372,17 +381,24 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     adjacency(F0, A0);
     if (stokes) adjacency(*F1, A1);
     I.A0 = &A0; I.A1 = &A1;
+    // Each row must belong to one (node, component): a dof shared by two
+    // nodes (or two components) gathers contributions from elements of both,
+    // which one row group's element list does not hold.  Such dof maps are
+    // valid input (sparse() sums whatever the COO stream holds) and go to the
+    // sort method instead.
     std::vector<int32_t> inv_node(n, -1);
     std::vector<uint8_t> inv_fc(n, 0);  // field << 4 | comp
     for (int64_t nd0 = 0; nd0 < F0.Nn; nd0++)
         for (int cc = 0; cc < nc; cc++) {
             const int32_t d = F0.h_dof[nd0 * nc + cc];
+            if (inv_node[d] >= 0) return set_err(c, ELFEL_ENOTSUP, "a dof shared by two nodes or components");
             inv_node[d] = (int32_t)nd0;
             inv_fc[d] = (uint8_t)cc;
         }
     if (stokes)
         for (int64_t m = 0; m < F1->Nn; m++) {
             const int32_t d = F1->h_dof[m];
+            if (inv_node[d] >= 0) return set_err(c, ELFEL_ENOTSUP, "a dof shared by two nodes or components");
             inv_node[d] = (int32_t)m;
             inv_fc[d] = (uint8_t)(1 << 4);
         }
@@ -606,18 +622,28 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
     if (nnz) HIPCHK(c, hipMemcpyAsync(P.ind.p, c->ind.p, 4 * nnz, hipMemcpyDeviceToDevice, c->stream));
     ReuseTables T;
     const int cap = block * reuse_tpt(pb->kind);  // task slots per tile
-    if ((rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, cap, block, T,
-                          env_int("ELFEL_HIP_PATCH_TILES", 1) != 0)))
+    P.fallback = false;
+    rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, cap, block, T, env_int("ELFEL_HIP_PATCH_TILES", 1) != 0);
+    if (rc == ELFEL_ENOTSUP) {
+        // the mesh does not fit the tiles: this pattern's assemblies run the
+        // plain FP64 method, whose result the pattern build just produced
+        P.fallback = true;
+        P.fallback_note = "plain FP64 method: " + c->err;
+        c->err.clear();
+    } else if (rc) {
         return rc;
-    if (env_int("ELFEL_HIP_TILE_ORDER", 1)) order_tiles(T, 8, cap, env_int("ELFEL_HIP_XCD_RUN", 0));
-    if ((rc = upload_tables(c, T, P))) return rc;
-    // the numeric kernel gathers from padded copies of the mesh arrays
-    if ((rc = ensure(c, P.conn_pad, 4 * (size_t)F0.nen * padded_len(F0.E, CONN_GAP))) ||
-        (rc = ensure(c, P.xy_pad, 16 * (size_t)padded_len(F0.Nn, XY_GAP))))
-        return rc;
-    if ((rc = copy_padded(c, P.conn_pad.p, F0.conn.p, F0.E, 4 * (size_t)F0.nen, CONN_GAP)) ||
-        (rc = copy_padded(c, P.xy_pad.p, F0.xy.p, F0.Nn, 16, XY_GAP)))
-        return rc;
+    }
+    if (!P.fallback) {
+        if (env_int("ELFEL_HIP_TILE_ORDER", 1)) order_tiles(T, 8, cap, env_int("ELFEL_HIP_XCD_RUN", 0));
+        if ((rc = upload_tables(c, T, P))) return rc;
+        // the numeric kernel gathers from padded copies of the mesh arrays
+        if ((rc = ensure(c, P.conn_pad, 4 * (size_t)F0.nen * padded_len(F0.E, CONN_GAP))) ||
+            (rc = ensure(c, P.xy_pad, 16 * (size_t)padded_len(F0.Nn, XY_GAP))))
+            return rc;
+        if ((rc = copy_padded(c, P.conn_pad.p, F0.conn.p, F0.E, 4 * (size_t)F0.nen, CONN_GAP)) ||
+            (rc = copy_padded(c, P.xy_pad.p, F0.xy.p, F0.Nn, 16, XY_GAP)))
+            return rc;
+    }
     P.block = cap;
     P.kind = pb->kind; P.fe = pb->fe; P.layout = pb->layout;
     P.key_mesh0 = F0.mesh_version; P.key_dof0 = F0.dof_version;
@@ -650,6 +676,16 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
         rc = build_symbolic(c, pb, tab, cf, block);
         tm_end(c, ELFEL_PH_SYMBOLIC, ev);
         if (rc) return rc;
+    }
+    if (P.fallback) {
+        elfel_hip_problem p2 = *pb;
+        p2.method = ELFEL_SORT;
+        p2.precision = ELFEL_FP64;
+        c->used_method = ELFEL_SORT;
+        c->used_precision = ELFEL_FP64;
+        c->used_note = P.fallback_note;
+        c->nnz = -1;
+        return plain_assemble(c, &p2, tab, cf);
     }
     const int64_t E = F0.E;
     const int nen = F0.nen;
@@ -713,7 +749,8 @@ template <int NEN, int NQ>
 __global__ __launch_bounds__(256) void k_geom_stats(int64_t E, const int32_t *__restrict__ conn,
                                                     const double2 *__restrict__ xy, QTable tab,
                                                     double *__restrict__ block_min,
-                                                    unsigned long long *__restrict__ nbad)
+                                                    unsigned long long *__restrict__ nbad,
+                                                    unsigned long long *__restrict__ nneg)
 {
     __shared__ double smin[256];
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -727,16 +764,18 @@ __global__ __launch_bounds__(256) void k_geom_stats(int64_t E, const int32_t *__
         }
         Geo<double, NEN, NQ> g;
         geometry<double, NEN, NQ>(tab, X, Y, g);
-        bool good = true;
+        bool good = true, pos = true;
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             mn = fmin(mn, g.det[q]);
             double j2 = 0.0;
 #pragma unroll
             for (int k = 0; k < NEN; k++) j2 += g.gx[q][k] * g.gx[q][k] + g.gy[q][k] * g.gy[q][k];
+            pos = pos && (g.det[q] > 0.0);
             good = good && (g.det[q] > 0.0) && (j2 * g.det[q] < REUSE_COND_MAX);
         }
         if (!good) atomicAdd(nbad, 1ull);
+        if (!pos) atomicAdd(nneg, 1ull);
     }
     smin[threadIdx.x] = mn;
     __syncthreads();
@@ -747,16 +786,18 @@ __global__ __launch_bounds__(256) void k_geom_stats(int64_t E, const int32_t *__
     if (threadIdx.x == 0) block_min[blockIdx.x] = smin[0];
 }
 
-int geometry_stats_run(elfel_hip_ctx *c, double *min_det, int64_t *nbad)
+// Geometry statistics of field 0 at the points of rule `rule` (0: the
+// element's common rule: T3 1 point, T6 3 points, Q4 2x2): smallest det J,
+// elements over the shape-quality threshold or with det J <= 0 somewhere
+// (nbad), and elements with det J <= 0 at some point (nneg).
+int geometry_check(elfel_hip_ctx *c, int rule, double *min_det, int64_t *nbad, int64_t *nneg)
 {
     const Field &F = c->f[0];
     if (!F.has_mesh) return set_err(c, ELFEL_ESTATE, "no mesh");
     if (F.sdim != 2 || (F.nen != 3 && F.nen != 4 && F.nen != 6))
         return set_err(c, ELFEL_ENOTSUP, "geometry statistics of T3, Q4 and T6 meshes only");
-    if (!c->has_result && c->pat.fe < 0) return set_err(c, ELFEL_ESTATE, "assemble first");
     const int fe = F.nen == 3 ? ELFEL_T3 : (F.nen == 6 ? ELFEL_T6 : ELFEL_Q4);
-    // Use the rule of the last pattern if any, else the element's common rule.
-    int rule = fe == ELFEL_T3 ? 1 : (fe == ELFEL_T6 ? 3 : 2);
+    if (rule <= 0) rule = fe == ELFEL_T3 ? 1 : (fe == ELFEL_T6 ? 3 : 2);
     QTable tab;
     if (make_qtable(fe, rule, &tab)) return set_err(c, ELFEL_EINVAL, "no quadrature table");
     const int64_t E = F.E;
@@ -764,23 +805,33 @@ int geometry_stats_run(elfel_hip_ctx *c, double *min_det, int64_t *nbad)
     int rc = ensure(c, c->scal, 8 * (size_t)(nb + 1));
     if (rc) return rc;
     double *bm = (double *)c->scal.p;
-    unsigned long long *cnt = (unsigned long long *)((int64_t *)c->dscal.p + 3);
-    HIPCHK(c, hipMemsetAsync(cnt, 0, 8, c->stream));
+    unsigned long long *cnt = (unsigned long long *)((int64_t *)c->dscal.p + 3);  // [0] bad shape, [1] det <= 0
+    HIPCHK(c, hipMemsetAsync(cnt, 0, 16, c->stream));
 #define GS(N, Q) hipLaunchKernelGGL((k_geom_stats<N, Q>), dim3(nb), dim3(256), 0, c->stream, E, \
-                                    (const int32_t *)F.conn.p, (const double2 *)F.xy.p, tab, bm, cnt)
-    if (fe == ELFEL_T3) GS(3, 1);
-    else if (fe == ELFEL_T6) GS(6, 3);
-    else GS(4, 4);
+                                    (const int32_t *)F.conn.p, (const double2 *)F.xy.p, tab, bm, cnt, cnt + 1)
+    if (fe == ELFEL_T3 && tab.nq == 1) GS(3, 1);
+    else if (fe == ELFEL_T3 && tab.nq == 3) GS(3, 3);
+    else if (fe == ELFEL_T6 && tab.nq == 3) GS(6, 3);
+    else if (fe == ELFEL_Q4 && tab.nq == 1) GS(4, 1);
+    else if (fe == ELFEL_Q4 && tab.nq == 4) GS(4, 4);
+    else return set_err(c, ELFEL_ENOTSUP, "geometry statistics: unsupported rule");
 #undef GS
     HIPCHK(c, hipGetLastError());
     std::vector<double> h(nb);
-    unsigned long long hc = 0;
+    unsigned long long hc[2] = {0, 0};
     HIPCHK(c, hipMemcpyAsync(h.data(), bm, 8 * nb, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(&hc, cnt, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hc, cnt, 16, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     double m = 1e300;
     for (double v : h) m = std::min(m, v);
     if (min_det) *min_det = m;
-    if (nbad) *nbad = (int64_t)hc;
+    if (nbad) *nbad = (int64_t)hc[0];
+    if (nneg) *nneg = (int64_t)hc[1];
     return 0;
+}
+
+int geometry_stats_run(elfel_hip_ctx *c, double *min_det, int64_t *nbad)
+{
+    if (!c->has_result && c->pat.fe < 0) return set_err(c, ELFEL_ESTATE, "assemble first");
+    return geometry_check(c, 0, min_det, nbad, nullptr);
 }

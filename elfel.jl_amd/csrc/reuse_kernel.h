@@ -639,19 +639,28 @@ void k_reuse(ReuseArgs A, Coef cf, QTable tab, int lds_vals)
     reuse_tile<KIND, R, NEN, NC, NQ, CSC, BLOCK, reuse_tpt(KIND)>(A, cf, tab, lds, lds_vals, blockIdx.x);
 }
 
+// Arguments of the batched kernel, kept in device memory: passed by value, the
+// two kinds' arguments (about 120 dwords) were loaded into scalar registers
+// at once, and 67 of them were spilled into vector-register lanes.
+struct BatchArgs {
+    ReuseArgs H, Ev;
+    Coef ch, ce;
+};
+
 // Batched problems of uneven size in one persistent launch: heat-T3 (1-point)
 // tiles first, then elasticity-T6 (3-point) tiles; each workgroup walks the
 // tile list with stride gridDim.x.  The tile kind is uniform per workgroup;
-// heat tiles hold two tasks per thread, elasticity tiles one.
+// heat tiles hold two tasks per thread, elasticity tiles one.  The mixed
+// kernels take their quadrature tables from qt_fixed, so no table is passed.
 template <bool CSC, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_reuse_batch(ReuseArgs H, ReuseArgs Ev, Coef cfh, Coef cfe, QTable th,
-                                                       QTable te, int lds_vals)
+__global__ __launch_bounds__(BLOCK) void k_reuse_batch(const BatchArgs *__restrict__ B, int nH, int nt,
+                                                       int lds_vals)
 {
     extern __shared__ double lds[];
-    const int nt = H.ntiles + Ev.ntiles;
+    constexpr QTable th = qt_fixed<3, 1>(), te = qt_fixed<6, 3>();
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
-        if (t < H.ntiles) reuse_tile<ELFEL_HEAT, float, 3, 1, 1, CSC, BLOCK, 2>(H, cfh, th, lds, lds_vals, t);
-        else reuse_tile<ELFEL_ELAST, float, 6, 2, 3, CSC, BLOCK, 1>(Ev, cfe, te, lds, lds_vals, t - H.ntiles);
+        if (t < nH) reuse_tile<ELFEL_HEAT, float, 3, 1, 1, CSC, BLOCK, 2>(B->H, B->ch, th, lds, lds_vals, t);
+        else reuse_tile<ELFEL_ELAST, float, 6, 2, 3, CSC, BLOCK, 1>(B->Ev, B->ce, te, lds, lds_vals, t - nH);
         __syncthreads();
     }
 }

@@ -13,6 +13,7 @@ LIBPATH = os.environ.get("ELFEL_HIP_LIB", os.path.join(os.path.dirname(_PKG), "l
 
 # status codes
 OK, EINVAL, ERANGE, ENOMEM, EHIP, ESTATE, EGEOM, ENOTSUP = 0, -1, -2, -3, -4, -5, -6, -7
+CHECK_GEOM = 1  # elfel_hip_problem.flags
 # enums
 HEAT, ELAST, STOKES_GEN, STOKES_VECLAP, STOKES_REDDY, MASS = 0, 1, 2, 3, 4, 5
 T3, T6, Q4, T3B, Q4L2, T4, L2 = 0, 1, 2, 3, 4, 5, 6
@@ -35,6 +36,7 @@ EXPORTED = [
     "elfel_hip_colliding_errors",
     "elfel_hip_block_mesh", "elfel_hip_t6_to_t3", "elfel_hip_boundary_nodes", "elfel_hip_copy_mesh",
     "elfel_hip_load_mesh", "elfel_hip_set_mesh_sdim", "elfel_hip_set_element_dofs",
+    "elfel_hip_assembly_info",
 ]
 
 
@@ -53,7 +55,7 @@ class Problem(ctypes.Structure):
         ("kind", ctypes.c_int32), ("fe", ctypes.c_int32), ("rule", ctypes.c_int32),
         ("precision", ctypes.c_int32), ("method", ctypes.c_int32), ("layout", ctypes.c_int32),
         ("kappa", ctypes.c_double), ("Q", ctypes.c_double), ("mu", ctypes.c_double),
-        ("D", ctypes.c_double * 9), ("fe_p", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("D", ctypes.c_double * 9), ("fe_p", ctypes.c_int32), ("flags", ctypes.c_int32),
     ]
 
 
@@ -90,6 +92,8 @@ def load():
     lib.elfel_hip_set_coefficients.argtypes = [P, ctypes.c_int64, f64p, f64p, f64p]
     lib.elfel_hip_assemble.argtypes = [P, ctypes.POINTER(Problem), i64p]
     lib.elfel_hip_copy_vector.argtypes = [P, f64p, ctypes.c_int64]
+    lib.elfel_hip_assembly_info.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                            ctypes.c_char_p, ctypes.c_int]
     lib.elfel_hip_synchronize.argtypes = [P]
     lib.elfel_hip_geometry_stats.argtypes = [P, f64p, i64p]
     lib.elfel_hip_number_dofs.argtypes = [P, ctypes.c_int32, i64p, ctypes.POINTER(ctypes.c_int32),
@@ -251,9 +255,9 @@ class Context:
                   "set_coefficients")
 
     def assemble(self, kind, fe, rule, precision=FP64, method=SORT, layout=CSR, kappa=1.0, Q=0.0, mu=1.0,
-                 D=None, wait=True, fe_p=T3):
+                 D=None, wait=True, fe_p=T3, flags=0):
         pb = Problem()
-        pb.kind, pb.fe, pb.rule, pb.fe_p = kind, fe, rule, fe_p
+        pb.kind, pb.fe, pb.rule, pb.fe_p, pb.flags = kind, fe, rule, fe_p, flags
         pb.precision, pb.method, pb.layout = precision, method, layout
         pb.kappa, pb.Q, pb.mu = kappa, Q, mu
         if D is not None:
@@ -265,6 +269,16 @@ class Context:
         self._chk(self.lib.elfel_hip_assemble(self.h, ctypes.byref(pb), ctypes.byref(nnz) if wait else None),
                   "assemble")
         return nnz.value if wait else None
+
+    def assembly_info(self):
+        """(method, precision, note) of what the last assemble ran: the REUSE
+        method falls back to the plain FP64 method on meshes its tiles do not
+        hold (note says why)."""
+        m, p = ctypes.c_int(), ctypes.c_int()
+        buf = ctypes.create_string_buffer(256)
+        self._chk(self.lib.elfel_hip_assembly_info(self.h, ctypes.byref(m), ctypes.byref(p), buf, 256),
+                  "assembly_info")
+        return m.value, p.value, buf.value.decode()
 
     def copy_vector(self, n):
         F = np.empty(n, np.float64)

@@ -130,8 +130,17 @@ typedef struct {
     double D[9];        /* elasticity / Stokes-gen material matrix, column-major */
     int32_t fe_p;       /* Reddy Stokes: pressure element, ELFEL_T3 (field 1's nodes) or ELFEL_Q4L2
                            (field 1's cell dofs); pairs (T6, T3), (T3B, T3), (Q4, Q4L2) */
-    int32_t reserved;
+    int32_t flags;      /* ELFEL_CHECK_GEOM or 0 */
 } elfel_hip_problem;
+
+/* elfel_hip_problem.flags: ELFEL_CHECK_GEOM makes elfel_hip_assemble first
+ * evaluate det J of field 0's elements (T3, T6, Q4 in 2D) at the points of
+ * the problem's rule, in FP64, and return ELFEL_EGEOM without assembling if
+ * any is <= 0.  The reference does no such check (jacjac, src/FElements.jl:
+ * 148-168, returns whatever determinant the nodes give and the element loop
+ * goes on), so without the flag an inverted element is assembled as the
+ * reference would assemble it. */
+enum { ELFEL_CHECK_GEOM = 1 };
 
 /* Integrates every element and builds the global matrix (and the load vector
  * for heat) on the device.  Replaces the user integrate! loops with
@@ -148,6 +157,16 @@ typedef struct {
  * dofs).  Blocks are appended as the drivers do: kxx, kxy, kxy', kyy, kxp,
  * kxp', kyp, kyp'. */
 int elfel_hip_assemble(elfel_hip_ctx *ctx, const elfel_hip_problem *pb, int64_t *nnz);
+/* What the last elfel_hip_assemble ran.  The REUSE method needs every node in
+ * at most 16 elements, rows of at most 128 entries, each dof owned by one
+ * (node, component) and elements whose dofs are distinct; a mesh or dof map
+ * that breaks one of these (valid input: the reference's sparse() assembles any
+ * COO stream, src/Assemblers.jl:97-123) is assembled by the plain FP64 method
+ * instead, with the same pattern and the reference's values.  method and
+ * precision receive the method and arithmetic used; note (len bytes, may be
+ * null) the reason when they differ from the request.  No counterpart in the
+ * reference, which has one method. */
+int elfel_hip_assembly_info(elfel_hip_ctx *ctx, int *method, int *precision, char *note, int len);
 /* Load vector of the last heat assembly (finish!(SysvecAssembler), :228-230). */
 int elfel_hip_copy_vector(elfel_hip_ctx *ctx, double *F, int64_t n);
 /* Blocks until the context's stream is idle. */
