@@ -13,6 +13,7 @@ int plain_coo_keys(elfel_hip_ctx *c, const int64_t *I, const int64_t *J, int64_t
                    int minor_bits, int csc, uint64_t *keys);
 int plain_vec_keys(elfel_hip_ctx *c, const int64_t *I, int64_t n, int64_t ndofs, uint64_t *keys);
 int geometry_stats_run(elfel_hip_ctx *c, double *min_det, int64_t *nbad);
+int noalloc_init_device(int device);
 int geometry_check(elfel_hip_ctx *c, int rule, double *min_det, int64_t *nbad, int64_t *nneg);
 
 // ---------------------------------------------------------------------------
@@ -184,6 +185,13 @@ extern "C" int elfel_hip_init(int device, elfel_hip_ctx **out)
         hipStreamDestroy(c->stream);
         delete c;
         return ELFEL_ENOMEM;
+    }
+    // this device's address of the allocation-free entry point's scratch
+    if (int rc = noalloc_init_device(device)) {
+        release(c->dscal);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
     }
     *out = c;
     return ELFEL_OK;

@@ -186,14 +186,28 @@ enum { DERR_RANGE = 1, DERR_GEOM = 2 };
 int make_qtable(int fe, int rule, QTable *t);
 
 // ---- sort.hip ---------------------------------------------------------------
+// Scratch of the sort and the segmented reduce handed in by the caller (the
+// allocation-free entry point); null: the context's buffers, grown on demand.
+struct SortScratch {
+    uint32_t *hist = nullptr;    // radix_hist_entries(n)
+    int64_t hist_cap = 0;
+    uint32_t *scan = nullptr;    // scan_scratch_entries of the longest scan
+    int64_t scan_cap = 0;
+    uint32_t *flags = nullptr;   // n entries
+    uint32_t *segpos = nullptr;  // n + 1 entries
+    int64_t n_cap = 0;
+};
 int bits_for(int64_t n);  // bits needed to hold 0..n-1
+int64_t radix_hist_entries(int64_t n);
+int64_t scan_scratch_entries(int64_t n);
 int radix_sort_pairs(elfel_hip_ctx *c, uint64_t *&keys, uint64_t *&vals, uint64_t *&keys_alt,
-                     uint64_t *&vals_alt, int64_t n, int nbits);
-int exclusive_scan_u32(elfel_hip_ctx *c, uint32_t *data, int64_t n, uint32_t *total_dev);
+                     uint64_t *&vals_alt, int64_t n, int nbits, const SortScratch *s = nullptr);
+int exclusive_scan_u32(elfel_hip_ctx *c, uint32_t *data, int64_t n, uint32_t *total_dev,
+                       const SortScratch *s = nullptr);
 // From sorted (key, value) pairs build CSR: key = (major << minor_bits) | minor.
 int build_csr_from_sorted(elfel_hip_ctx *c, const uint64_t *keys, const double *vals, int64_t n,
                           int minor_bits, int64_t nmaj, int32_t *ptr, int32_t *ind, double *val,
-                          int32_t *nnz_dev, bool values);
+                          int32_t *nnz_dev, bool values, const SortScratch *s = nullptr);
 // Sorted (row, value) pairs -> dense vector (rows without entries = 0).
 int build_dense_from_sorted(elfel_hip_ctx *c, const uint64_t *keys, const double *vals, int64_t n,
                             int64_t ndofs, double *out);

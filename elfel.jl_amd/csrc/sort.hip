@@ -110,10 +110,16 @@ static int scan_rec(elfel_hip_ctx *c, uint32_t *data, int64_t n, uint32_t *tmp, 
     return 0;
 }
 
-int exclusive_scan_u32(elfel_hip_ctx *c, uint32_t *data, int64_t n, uint32_t *total_dev)
+int64_t scan_scratch_entries(int64_t n) { return n / (SC_CHUNK - 1) + 64; }
+
+int exclusive_scan_u32(elfel_hip_ctx *c, uint32_t *data, int64_t n, uint32_t *total_dev, const SortScratch *s)
 {
     // Scratch: nb + nb/4096 + ... < n/4000 + 64 entries.
-    int64_t cap = n / (SC_CHUNK - 1) + 64;
+    int64_t cap = scan_scratch_entries(n);
+    if (s) {
+        if (s->scan_cap < cap) return set_err(c, ELFEL_EINVAL, "internal: scan scratch too small");
+        return scan_rec(c, data, n, s->scan, s->scan_cap, total_dev);
+    }
     int rc = ensure(c, c->scan_tmp, sizeof(uint32_t) * cap);
     if (rc) return rc;
     return scan_rec(c, data, n, (uint32_t *)c->scan_tmp.p, cap, total_dev);
@@ -231,20 +237,28 @@ __global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint64_t *__restr
     }
 }
 
+int64_t radix_hist_entries(int64_t n) { return (int64_t)RS_RADIX * ceil_div(std::max<int64_t>(n, 1), RS_TILE); }
+
 int radix_sort_pairs(elfel_hip_ctx *c, uint64_t *&keys, uint64_t *&vals, uint64_t *&keys_alt,
-                     uint64_t *&vals_alt, int64_t n, int nbits)
+                     uint64_t *&vals_alt, int64_t n, int nbits, const SortScratch *s)
 {
     if (n <= 1 || nbits <= 0) return 0;
     if (n >= ((int64_t)1 << 32)) return set_err(c, ELFEL_ENOTSUP, "sort of more than 2^32 keys");
     const int ntiles = ceil_div(n, RS_TILE);
     const int64_t nh = (int64_t)RS_RADIX * ntiles;
-    int rc = ensure(c, c->hist, sizeof(uint32_t) * nh);
-    if (rc) return rc;
-    uint32_t *hist = (uint32_t *)c->hist.p;
+    uint32_t *hist = nullptr;
+    int rc;
+    if (s) {
+        if (s->hist_cap < nh) return set_err(c, ELFEL_EINVAL, "internal: histogram scratch too small");
+        hist = s->hist;
+    } else {
+        if ((rc = ensure(c, c->hist, sizeof(uint32_t) * nh))) return rc;
+        hist = (uint32_t *)c->hist.p;
+    }
     for (int shift = 0; shift < nbits; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist, dim3(ntiles), dim3(RS_BLOCK), 0, c->stream, keys, n, shift, hist, ntiles);
         HIPCHK(c, hipGetLastError());
-        rc = exclusive_scan_u32(c, hist, nh, nullptr);
+        rc = exclusive_scan_u32(c, hist, nh, nullptr, s);
         if (rc) return rc;
         hipLaunchKernelGGL(k_rs_scatter, dim3(ntiles), dim3(RS_BLOCK), 2 * RS_TILE * sizeof(uint64_t), c->stream,
                            keys, vals, keys_alt, vals_alt, n, shift, hist, ntiles);
@@ -321,7 +335,7 @@ __global__ void k_fill_f64(double *p, int64_t n, double v)
 
 int build_csr_from_sorted(elfel_hip_ctx *c, const uint64_t *keys, const double *vals, int64_t n,
                           int minor_bits, int64_t nmaj, int32_t *ptr, int32_t *ind, double *val,
-                          int32_t *nnz_dev, bool values)
+                          int32_t *nnz_dev, bool values, const SortScratch *s)
 {
     if (n == 0) {
         hipLaunchKernelGGL(k_fill_i32, dim3(ceil_div(nmaj + 1, 256)), dim3(256), 0, c->stream, ptr, nmaj + 1, 0);
@@ -330,18 +344,28 @@ int build_csr_from_sorted(elfel_hip_ctx *c, const uint64_t *keys, const double *
         HIPCHK(c, hipGetLastError());
         return 0;
     }
-    int rc = ensure(c, c->flags, sizeof(uint32_t) * n);
-    if (rc) return rc;
-    uint32_t *seg = (uint32_t *)c->flags.p;
+    int rc;
+    uint32_t *seg = nullptr, *hpos = nullptr;
+    if (s) {
+        if (s->n_cap < n) return set_err(c, ELFEL_EINVAL, "internal: segment scratch too small");
+        seg = s->flags;
+        hpos = s->segpos;
+    } else {
+        if ((rc = ensure(c, c->flags, sizeof(uint32_t) * n))) return rc;
+        seg = (uint32_t *)c->flags.p;
+    }
     hipLaunchKernelGGL(k_mark_heads, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, keys, n, seg);
     HIPCHK(c, hipGetLastError());
-    rc = exclusive_scan_u32(c, seg, n, nullptr);
+    rc = exclusive_scan_u32(c, seg, n, nullptr, s);
     if (rc) return rc;
     // one work-item per segment over the compacted head positions (the
     // per-entry form idled the lanes of every entry that does not start a
     // segment while the head lanes summed)
-    if ((rc = ensure(c, c->segpos, sizeof(uint32_t) * (n + 1)))) return rc;
-    uint32_t *hpos = (uint32_t *)c->segpos.p, *nseg = hpos + n;
+    if (!s) {
+        if ((rc = ensure(c, c->segpos, sizeof(uint32_t) * (n + 1)))) return rc;
+        hpos = (uint32_t *)c->segpos.p;
+    }
+    uint32_t *nseg = hpos + n;
     hipLaunchKernelGGL(k_head_pos, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, keys, n, seg, hpos, nseg);
     HIPCHK(c, hipGetLastError());
     const int grid = (int)std::min<int64_t>(ceil_div(n, 256), 8192);

@@ -36,8 +36,9 @@ EXPORTED = [
     "elfel_hip_colliding_errors",
     "elfel_hip_block_mesh", "elfel_hip_t6_to_t3", "elfel_hip_boundary_nodes", "elfel_hip_copy_mesh",
     "elfel_hip_load_mesh", "elfel_hip_set_mesh_sdim", "elfel_hip_set_element_dofs",
-    "elfel_hip_assembly_info",
+    "elfel_hip_assembly_info", "elfel_hip_sparse_noalloc",
 ]
+NOALLOC_MAX_COO = 1 << 20  # ELFEL_NOALLOC_MAX_COO
 
 
 class ElfelHipError(RuntimeError):
@@ -81,6 +82,8 @@ def load():
     lib.elfel_hip_sparse.argtypes = [P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, i64p, i64p, f64p,
                                      ctypes.c_int, i64p]
     lib.elfel_hip_copy_sparse.argtypes = [P, ctypes.c_int, i64p, i64p, f64p]
+    lib.elfel_hip_sparse_noalloc.argtypes = [P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, P, P, P, ctypes.c_int,
+                                             P, P, P, i64p]
     lib.elfel_hip_result_info.argtypes = [P, i64p, i64p, i64p, ctypes.POINTER(ctypes.c_int)]
     lib.elfel_hip_vector.argtypes = [P, ctypes.c_int64, ctypes.c_int64, i64p, f64p, f64p]
     lib.elfel_hip_set_mesh.argtypes = [P, ctypes.c_int, ctypes.c_int32, ctypes.c_int64, i64p, ctypes.c_int64, f64p]
@@ -201,6 +204,14 @@ class Context:
         self._chk(self.lib.elfel_hip_copy_sparse(self.h, base, _ptr(ptr, ctypes.c_int64), _ptr(ind, ctypes.c_int64),
                                                  _ptr(val, ctypes.c_double)), "copy_sparse")
         return ptr, ind, val, (nr, nc), lay
+
+    def sparse_noalloc(self, nrow, ncol, n, row, col, val, layout, ptr, ind, out):
+        """elfel_hip_sparse_noalloc on device arrays given by address (row, col:
+        1-based int64; val float64; ptr, ind: int32; out: float64); returns nnz."""
+        nnz = ctypes.c_int64(0)
+        self._chk(self.lib.elfel_hip_sparse_noalloc(self.h, nrow, ncol, n, row, col, val, layout, ptr, ind, out,
+                                                    ctypes.byref(nnz)), "sparse_noalloc")
+        return nnz.value
 
     def vector(self, idx, val, ndofs):
         idx, val = _i64(idx), _f64(val)

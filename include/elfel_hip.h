@@ -78,6 +78,25 @@ int elfel_hip_sparse(elfel_hip_ctx *ctx, int64_t nrow, int64_t ncol, int64_t n,
                      const int64_t *row, const int64_t *col, const double *val,
                      int layout, int64_t *nnz);
 
+/* Allocation-free variant of elfel_hip_sparse.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc): the call makes no
+ * hipMalloc / hipFree and no allocating library call.  row, col (1-based
+ * Int64), val are DEVICE arrays of n COO entries; the result goes to the
+ * caller's DEVICE arrays ptr (nmajor + 1 int32), ind and out (room for n
+ * entries: nnz <= n), 0-based; *nnz (host) receives the entry count.  The
+ * sort's scratch lives in one static device array per GPU, sized for
+ * n <= ELFEL_NOALLOC_MAX_COO entries (about 41 MiB, held until the process
+ * ends); a larger n returns ELFEL_EINVAL.  All calls on one GPU share that
+ * array, so calls that overlap on one GPU (two contexts, two threads) race;
+ * the context's own calls are serialised on its stream.  Same order and
+ * semantics as elfel_hip_sparse (stable order, structural zeros kept,
+ * ELFEL_ERANGE for indices out of range); the call waits for the result. */
+#define ELFEL_NOALLOC_MAX_COO (1 << 20)
+int elfel_hip_sparse_noalloc(elfel_hip_ctx *ctx, int64_t nrow, int64_t ncol, int64_t n, const int64_t *row,
+                             const int64_t *col, const double *val, int layout, int32_t *ptr, int32_t *ind,
+                             double *out, int64_t *nnz);
+
 /* Copies the matrix built by the last elfel_hip_sparse / elfel_hip_assemble.
  * base = 1: 1-based Int64 (SparseMatrixCSC colptr/rowval when layout = CSC);
  * base = 0: 0-based.  ptr has nmajor+1 entries, ind and val nnz entries. */
