@@ -404,6 +404,36 @@ def bench_pcg(ctx, L, cs, res, args_maxit=60000):
 # ---------------------------------------------------------------------------
 # C4: batched uneven problems
 # ---------------------------------------------------------------------------
+def batch_fill(ctx, L, probs, precision):
+    ctx.batch_begin()
+    for p in probs:
+        f = p["fesp"]
+        if p["kind"] == "heat":
+            ctx.batch_add(L.HEAT, L.T3, 1, f.mesh.conn, f.mesh.xy, f.dofnums, f.ndofs(),
+                          coef_e=p["kappa_e"], kappa=1.0, Q=p["Q"], precision=precision)
+        else:
+            ctx.batch_add(L.ELAST, L.T6, 3, f.mesh.conn, f.mesh.xy, f.dofnums, f.ndofs(),
+                          coef_e=p["scale_e"], D=p["D"], precision=precision)
+
+
+def batch_time(ctx, steps, warmup):
+    """(event ms per replay, wall ms per replay, clock-ramp launches)."""
+    ctx.batch_assemble()
+    ramp = clock_ramp(lambda: ctx.batch_assemble(wait=False), ctx.synchronize)
+    for _ in range(warmup):
+        ctx.batch_assemble(wait=False)
+    ctx.synchronize()
+    # one pair of hipEvents on the library's stream around the timed replays
+    ctx.timing_region()
+    w0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.batch_assemble(wait=False)
+    ctx.synchronize()
+    wall = (time.perf_counter() - w0) * 1e3 / steps
+    ph = ctx.timing_read()
+    return ph["total"][0] / max(1, ph["total"][1]), wall, ramp
+
+
 def bench_batch(args, L, W):
     ctx = L.Context(0)
     devname, ncu = ctx.device_name()
@@ -411,40 +441,35 @@ def bench_batch(args, L, W):
     t0 = time.time()
     probs = [W.batch_problem(k, kind, nn) for k, (kind, nn) in enumerate(specs)]
     t_host = time.time() - t0
-    ctx.batch_begin()
     E = Nn = conn_b = coef_b = F_b = 0
     for p in probs:
         f = p["fesp"]
         if p["kind"] == "heat":
-            ctx.batch_add(L.HEAT, L.T3, 1, f.mesh.conn, f.mesh.xy, f.dofnums, f.ndofs(),
-                          coef_e=p["kappa_e"], kappa=1.0, Q=p["Q"])
             F_b += f.ndofs() * 8
-        else:
-            ctx.batch_add(L.ELAST, L.T6, 3, f.mesh.conn, f.mesh.xy, f.dofnums, f.ndofs(),
-                          coef_e=p["scale_e"], D=p["D"])
         E += f.mesh.nelements
         Nn += f.mesh.nnodes
         conn_b += f.mesh.conn.size * 4
         coef_b += f.mesh.nelements * 8
+    batch_fill(ctx, L, probs, L.MIXED)
     t0 = time.perf_counter()
     ctx.batch_build(autotune=True)
     t_build = (time.perf_counter() - t0) * 1e3
     st = ctx.batch_stats()
-    ctx.batch_assemble()
-    ramp = clock_ramp(lambda: ctx.batch_assemble(wait=False), ctx.synchronize)
-    for _ in range(args.warmup):
-        ctx.batch_assemble(wait=False)
-    ctx.synchronize()
-    # one pair of hipEvents on the library's stream around the timed replays
-    ctx.timing_region()
-    w0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.batch_assemble(wait=False)
-    ctx.synchronize()
-    wall = (time.perf_counter() - w0) * 1e3 / args.steps
-    ph = ctx.timing_read()
-    ms = ph["total"][0] / max(1, ph["total"][1])
+    ms, wall, ramp = batch_time(ctx, args.steps, args.warmup)
+    progress("mixed batch timed")
     info0 = ctx.batch_info(0)
+    # the same batch in FP64 (the FP64 REUSE arithmetic in the same launch)
+    c64 = L.Context(0)
+    batch_fill(c64, L, probs, L.FP64)
+    c64.batch_build(autotune=True)
+    ms64, wall64, _ = batch_time(c64, max(1, args.steps // 4), min(args.warmup, 5))
+    progress("fp64 batch timed")
+    err64 = 0.0
+    for k in range(len(probs)):
+        a, b = ctx.batch_copy(k), c64.batch_copy(k)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        err64 = max(err64, rel_frob(a[2], b[2]))
+    c64.close()
     kernel_bytes = conn_b + Nn * 16 + coef_b + st["nnz"] * 8 + F_b
     # errors of a sample of problems against the FP64 plain method
     errs = []
@@ -491,9 +516,12 @@ def bench_batch(args, L, W):
         "cpu_baseline": None,
         "wall_ms_per_step": wall,
         "clock_ramp_steps": ramp,
+        "fp64_batch": {"ms_per_step": ms64, "wall_ms_per_step": wall64, "elements_per_s": E / (ms64 * 1e-3)},
+        "speedup_mixed_vs_fp64_batch": ms64 / ms,
         "fp64_plain_elements_per_s_sample": fp64_eps,
         "speedup_mixed_vs_fp64_plain": (E / (ms * 1e-3)) / fp64_eps,
-        "errors": {"max_frobenius_vs_fp64_sample": max(errs), "problems_checked": len(sample)},
+        "errors": {"max_frobenius_vs_fp64_sample": max(errs), "problems_checked": len(sample),
+                   "max_frobenius_vs_fp64_batch_all_problems": err64},
         "setup": {"host_meshes_s": t_host, "build_and_autotune_ms": t_build, "tuned_ms": st["tuned_ms"]},
         "device": {"name": devname, "cus": ncu},
     }

@@ -13,8 +13,8 @@ int plain_coo_keys(elfel_hip_ctx *c, const int64_t *I, const int64_t *J, int64_t
                    int minor_bits, int csc, uint64_t *keys);
 int plain_vec_keys(elfel_hip_ctx *c, const int64_t *I, int64_t n, int64_t ndofs, uint64_t *keys);
 int geometry_stats_run(elfel_hip_ctx *c, double *min_det, int64_t *nbad);
-int noalloc_init_device(int device);
 int geometry_check(elfel_hip_ctx *c, int rule, double *min_det, int64_t *nbad, int64_t *nneg);
+int noalloc_init_device(int device);
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -208,7 +208,7 @@ extern "C" int elfel_hip_free(elfel_hip_ctx *c)
                    &c->fvals[1], &c->stage64, &c->segpos, &c->ptr, &c->ind, &c->val, &c->F, &c->dscal, &c->pcg, &c->pcg_sell,
                    &c->pat.ptr, &c->pat.ind, &c->pat.tile, &c->pat.tile_rows, &c->pat.task_elem,
                    &c->pat.task_info, &c->pat.task_pos, &c->pat.tile_elem, &c->pat.conn_pad, &c->pat.xy_pad,
-                   &c->pat.truns, &c->pat.runs};
+                   &c->pat.truns, &c->pat.runs, &c->pat.rrec};
     for (Buf *b : bufs) release(*b);
     batch_free(c);
     for (auto &p : c->tm.pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }

@@ -1,6 +1,6 @@
 // batch.hip -- many independent problems of uneven size assembled by one
-// persistent fused launch (heat-T3 1-point and elasticity-T6 3-point, mixed
-// precision), replayed from a hipGraph.
+// persistent fused launch (heat T3 1-point, heat Q4 2x2 and elasticity T6
+// 3-point; all mixed or all FP64), replayed from a hipGraph.
 //
 // build: per problem, the pattern comes from the plain method and the task
 // tables from the reuse tiler; the tables of all problems of one kind are
@@ -35,16 +35,27 @@ struct KindData {
     Pattern tab[2];  // tables for block 128 and 256
 };
 
+// The problem kinds of a batch (index into Batch::kd): the element, rule and
+// dof layout each stands for.
+enum { BK_HEAT_T3 = 0, BK_ELAST_T6 = 1, BK_HEAT_Q4 = 2, BK_N = 3 };
+struct KindSpec {
+    int kind, fe, rule, nen, nc;
+    bool heat;
+    const char *name;
+};
+static const KindSpec KS[BK_N] = {{ELFEL_HEAT, ELFEL_T3, 1, 3, 1, true, "heatT3r1"},
+                                  {ELFEL_ELAST, ELFEL_T6, 3, 6, 2, false, "elastT6r3"},
+                                  {ELFEL_HEAT, ELFEL_Q4, 2, 4, 1, true, "heatQ4r2"}};
+
 struct Batch {
     std::vector<BatchProb> probs;
-    elfel_hip_problem pb_heat{}, pb_el{};
-    bool have_heat = false, have_el = false, built = false;
-    int layout = ELFEL_CSC;
-    KindData kd[2];  // 0 heat T3, 1 elasticity T6
+    elfel_hip_problem pb[BK_N]{};  // the uniform coefficients of each kind
+    bool have[BK_N] = {false, false, false}, built = false;
+    int layout = ELFEL_CSC, precision = -1;
+    KindData kd[BK_N];
     Buf ptr, ind, val, F;
     int64_t total_nnz = 0, total_ptr = 0, total_rows = 0;
     int block = 256, grid = 0, lds_vals = 0;
-    Buf args[2];  // BatchArgs in device memory, per workgroup size (128, 256)
     double tuned_ms = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -73,11 +84,10 @@ void batch_free(elfel_hip_ctx *c)
         release(k.conn); release(k.xy); release(k.coef);
         for (auto &t : k.tab) {
             release(t.tile); release(t.tile_rows); release(t.task_elem); release(t.task_info); release(t.task_pos);
-            release(t.tile_elem);
+            release(t.tile_elem); release(t.rrec);
         }
     }
     release(b->ptr); release(b->ind); release(b->val); release(b->F);
-    release(b->args[0]); release(b->args[1]);
     delete b;
     c->batch = nullptr;
 }
@@ -103,23 +113,31 @@ extern "C" int elfel_hip_batch_add(elfel_hip_ctx *c, const elfel_hip_problem *pb
     if (b->built) return set_err(c, ELFEL_ESTATE, "batch already built; call elfel_hip_batch_begin");
     if (!pb || E <= 0 || Nn <= 0 || !conn || !xy || !dofnums || ndofs <= 0)
         return set_err(c, ELFEL_EINVAL, "elfel_hip_batch_add: bad arguments");
-    const bool heat = pb->kind == ELFEL_HEAT && pb->fe == ELFEL_T3 && pb->rule == 1 && nen == 3 && ncomp == 1;
-    const bool el = pb->kind == ELFEL_ELAST && pb->fe == ELFEL_T6 && pb->rule == 3 && nen == 6 && ncomp == 2;
-    if (!heat && !el) return set_err(c, ELFEL_ENOTSUP, "batch problems: heat T3 1-point or elasticity T6 3-point");
-    if (pb->precision != ELFEL_MIXED) return set_err(c, ELFEL_ENOTSUP, "the batched launch is the mixed method");
-    elfel_hip_problem &ref = heat ? b->pb_heat : b->pb_el;
-    bool &have = heat ? b->have_heat : b->have_el;
-    if (!have) { ref = *pb; have = true; }
+    int kind = -1;
+    for (int k = 0; k < BK_N; k++)
+        if (pb->kind == KS[k].kind && pb->fe == KS[k].fe && pb->rule == KS[k].rule && nen == KS[k].nen &&
+            ncomp == KS[k].nc)
+            kind = k;
+    if (kind < 0)
+        return set_err(c, ELFEL_ENOTSUP, "batch problems: heat T3 1-point, heat Q4 2x2 or elasticity T6 3-point");
+    if (pb->precision != ELFEL_MIXED && pb->precision != ELFEL_FP64)
+        return set_err(c, ELFEL_EINVAL, "elfel_hip_batch_add: bad precision");
+    if (b->precision >= 0 && pb->precision != b->precision)
+        return set_err(c, ELFEL_ENOTSUP, "the problems of one batch share their precision");
+    if (pb->flags) return set_err(c, ELFEL_ENOTSUP, "batch problems take no flags");
+    elfel_hip_problem &ref = b->pb[kind];
+    if (!b->have[kind]) { ref = *pb; b->have[kind] = true; }
     else if (memcmp(&ref, pb, sizeof(*pb)) != 0)
         return set_err(c, ELFEL_ENOTSUP, "problems of one kind must share their uniform coefficients");
     if (!b->probs.empty() && pb->layout != b->layout) return set_err(c, ELFEL_EINVAL, "mixed layouts in one batch");
     b->layout = pb->layout;
+    b->precision = pb->precision;
     for (int64_t i = 0; i < E * nen; i++)
         if (conn[i] < 1 || conn[i] > Nn) return set_err(c, ELFEL_ERANGE, "connectivity outside 1..Nn");
     for (int64_t i = 0; i < Nn * ncomp; i++)
         if (dofnums[i] < 1 || dofnums[i] > ndofs) return set_err(c, ELFEL_ERANGE, "dof number outside 1..ndofs");
     BatchProb p;
-    p.kind = heat ? 0 : 1;
+    p.kind = kind;
     p.E = E; p.Nn = Nn; p.ndofs = ndofs;
     p.conn.assign(conn, conn + E * nen);
     p.dofs.assign(dofnums, dofnums + Nn * ncomp);
@@ -144,8 +162,13 @@ static bool tune_lookup(const std::string &key, int *block, int *gmult)
     char k[512];
     int bl, gm;
     bool found = false;
+    // only the launch shapes the autotuner times are taken from the file
     while (fscanf(f, "%511s %d %d", k, &bl, &gm) == 3)
-        if (key == k) { *block = bl; *gmult = gm; found = true; }
+        if (key == k && (bl == 128 || bl == 256) && (gm == 0 || gm == 2 || gm == 4 || gm == 8 || gm == 16)) {
+            *block = bl;
+            *gmult = gm;
+            found = true;
+        }
     fclose(f);
     return found;
 }
@@ -158,81 +181,72 @@ static void tune_store(const std::string &key, int block, int gmult)
     fclose(f);
 }
 
-// Task slots per tile: heat tiles hold two light tasks per thread (as in the
-// single-problem kernel), elasticity tiles one.
-static int batch_slots(int kind, int block) { return kind == 0 ? 2 * block : block; }
+// Task slots per tile: heat tiles hold two light tasks per thread,
+// elasticity tiles one.
+static int batch_slots(int kind, int block) { return KS[kind].heat ? 2 * block : block; }
 
-static ReuseArgs kind_args(KindData &k, int bi, double *vals, double *F)
-{
-    Pattern &P = k.tab[bi];
-    ReuseArgs A;
-    A.tiles = (const int4 *)P.tile.p;
-    A.tile_rows = (const int4 *)P.tile_rows.p;
-    A.task_elem = (const int32_t *)P.task_elem.p;
-    A.task_elem8 = P.elem8 ? (const uint8_t *)P.task_elem.p : nullptr;
-    A.tile_elem = (const int32_t *)P.tile_elem.p;
-    A.maxu = P.maxu;
-    A.task_info = (const uint32_t *)P.task_info.p;
-    A.task_pos = (const uint8_t *)P.task_pos.p;
-    A.ncolp = P.ncolmax;
-    A.ntiles = (int)P.ntiles;
-    A.geom = nullptr;  // the kernel gathers conn and xy itself
-    A.conn = (const int32_t *)k.conn.p;
-    A.xy = (const double2 *)k.xy.p;
-    A.ngap = A.xgap = 0;  // the concatenated arrays are stored plain
-    A.vals = vals;
-    A.F = F;
-    return A;
-}
-
-// The kernel arguments of workgroup size block (index bi), written to device
-// memory once per build.
-static int batch_upload_args(elfel_hip_ctx *c, Batch *b, int bi)
-{
-    KindData &H = b->kd[0], &Ev = b->kd[1];
-    BatchArgs a{};
-    a.ch.kappa = b->pb_heat.kappa; a.ch.Q = b->pb_heat.Q; a.ch.mu = b->pb_heat.mu;
-    a.ch.kappa_e = H.coef.p ? (const double *)H.coef.p : nullptr;
-    a.ce.mu = b->pb_el.mu;
-    for (int i = 0; i < 9; i++) { a.ch.D[i] = b->pb_heat.D[i]; a.ce.D[i] = b->pb_el.D[i]; }
-    a.ce.scale_e = Ev.coef.p ? (const double *)Ev.coef.p : nullptr;
-    a.H = kind_args(H, bi, (double *)b->val.p, (double *)b->F.p);
-    a.Ev = kind_args(Ev, bi, (double *)b->val.p, nullptr);
-    if (!H.E) a.H.ntiles = 0;
-    if (!Ev.E) a.Ev.ntiles = 0;
-    int rc = ensure(c, b->args[bi], sizeof(BatchArgs));
-    if (rc) return rc;
-    HIPCHK(c, hipMemcpy(b->args[bi].p, &a, sizeof(BatchArgs), hipMemcpyHostToDevice));
-    return 0;
-}
-
-// One batched assembly on the stream (no synchronisation).
+// One batched assembly on the stream (no synchronisation).  Tile ranges:
+// heat T3, then heat Q4, then elasticity (k_reuse_batch).
 static int batch_launch(elfel_hip_ctx *c, Batch *b, int block, int grid)
 {
     const int bi = block == 128 ? 0 : 1;
-    KindData &H = b->kd[0], &Ev = b->kd[1];
-    const int nH = H.E ? (int)H.tab[bi].ntiles : 0, nE = Ev.E ? (int)Ev.tab[bi].ntiles : 0;
-    const int nt = nH + nE;
-    if (nt == 0) return 0;
-    const BatchArgs *args = (const BatchArgs *)b->args[bi].p;
-    int lds_vals = std::max(H.E ? lds_span((int)H.tab[bi].lds_vals) : 0, Ev.E ? lds_span((int)Ev.tab[bi].lds_vals) : 0);
-    if (H.E)  // heat-T3 element records (floats) alias the value sums
-        lds_vals = std::max(lds_vals, (H.tab[bi].maxu * heat_rec<3>() * 4 + 7) / 8);
-    lds_vals = (lds_vals + 1) & ~1;  // even: the load-vector sums stay 16-byte aligned
-    // + the heat tiles' load-vector sums (up to REUSE_MAX_TILE_ROWS rows)
-    const size_t lds = sizeof(double) * ((size_t)lds_vals + REUSE_MAX_TILE_ROWS);
-    const int g = grid > 0 ? std::min(grid, nt) : nt;
-    const bool csc = b->layout == ELFEL_CSC;
-    if (block == 128) {
-        if (csc) hipLaunchKernelGGL((k_reuse_batch<true, 128>), dim3(g), dim3(128), lds, c->stream, args, nH, nt, lds_vals);
-        else hipLaunchKernelGGL((k_reuse_batch<false, 128>), dim3(g), dim3(128), lds, c->stream, args, nH, nt, lds_vals);
-    } else {
-        if (csc) hipLaunchKernelGGL((k_reuse_batch<true, 256>), dim3(g), dim3(256), lds, c->stream, args, nH, nt, lds_vals);
-        else hipLaunchKernelGGL((k_reuse_batch<false, 256>), dim3(g), dim3(256), lds, c->stream, args, nH, nt, lds_vals);
+    BatchLaunch L{};
+    int nk[BK_N];
+    int lds_vals = 0;
+    for (int k = 0; k < BK_N; k++) {
+        KindData &K = b->kd[k];
+        const Pattern &P = K.tab[bi];
+        nk[k] = K.E ? (int)P.ntiles : 0;
+        ReuseArgs &A = L.A[k];
+        A = ReuseArgs{};
+        A.tiles = (const int4 *)P.tile.p;
+        A.tile_rows = (const int4 *)P.tile_rows.p;
+        A.task_elem = (const int32_t *)P.task_elem.p;
+        A.task_elem8 = P.elem8 ? (const uint8_t *)P.task_elem.p : nullptr;
+        A.tile_elem = (const int32_t *)P.tile_elem.p;
+        A.maxu = P.maxu;
+        A.task_info = (const uint32_t *)P.task_info.p;
+        A.task_pos = (const uint8_t *)P.task_pos.p;
+        A.ncolp = P.ncolmax;
+        A.ntiles = nk[k];
+        A.geom = nullptr;  // the kernel gathers conn and xy itself
+        A.conn = (const int32_t *)K.conn.p;
+        A.xy = (const double2 *)K.xy.p;
+        A.ngap = A.xgap = 0;  // the concatenated arrays are stored plain
+        A.vals = (double *)b->val.p;
+        A.F = KS[k].heat ? (double *)b->F.p : nullptr;
+        Coef &cf = L.cf[k];
+        cf = Coef{};
+        cf.kappa = b->pb[k].kappa; cf.Q = b->pb[k].Q; cf.mu = b->pb[k].mu;
+        for (int i = 0; i < 9; i++) cf.D[i] = b->pb[k].D[i];
+        const double *ce = K.coef.p ? (const double *)K.coef.p : nullptr;
+        if (KS[k].heat) cf.kappa_e = ce;
+        else cf.scale_e = ce;
+        if (!nk[k]) continue;
+        lds_vals = std::max(lds_vals, lds_span((int)P.lds_vals));
+        if (KS[k].heat) {  // element records alias the value sums
+            const size_t rb = (size_t)P.maxu * (k == BK_HEAT_T3 ? heat_rec<3>() : heat_rec<4>()) *
+                              (b->precision == ELFEL_FP64 ? 8 : 4);
+            lds_vals = std::max(lds_vals, (int)((rb + 7) / 8));
+        }
     }
-    HIPCHK(c, hipGetLastError());
+    L.n0 = nk[BK_HEAT_T3];
+    L.n1 = L.n0 + nk[BK_HEAT_Q4];
+    L.nt = L.n1 + nk[BK_ELAST_T6];
+    if (L.nt == 0) return 0;
+    lds_vals = (lds_vals + 1) & ~1;  // even: the load-vector sums stay 16-byte aligned
+    L.lds_vals = lds_vals;
+    // + the heat tiles' load-vector sums (up to REUSE_MAX_TILE_ROWS rows)
+    L.lds = sizeof(double) * ((size_t)lds_vals + REUSE_MAX_TILE_ROWS);
+    L.grid = grid > 0 ? std::min(grid, L.nt) : L.nt;
+    L.block = block;
+    L.csc = b->layout == ELFEL_CSC;
+    const int rc = b->precision == ELFEL_FP64 ? launch_batch64(c->stream, L) : launch_batch32(c->stream, L);
+    if (rc) return set_err(c, ELFEL_EHIP, std::string("batched launch: ") + hipGetErrorString(hipGetLastError()));
     return 0;
 }
+
+int launch_batch32(hipStream_t st, const BatchLaunch &L) { return launch_batch_t<float>(st, L); }
 
 static int time_variant(elfel_hip_ctx *c, Batch *b, int block, int grid, double *ms)
 {
@@ -267,12 +281,12 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
     std::vector<ReuseTables> tabs[2];
     tabs[0].resize(b->probs.size());
     tabs[1].resize(b->probs.size());
-    int64_t koff_E[2] = {0, 0}, koff_N[2] = {0, 0};
+    int64_t koff_E[BK_N] = {0, 0, 0}, koff_N[BK_N] = {0, 0, 0};
     int64_t val_base = 0, ptr_base = 0, row_base = 0;
     for (size_t k = 0; k < b->probs.size(); k++) {
         BatchProb &p = b->probs[k];
-        const elfel_hip_problem &pb = p.kind == 0 ? b->pb_heat : b->pb_el;
-        const int nen = p.kind == 0 ? 3 : 6, nc = p.kind == 0 ? 1 : 2;
+        const elfel_hip_problem &pb = b->pb[p.kind];
+        const int nen = KS[p.kind].nen, nc = KS[p.kind].nc;
         int rc = elfel_hip_set_mesh(c, 0, nen, p.E, p.conn.data(), p.Nn, p.xy.data());
         if (!rc) rc = elfel_hip_set_dofs(c, 0, nc, p.Nn, p.dofs.data(), p.ndofs);
         if (rc) return rc;
@@ -296,7 +310,7 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
         p.row_base = row_base;
         val_base += p.nnz;
         ptr_base += p.ndofs + 1;
-        if (p.kind == 0) row_base += p.ndofs;
+        if (KS[p.kind].heat) row_base += p.ndofs;
     }
     b->total_nnz = val_base;
     b->total_ptr = ptr_base;
@@ -313,12 +327,12 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
         if (p.nnz)
             HIPCHK(c, hipMemcpy((int32_t *)b->ind.p + p.val_base, hinds[k].data(), 4 * p.nnz, hipMemcpyHostToDevice));
     }
-    for (int kind = 0; kind < 2; kind++) {
+    for (int kind = 0; kind < BK_N; kind++) {
         KindData &K = b->kd[kind];
         K.E = koff_E[kind];
         K.Nn = koff_N[kind];
         if (!K.E) continue;
-        const int nen = kind == 0 ? 3 : 6;
+        const int nen = KS[kind].nen;
         std::vector<int32_t> conn(K.E * nen);
         std::vector<double> xy(2 * K.Nn), coef;
         bool any_coef = false;
@@ -329,7 +343,7 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
             for (int64_t i = 0; i < p.E * nen; i++) conn[p.elem_base * nen + i] = (int32_t)(p.conn[i] - 1 + p.node_base);
             std::copy(p.xy.begin(), p.xy.end(), xy.begin() + 2 * p.node_base);
             if (any_coef) {
-                const double u = kind == 0 ? b->pb_heat.kappa : 1.0;
+                const double u = KS[kind].heat ? b->pb[kind].kappa : 1.0;
                 for (int64_t e = 0; e < p.E; e++) coef[p.elem_base + e] = p.has_coef ? p.coef[e] : u;
             }
         }
@@ -350,6 +364,7 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
                 const ReuseTables &t = tabs[bi][k];
                 const int task0 = (int)cat.elem.size();
                 cat.ncolp = t.ncolp;
+                cat.tstride = t.tstride;
                 cat.maxv = std::max(cat.maxv, t.maxv);
                 cat.maxrank = std::max(cat.maxrank, t.maxrank);
                 for (size_t i = 0; i < t.tiles.size(); i++) {
@@ -385,17 +400,23 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
     }
     if (b->total_nnz >= INT32_MAX || b->total_ptr >= INT32_MAX)
         return set_err(c, ELFEL_ENOTSUP, "batch beyond int32 value offsets");
-    for (int bi = 0; bi < 2; bi++)
-        if ((rc = batch_upload_args(c, b, bi))) return rc;
     b->built = true;
     // launch shape: cache or autotune
     hipDeviceProp_t prop;
     HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
     const int ncu = prop.multiProcessorCount;
-    char key[512];
-    snprintf(key, sizeof key, "%s:batch:heatT3r1x%d+elastT6r3x%d:%s:tiles%lld", prop.gcnArchName,
-             b->have_heat ? 1 : 0, b->have_el ? 1 : 0, b->layout == ELFEL_CSC ? "csc" : "csr",
-             (long long)((b->kd[0].ntiles + b->kd[1].ntiles) / 1024));
+    // the key names the batch exactly: precision, layout, and per kind the
+    // tile counts at both workgroup sizes and the element count; the matrix
+    // entries
+    std::string key = std::string(prop.gcnArchName) + ":batch:" + (b->precision == ELFEL_FP64 ? "fp64" : "mixed") +
+                      ":" + (b->layout == ELFEL_CSC ? "csc" : "csr");
+    for (int k = 0; k < BK_N; k++) {
+        char part[160];
+        snprintf(part, sizeof part, ":%s.%lld.%lld.%lld", KS[k].name, (long long)b->kd[k].tab[0].ntiles,
+                 (long long)b->kd[k].tab[1].ntiles, (long long)b->kd[k].E);
+        key += part;
+    }
+    key += ":nnz" + std::to_string((long long)b->total_nnz);
     int block = 256, gmult = 0;
     bool cached = tune_lookup(key, &block, &gmult);
     if (autotune || !cached) {
@@ -491,7 +512,7 @@ extern "C" int elfel_hip_batch_copy(elfel_hip_ctx *c, int32_t k, int base, int64
     }
     if (val && p.nnz)
         HIPCHK(c, hipMemcpyAsync(val, (double *)b->val.p + p.val_base, 8 * p.nnz, hipMemcpyDeviceToHost, c->stream));
-    if (F && p.kind == 0)
+    if (F && KS[p.kind].heat)
         HIPCHK(c, hipMemcpyAsync(F, (double *)b->F.p + p.row_base, 8 * p.ndofs, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return ELFEL_OK;
@@ -504,9 +525,9 @@ extern "C" int elfel_hip_batch_stats(elfel_hip_ctx *c, int64_t *nprob, int64_t *
     Batch *b = (Batch *)c->batch;
     if (!b || !b->built) return set_err(c, ELFEL_ESTATE, "build the batch first");
     if (nprob) *nprob = (int64_t)b->probs.size();
-    if (nelem) *nelem = b->kd[0].E + b->kd[1].E;
+    if (nelem) *nelem = b->kd[0].E + b->kd[1].E + b->kd[2].E;
     if (nnz) *nnz = b->total_nnz;
-    if (ntiles) *ntiles = b->kd[0].ntiles + b->kd[1].ntiles;
+    if (ntiles) *ntiles = b->kd[0].ntiles + b->kd[1].ntiles + b->kd[2].ntiles;
     if (tuned_ms) *tuned_ms = b->tuned_ms;
     return ELFEL_OK;
 }

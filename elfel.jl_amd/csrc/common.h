@@ -109,6 +109,10 @@ struct Pattern {
     // repeats a dof): assemblies of this pattern run the plain FP64 method
     bool fallback = false;
     std::string fallback_note;
+    // row-owner heat tiles: one record of rw words per row (rw 0: task tables),
+    // tile_elem with tstride entries per tile
+    Buf rrec;
+    int rw = 0, tstride = 0;
 };
 
 struct Timing {

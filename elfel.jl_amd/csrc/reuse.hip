@@ -91,6 +91,7 @@ struct TileInput {
     bool stokes, staged, by_rank;
     const int32_t *seq;  // patch tiles: the rows in packing order (null: dof order)
     int nen, nc, ncol, ncolp, cap, ucap, vcap;
+    int rw, kmax;  // row-owner heat tiles: words per row record, most elements per row (rw 0: task tables)
     int64_t n;
     const Field *F0, *F1;
     const Adj *A0, *A1;
@@ -113,6 +114,8 @@ int tile_range(const TileInput &I, int64_t r0, int64_t r1, ReuseTables &T, std::
     const int64_t n = I.n;
     T = ReuseTables{};
     T.ncolp = ncolp;
+    T.rw = I.rw;
+    T.tstride = I.rw ? HEAT_ROWS_BLOCK : cap;
     int4 cur = {0, 0, 0, 0}, cr = {0, 0, 0, 0};
     bool open = false;
     int nu_open = 0;
@@ -121,6 +124,9 @@ int tile_range(const TileInput &I, int64_t r0, int64_t r1, ReuseTables &T, std::
     std::vector<uint8_t> tp;
     // patch tiles: the open tile's rows in insertion order (row, values)
     std::vector<std::pair<int32_t, int>> orows;
+    // row-owner records of the open tile: task of (row, rank), counts, LDS offsets
+    std::vector<int> rrow, rcnt, rvb;
+    bool row_err = false;
     std::vector<int> ord, newidx, newvb;
     auto close = [&]() {
         if (!open) return;
@@ -185,7 +191,45 @@ int tile_range(const TileInput &I, int64_t r0, int64_t r1, ReuseTables &T, std::
             cr.w = (int)te.size();
             T.maxu = std::max(T.maxu, cr.w);
             T.telem.insert(T.telem.end(), te.begin(), te.end());
-            T.telem.insert(T.telem.end(), (size_t)cap - te.size(), te.empty() ? 0 : te[0]);
+            T.telem.insert(T.telem.end(), (size_t)T.tstride - te.size(), te.empty() ? 0 : te[0]);
+        }
+        if (I.rw) {
+            // one record per tile-local row: its tasks in rank (element) order
+            const size_t t0 = (size_t)cur.x, nt = (size_t)cur.y;
+            const int nrw = cr.y;
+            rrow.assign((size_t)nrw * I.kmax, -1);
+            rcnt.assign((size_t)nrw, 0);
+            rvb.assign((size_t)nrw, 0);
+            for (size_t k = 0; k < nt; k++) {
+                const uint32_t in = T.info[t0 + k];
+                const int row = (int)(in >> 24), rank = (int)((in >> 3) & 15);
+                if (row >= nrw || rank >= I.kmax) { err = "internal: row record"; row_err = true; return; }
+                rrow[(size_t)row * I.kmax + rank] = (int)k;
+                rcnt[row] = std::max(rcnt[row], rank + 1);
+                rvb[row] = (int)((in >> 10) & 0x3FFF);
+            }
+            const size_t base = T.rrec.size();
+            T.rrec.resize(base + (size_t)HEAT_ROWS_BLOCK * I.rw, 0u);
+            uint32_t *rec = T.rrec.data() + base;
+            const int fb = 10 + 4 * I.nen;
+            for (int r = 0; r < nrw; r++) {
+                uint32_t *w = rec + (size_t)r * I.rw;
+                auto put = [&](int o, uint32_t v, int nb) {
+                    for (int b = 0; b < nb; b++)
+                        if ((v >> b) & 1u) w[(o + b) >> 5] |= 1u << ((o + b) & 31);
+                };
+                put(0, (uint32_t)rcnt[r], 4);
+                put(4, (uint32_t)rvb[r], 13);
+                for (int k = 0; k < rcnt[r]; k++) {
+                    const int q = rrow[(size_t)r * I.kmax + k];
+                    if (q < 0) { err = "internal: row record gap"; row_err = true; return; }
+                    const uint32_t in = T.info[t0 + q];
+                    const int o = 17 + k * fb;
+                    put(o, (uint32_t)T.elem[t0 + q], 8);
+                    put(o + 8, in & 7u, 2);
+                    for (int j = 0; j < I.nen; j++) put(o + 10 + 4 * j, T.pos[(t0 + q) * ncolp + j] & 0x7Fu, 4);
+                }
+            }
         }
         // pad to cap tasks, so that tile t's tasks start at t * cap (the kernel
         // computes a thread's task index without reading the tile first)
@@ -230,6 +274,7 @@ int tile_range(const TileInput &I, int64_t r0, int64_t r1, ReuseTables &T, std::
         if (open && (cur.y + deg > cap || cur.w + nr * L > vcap || cr.y + nr > std::min(cap, REUSE_MAX_TILE_ROWS) ||
                      nu_open + newu > ucap)) {
             close();
+            if (row_err) return ELFEL_EINVAL;
             newu = deg;  // a node's elements are distinct
         }
         if (!open) {
@@ -289,6 +334,7 @@ int tile_range(const TileInput &I, int64_t r0, int64_t r1, ReuseTables &T, std::
         k += nr;
     }
     close();
+    if (row_err) return ELFEL_EINVAL;
     return 0;
 }
 
@@ -352,7 +398,7 @@ void band_order(const std::vector<double2> &xy, std::vector<int32_t> &order)
 // parallel; a tile never spans two ranges.
 int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1, int64_t n,
                 const std::vector<int32_t> &hptr, const std::vector<int32_t> &hind, int cap, int ucap,
-                ReuseTables &T, bool patch)
+                ReuseTables &T, bool patch, bool rows)
 {
     const int nen = F0.nen, nc = F0.ncomp, nd = nen * nc;
     const int ncol = nd + (stokes ? 3 : 0);
@@ -403,6 +449,25 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
             inv_fc[d] = (uint8_t)(1 << 4);
         }
     I.hptr = &hptr; I.hind = &hind; I.inv_node = &inv_node; I.inv_fc = &inv_fc;
+    // row-owner heat tiles (T3 and Q4): when every node lies in at most kmax
+    // elements and every row has at most 16 entries, each row gets one record
+    // (heat_rows_tile) in place of its tasks' tables; the tiles hold up to
+    // HEAT_ROWS_BLOCK rows and as many distinct elements
+    I.rw = I.kmax = 0;
+    if (rows && I.staged && (nen == 3 || nen == 4)) {
+        const int kmax = nen == 4 ? hrow_kmax<4>() : hrow_kmax<3>();
+        int maxdeg = 0;
+        for (int64_t m = 0; m < F0.Nn; m++) maxdeg = std::max(maxdeg, A0.ptr[m + 1] - A0.ptr[m]);
+        int maxl = 0;
+        for (int64_t r = 0; r < n; r++) maxl = std::max(maxl, hptr[r + 1] - hptr[r]);
+        if (maxdeg <= kmax && maxl <= 16) {
+            I.kmax = kmax;
+            I.rw = nen == 4 ? hrow_words<4, hrow_kmax<4>()>() : hrow_words<3, hrow_kmax<3>()>();
+            I.cap = HEAT_ROWS_BLOCK * kmax;
+            I.ucap = HEAT_ROWS_BLOCK;
+            I.vcap = REUSE_VCAP_PER_THREAD * HEAT_ROWS_BLOCK;
+        }
+    }
     // patch tiles for the element-staged (scalar) problems of 2^18 rows or
     // more: rows packed in a locality order of their nodes
     std::vector<int32_t> seq;
@@ -456,6 +521,8 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     // concatenation: tile task offsets become global
     T = ReuseTables{};
     T.ncolp = I.ncolp;
+    T.rw = I.rw;
+    T.tstride = I.rw ? HEAT_ROWS_BLOCK : I.cap;
     size_t ntiles = 0, nslots = 0, ntel = 0, nruns = 0;
     for (const ReuseTables &p : part) {
         ntiles += p.tiles.size(); nslots += p.elem.size(); ntel += p.telem.size(); nruns += p.runs.size();
@@ -471,6 +538,7 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
         T.info.insert(T.info.end(), p.info.begin(), p.info.end());
         T.pos.insert(T.pos.end(), p.pos.begin(), p.pos.end());
         T.telem.insert(T.telem.end(), p.telem.begin(), p.telem.end());
+        T.rrec.insert(T.rrec.end(), p.rrec.begin(), p.rrec.end());
         const int roff = (int)T.runs.size();
         for (int2 tr : p.truns) { tr.x += roff; T.truns.push_back(tr); }
         T.runs.insert(T.runs.end(), p.runs.begin(), p.runs.end());
@@ -528,8 +596,9 @@ void order_tiles(ReuseTables &T, int nxcd, int cap, int run)
     std::vector<uint32_t> info(T.info.size());
     std::vector<uint8_t> pos(T.pos.size());
     std::vector<int32_t> telem(T.telem.size());
+    std::vector<uint32_t> rrec(T.rrec.size());
     std::vector<int2> truns(T.truns.size());
-    const size_t ncolp = (size_t)T.ncolp;
+    const size_t ncolp = (size_t)T.ncolp, ts = (size_t)T.tstride, rs = (size_t)HEAT_ROWS_BLOCK * (size_t)T.rw;
     for (size_t b = 0; b < nt; b++) {
         const size_t o = (size_t)key[src[b]].second;
         tiles[b] = T.tiles[o];
@@ -540,7 +609,9 @@ void order_tiles(ReuseTables &T, int nxcd, int cap, int run)
         std::copy(T.info.begin() + o * cap, T.info.begin() + (o + 1) * cap, info.begin() + b * cap);
         std::copy(T.pos.begin() + o * cap * ncolp, T.pos.begin() + (o + 1) * cap * ncolp, pos.begin() + b * cap * ncolp);
         if (!T.telem.empty())
-            std::copy(T.telem.begin() + o * cap, T.telem.begin() + (o + 1) * cap, telem.begin() + b * cap);
+            std::copy(T.telem.begin() + o * ts, T.telem.begin() + (o + 1) * ts, telem.begin() + b * ts);
+        if (!T.rrec.empty())
+            std::copy(T.rrec.begin() + o * rs, T.rrec.begin() + (o + 1) * rs, rrec.begin() + b * rs);
     }
     T.tiles.swap(tiles);
     T.trows.swap(trows);
@@ -548,11 +619,51 @@ void order_tiles(ReuseTables &T, int nxcd, int cap, int run)
     T.info.swap(info);
     T.pos.swap(pos);
     T.telem.swap(telem);
+    T.rrec.swap(rrec);
     T.truns.swap(truns);
 }
 
 int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P)
 {
+    P.rw = T.rw;
+    P.tstride = T.tstride;
+    if (T.rw) {
+        // row-owner heat tiles: the row records and the tiles' element lists
+        // replace the task tables
+        int rc;
+        if ((rc = ensure(c, P.tile, sizeof(int4) * std::max<size_t>(T.tiles.size(), 1))) ||
+            (rc = ensure(c, P.tile_rows, sizeof(int4) * std::max<size_t>(T.trows.size(), 1))) ||
+            (rc = ensure(c, P.rrec, 4 * std::max<size_t>(T.rrec.size(), 1))) ||
+            (rc = ensure(c, P.tile_elem, 4 * std::max<size_t>(T.telem.size(), 1))))
+            return rc;
+        P.has_runs = !T.truns.empty();
+        if (P.has_runs) {
+            if ((rc = ensure(c, P.truns, sizeof(int2) * T.truns.size())) ||
+                (rc = ensure(c, P.runs, sizeof(int4) * std::max<size_t>(T.runs.size(), 1))))
+                return rc;
+            HIPCHK(c, hipMemcpy(P.truns.p, T.truns.data(), sizeof(int2) * T.truns.size(), hipMemcpyHostToDevice));
+            if (!T.runs.empty())
+                HIPCHK(c, hipMemcpy(P.runs.p, T.runs.data(), sizeof(int4) * T.runs.size(), hipMemcpyHostToDevice));
+        }
+        if (!T.tiles.empty()) {
+            HIPCHK(c, hipMemcpy(P.tile.p, T.tiles.data(), sizeof(int4) * T.tiles.size(), hipMemcpyHostToDevice));
+            HIPCHK(c, hipMemcpy(P.tile_rows.p, T.trows.data(), sizeof(int4) * T.trows.size(), hipMemcpyHostToDevice));
+        }
+        if (!T.rrec.empty())
+            HIPCHK(c, hipMemcpy(P.rrec.p, T.rrec.data(), 4 * T.rrec.size(), hipMemcpyHostToDevice));
+        if (!T.telem.empty())
+            HIPCHK(c, hipMemcpy(P.tile_elem.p, T.telem.data(), 4 * T.telem.size(), hipMemcpyHostToDevice));
+        release(P.task_elem); release(P.task_info); release(P.task_pos);
+        P.elem8 = false;
+        P.ntiles = (int64_t)T.tiles.size();
+        P.ntasks = (int64_t)T.elem.size();
+        P.ncolmax = T.ncolp;
+        P.maxrank = T.maxrank;
+        P.lds_vals = T.maxv;
+        P.maxu = T.maxu;
+        return 0;
+    }
+    release(P.rrec);
     const int64_t ntasks = (int64_t)T.elem.size();
     int rc;
     if ((rc = ensure(c, P.tile, sizeof(int4) * std::max<size_t>(T.tiles.size(), 1))) ||
@@ -623,7 +734,8 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
     ReuseTables T;
     const int cap = block * reuse_tpt(pb->kind);  // task slots per tile
     P.fallback = false;
-    rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, cap, block, T, env_int("ELFEL_HIP_PATCH_TILES", 1) != 0);
+    rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, cap, block, T, env_int("ELFEL_HIP_PATCH_TILES", 1) != 0,
+                     pb->kind == ELFEL_HEAT && env_int("ELFEL_HIP_HEAT_ROWS", 1) != 0);
     if (rc == ELFEL_ENOTSUP) {
         // the mesh does not fit the tiles: this pattern's assemblies run the
         // plain FP64 method, whose result the pattern build just produced
@@ -666,9 +778,9 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
                        P.nrow == c->nrow;
     const int block = env_int("ELFEL_HIP_REUSE_BLOCK", 256) == 128 ? 128 : 256;
     const bool mixed = pb->precision == ELFEL_MIXED;
-    reuse_launch_fn fn = mixed ? pick_reuse32(pb->kind, pb->fe, tab.nq, pb->layout == ELFEL_CSC, block)
-                               : pick_reuse64(pb->kind, pb->fe, tab.nq, pb->layout == ELFEL_CSC, block);
-    if (!fn) return set_err(c, ELFEL_ENOTSUP, "unsupported (kind, element, rule) for the reuse method");
+    const int csc = pb->layout == ELFEL_CSC;
+    if (!(mixed ? pick_reuse32(pb->kind, pb->fe, tab.nq, csc, block) : pick_reuse64(pb->kind, pb->fe, tab.nq, csc, block)))
+        return set_err(c, ELFEL_ENOTSUP, "unsupported (kind, element, rule) for the reuse method");
     int rc;
     if (!fresh || P.block != block * reuse_tpt(pb->kind)) {
         hipEvent_t ev = nullptr;
@@ -687,6 +799,10 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
         c->nnz = -1;
         return plain_assemble(c, &p2, tab, cf);
     }
+    reuse_launch_fn fn = P.rw ? (mixed ? pick_heat_rows32(pb->fe, tab.nq, csc) : pick_heat_rows64(pb->fe, tab.nq, csc))
+                              : (mixed ? pick_reuse32(pb->kind, pb->fe, tab.nq, csc, block)
+                                       : pick_reuse64(pb->kind, pb->fe, tab.nq, csc, block));
+    if (!fn) return set_err(c, ELFEL_ENOTSUP, "unsupported (kind, element, rule) for the reuse method");
     const int64_t E = F0.E;
     const int nen = F0.nen;
     if ((rc = ensure(c, c->geom, (size_t)E * nen * 2 * 8)) ||
@@ -727,6 +843,8 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
         A.tile_runs = (const int2 *)P.truns.p;
         A.runs = (const int4 *)P.runs.p;
     }
+    A.rrec = P.rw ? (const uint32_t *)P.rrec.p : nullptr;
+    A.tstride = P.tstride;
     tm_begin(c, ELFEL_PH_NUMERIC, &ev);
     rc = fn(c, A, cf, tab, P.lds_vals);
     if (rc) return rc;

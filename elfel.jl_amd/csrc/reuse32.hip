@@ -5,6 +5,7 @@
 
 static reuse_launch_fn pick128(int kind, int fe, int nq, int csc) { REUSE_PICK_BODY(float, 128) }
 static reuse_launch_fn pick256(int kind, int fe, int nq, int csc) { REUSE_PICK_BODY(float, 256) }
+reuse_launch_fn pick_heat_rows32(int fe, int nq, int csc) { HEAT_ROWS_PICK_BODY(float) }
 reuse_launch_fn pick_reuse32(int kind, int fe, int nq, int csc, int block)
 {
     return block == 128 ? pick128(kind, fe, nq, csc) : pick256(kind, fe, nq, csc);

@@ -65,6 +65,10 @@ struct ReuseArgs {
     // heat (element-staged tiles): the task's element as its index in the
     // tile's list (< 256), one byte per task
     const uint8_t *task_elem8 = nullptr;
+    // row-owner heat tiles (heat_rows_tile): one record of RW words per row,
+    // BLOCK records per tile; tile_elem holds tstride entries per tile
+    const uint32_t *rrec = nullptr;
+    int tstride = 0;
 };
 
 // The owner's entries of one element block.  The owner node a and its
@@ -401,6 +405,60 @@ __device__ __forceinline__ void heat_tile(const ReuseArgs &A, const Coef &cf, co
     // the caller's barrier separates these reads from the sums that overwrite the records
 }
 
+// Coalesced write of a tile's values, 16 bytes per lane (the pair of
+// non-temporal stores merges into one): an odd first (or last) value of the
+// range goes out alone.  Non-temporal stores stream the values past the
+// caches -- nothing in this launch reads them again -- and cut the kernel
+// time by a quarter on MI355X.  HEAT: also the tile's load-vector sums accF.
+template <bool HEAT, int BLOCK>
+__device__ __forceinline__ void tile_writeout(const ReuseArgs &A, const double *acc, const double *accF, int tile,
+                                              int4 td, int4 tr)
+{
+    const int tid = threadIdx.x;
+    if (A.runs) {
+        // patch tile: the LDS values are the tile's rows in dof order; each run
+        // of consecutive rows goes out as one contiguous range, a wave per run
+        const int2 rp = A.tile_runs[tile];
+        const int wave = tid >> 6, lane = tid & 63;
+        for (int k = wave; k < rp.y; k += BLOCK / 64) {
+            const int4 run = A.runs[rp.x + k];
+            const int lv0 = run.z & 0xFFFF, nv = run.z >> 16;
+            double *rd = A.vals + run.x;
+            const int rhead = run.x & 1;
+            if (lane == 0) {
+                if (rhead && nv > 0) __builtin_nontemporal_store(acc[lds_slot(lv0)], &rd[0]);
+                if (((nv - rhead) & 1) && nv > rhead) __builtin_nontemporal_store(acc[lds_slot(lv0 + nv - 1)], &rd[nv - 1]);
+            }
+            const int np = (nv - rhead) >> 1;
+            for (int p = lane; p < np; p += 64) {
+                const int v = rhead + 2 * p;
+                __builtin_nontemporal_store(acc[lds_slot(lv0 + v)], &rd[v]);
+                __builtin_nontemporal_store(acc[lds_slot(lv0 + v + 1)], &rd[v + 1]);
+            }
+            if constexpr (HEAT) {
+                const int lr0 = run.w & 0xFFFF, nr = run.w >> 16;
+                for (int i = lane; i < nr; i += 64) A.F[run.y + i] = accF[lr0 + i];
+            }
+        }
+        return;
+    }
+    double *dst = A.vals + td.z;
+    const int head = td.z & 1;
+    const int npair = (td.w - head) >> 1;
+    if (tid == 0) {
+        if (head && td.w > 0) __builtin_nontemporal_store(acc[lds_slot(0)], &dst[0]);
+        if (((td.w - head) & 1) && td.w > head) __builtin_nontemporal_store(acc[lds_slot(td.w - 1)], &dst[td.w - 1]);
+    }
+#pragma unroll 4
+    for (int p = tid; p < npair; p += BLOCK) {
+        const int v = head + 2 * p;
+        __builtin_nontemporal_store(acc[lds_slot(v)], &dst[v]);
+        __builtin_nontemporal_store(acc[lds_slot(v + 1)], &dst[v + 1]);
+    }
+    if constexpr (HEAT)
+        for (int v = tid; v < tr.y; v += BLOCK) A.F[tr.x + v] = accF[v];
+}
+
 // One tile: a workgroup of BLOCK threads, TPT tasks per thread (task slot
 // u * BLOCK + tid).  The loads of all of a thread's tasks are issued stage by
 // stage (task records, then node numbers and coefficients, then coordinates)
@@ -574,54 +632,151 @@ __device__ __forceinline__ void reuse_tile(const ReuseArgs &A, const Coef &cf, c
         }
         __syncthreads();
     }
-    // Coalesced write of the tile's values, 16 bytes per lane (the pair of
-    // non-temporal stores merges into one): an odd first (or last) value of the
-    // range goes out alone.  Non-temporal stores stream the values past the
-    // caches -- nothing in this launch reads them again -- and cut the kernel
-    // time by a quarter on MI355X.
-    if (A.runs) {
-        // patch tile: the LDS values are the tile's rows in dof order; each run
-        // of consecutive rows goes out as one contiguous range, a wave per run
-        const int2 rp = A.tile_runs[tile];
-        const int wave = tid >> 6, lane = tid & 63;
-        for (int k = wave; k < rp.y; k += BLOCK / 64) {
-            const int4 run = A.runs[rp.x + k];
-            const int lv0 = run.z & 0xFFFF, nv = run.z >> 16;
-            double *rd = A.vals + run.x;
-            const int rhead = run.x & 1;
-            if (lane == 0) {
-                if (rhead && nv > 0) __builtin_nontemporal_store(acc[lds_slot(lv0)], &rd[0]);
-                if (((nv - rhead) & 1) && nv > rhead) __builtin_nontemporal_store(acc[lds_slot(lv0 + nv - 1)], &rd[nv - 1]);
+    tile_writeout<KIND == ELFEL_HEAT, BLOCK>(A, acc, accF, tile, td, tr);
+}
+
+// ---------------------------------------------------------------------------
+// Row-owner heat tiles.  The tile's distinct elements are computed as in
+// heat_tile (thread t: element t of the tile's list, its whole record into
+// LDS); then thread t owns tile-local row t and adds that row's
+// contributions -- its elements in element order, within an element its NEN
+// columns -- into the row's LDS values.  A row's sums are one thread's, so no
+// rank steps and no barriers between contributions: each entry starts from
+// its first contribution and adds the rest in element order, the order of
+// sparse().  Per row one record of RW 32-bit words replaces the per-task
+// element, descriptor and position bytes:
+//   bits 0-3    count: elements of the row (<= KMAX)
+//   bits 4-16   LDS offset of the row's values in the tile
+//   then per element k < count, at bit 17 + k (10 + 4 NEN):
+//     8 bits slot (index in the tile's element list), 2 bits local node a of
+//     the row in the element, NEN x 4 bits the position of each element
+//     column in the row's sorted column list (rows of <= 16 entries)
+// ---------------------------------------------------------------------------
+template <int NEN>
+__host__ __device__ constexpr int hrow_field_bits() { return 10 + 4 * NEN; }
+template <int NEN, int KMAX>
+__host__ __device__ constexpr int hrow_words() { return (17 + KMAX * hrow_field_bits<NEN>() + 31) / 32; }
+// The largest element count per row of the record formats built here: Q4
+// rows (4 elements at interior nodes) fit 4 words, T3 rows 7 elements in 6.
+template <int NEN>
+__host__ __device__ constexpr int hrow_kmax() { return NEN == 4 ? 4 : 7; }
+
+// bits [o, o + w) of the record (o, w known at compile time after unrolling)
+template <int RW>
+__device__ __forceinline__ uint32_t hrow_bits(const uint32_t (&w)[RW], int o, int nb)
+{
+    const int i = o >> 5, sh = o & 31;
+    uint32_t v = w[i] >> sh;
+    if (sh + nb > 32 && i + 1 < RW) v |= w[i + 1] << (32 - sh);
+    return v & ((nb >= 32) ? 0xFFFFFFFFu : ((1u << nb) - 1u));
+}
+
+template <typename R, int NEN, int NQ, bool CSC, int BLOCK>
+__device__ __forceinline__ void heat_rows_tile(const ReuseArgs &A, const Coef &cf, const QTable &tab, double *lds,
+                                               int lds_vals, int tile)
+{
+    constexpr int KMAX = hrow_kmax<NEN>();
+    constexpr int RW = hrow_words<NEN, KMAX>();
+    constexpr int FB = hrow_field_bits<NEN>();
+    constexpr int RECN = heat_rec<NEN>();
+    double *acc = lds;
+    double *accF = lds + lds_vals;
+    R *rec = (R *)lds;
+    const int tid = threadIdx.x;
+    // the element number first (head of the chain element -> nodes ->
+    // coordinates), with the tile descriptors
+    const int32_t eg = A.tile_elem[(int64_t)tile * A.tstride + tid];
+    const int4 td = A.tiles[tile];
+    const int4 tr = A.tile_rows[tile];
+    const int nrows = tr.y, nU = tr.w;
+    uint32_t w[RW];
+#pragma unroll
+    for (int i = 0; i < RW; i++) w[i] = 0u;
+    if (tid < nrows) {
+        const uint32_t *rp = A.rrec + ((int64_t)tile * BLOCK + tid) * RW;
+        if constexpr (RW % 4 == 0) {
+#pragma unroll
+            for (int i = 0; i < RW; i += 4) {
+                const uint4 q = *(const uint4 *)(rp + i);
+                w[i] = q.x; w[i + 1] = q.y; w[i + 2] = q.z; w[i + 3] = q.w;
             }
-            const int np = (nv - rhead) >> 1;
-            for (int p = lane; p < np; p += 64) {
-                const int v = rhead + 2 * p;
-                __builtin_nontemporal_store(acc[lds_slot(lv0 + v)], &rd[v]);
-                __builtin_nontemporal_store(acc[lds_slot(lv0 + v + 1)], &rd[v + 1]);
-            }
-            if constexpr (KIND == ELFEL_HEAT) {
-                const int lr0 = run.w & 0xFFFF, nr = run.w >> 16;
-                for (int i = lane; i < nr; i += 64) A.F[run.y + i] = accF[lr0 + i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < RW; i += 2) {
+                const uint2 q = *(const uint2 *)(rp + i);
+                w[i] = q.x; w[i + 1] = q.y;
             }
         }
-        return;
     }
-    double *dst = A.vals + td.z;
-    const int head = td.z & 1;
-    const int npair = (td.w - head) >> 1;
-    if (tid == 0) {
-        if (head && td.w > 0) __builtin_nontemporal_store(acc[lds_slot(0)], &dst[0]);
-        if (((td.w - head) & 1) && td.w > head) __builtin_nontemporal_store(acc[lds_slot(td.w - 1)], &dst[td.w - 1]);
+    if (tid < nU) {
+        double kap = cf.kappa, Qv = cf.Q;
+        if (cf.kappa_e) kap = cf.kappa_e[padded(eg, cf.cgap)];
+        if (cf.Q_e) Qv = cf.Q_e[padded(eg, cf.cgap)];
+        int32_t nd[NEN];
+#pragma unroll
+        for (int k = 0; k < NEN; k++) nd[k] = A.conn[padded(eg, A.ngap) * NEN + k];
+        double X[NEN], Y[NEN];
+#pragma unroll
+        for (int k = 0; k < NEN; k++) { const double2 p = A.xy[padded(nd[k], A.xgap)]; X[k] = p.x; Y[k] = p.y; }
+        if constexpr (sizeof(R) == 4) {
+            const double x0 = X[0], y0 = Y[0];
+#pragma unroll
+            for (int k = 0; k < NEN; k++) { X[k] -= x0; Y[k] -= y0; }
+        }
+        heat_record<R, NEN, NQ>(tab, kap, Qv, X, Y, rec + (size_t)tid * RECN);
     }
-#pragma unroll 4
-    for (int p = tid; p < npair; p += BLOCK) {
-        const int v = head + 2 * p;
-        __builtin_nontemporal_store(acc[lds_slot(v)], &dst[v]);
-        __builtin_nontemporal_store(acc[lds_slot(v + 1)], &dst[v + 1]);
+    __syncthreads();
+    // the row's contributions into registers (the records are overwritten next)
+    const int cnt = (int)(w[0] & 15u);
+    const int vbase = (int)((w[0] >> 4) & 0x1FFFu);
+    R v[KMAX][NEN];
+    uint32_t pos[KMAX];  // NEN x 4 bits
+    double fsum = 0.0;   // start! zeros the load vector; the elements add in order
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        pos[k] = 0u;
+#pragma unroll
+        for (int j = 0; j < NEN; j++) v[k][j] = R(0);
+        if (k < cnt) {
+            const int o = 17 + k * FB;
+            const int slot = (int)hrow_bits<RW>(w, o, 8);
+            const int a = (int)hrow_bits<RW>(w, o + 8, 2);
+            pos[k] = hrow_bits<RW>(w, o + 10, 4 * NEN);
+            const R *r = rec + (size_t)slot * RECN;
+#pragma unroll
+            for (int j = 0; j < NEN; j++) v[k][j] = CSC ? r[j * NEN + a] : r[a * NEN + j];
+            fsum += (double)r[NEN * NEN + a];
+        }
     }
-    if constexpr (KIND == ELFEL_HEAT)
-        for (int v = tid; v < tr.y; v += BLOCK) A.F[tr.x + v] = accF[v];
+    __syncthreads();
+    if (tid < nrows) {
+        uint32_t seen = 0u;
+#pragma unroll
+        for (int k = 0; k < KMAX; k++) {
+            if (k >= cnt) break;
+#pragma unroll
+            for (int j = 0; j < NEN; j++) {
+                const int p = (int)((pos[k] >> (4 * j)) & 15u);
+                double *dst = &acc[lds_slot(vbase + p)];
+                if ((seen >> p) & 1u) *dst = *dst + (double)v[k][j];
+                else *dst = (double)v[k][j];
+                seen |= 1u << p;
+            }
+        }
+        accF[tid] = fsum;
+    }
+    __syncthreads();
+    tile_writeout<true, BLOCK>(A, acc, accF, tile, td, tr);
 }
+
+template <typename R, int NEN, int NQ, bool CSC, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_heat_rows(ReuseArgs A, Coef cf, QTable tab, int lds_vals)
+{
+    extern __shared__ double lds[];
+    heat_rows_tile<R, NEN, NQ, CSC, BLOCK>(A, cf, tab, lds, lds_vals, blockIdx.x);
+}
+
+#define HEAT_ROWS_BLOCK 256
 
 // Tasks per thread of the single-problem kernels: heat tasks are light (one
 // row of NEN columns), three per thread overlap their loads and make tiles of
@@ -639,30 +794,64 @@ void k_reuse(ReuseArgs A, Coef cf, QTable tab, int lds_vals)
     reuse_tile<KIND, R, NEN, NC, NQ, CSC, BLOCK, reuse_tpt(KIND)>(A, cf, tab, lds, lds_vals, blockIdx.x);
 }
 
-// Arguments of the batched kernel, kept in device memory: passed by value, the
-// two kinds' arguments (about 120 dwords) were loaded into scalar registers
-// at once, and 67 of them were spilled into vector-register lanes.
-struct BatchArgs {
-    ReuseArgs H, Ev;
-    Coef ch, ce;
-};
-
-// Batched problems of uneven size in one persistent launch: heat-T3 (1-point)
-// tiles first, then elasticity-T6 (3-point) tiles; each workgroup walks the
-// tile list with stride gridDim.x.  The tile kind is uniform per workgroup;
-// heat tiles hold two tasks per thread, elasticity tiles one.  The mixed
-// kernels take their quadrature tables from qt_fixed, so no table is passed.
-template <bool CSC, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_reuse_batch(const BatchArgs *__restrict__ B, int nH, int nt,
-                                                       int lds_vals)
+// Batched problems of uneven size in one persistent launch.  Three tile
+// kinds: heat T3 1-point (tiles 0 .. n0-1), heat Q4 2x2 (n0 .. n1-1) and
+// plane elasticity T6 3-point (n1 .. nt-1); each workgroup walks the tile list
+// with stride gridDim.x, and the tile kind is uniform per workgroup.  Heat
+// tiles hold two tasks per thread, elasticity tiles one.  R = float: the
+// mixed arithmetic; R = double: the FP64 arithmetic (built without
+// contraction, equal to the single-problem FP64 kernels bit for bit).  The
+// arguments are passed by value: kept in device memory and loaded at each
+// tile, they added a dependent load to every tile's start (256 problems:
+// 1.47 against 1.24 ms).  The quadrature tables are the compile-time ones.
+// KINDS: the tile kinds compiled in (1 heat T3, 2 heat Q4, 4 elasticity T6);
+// a kind left out takes no registers (with all three the mixed kernel needs
+// 129 VGPRs and drops to 3 waves per SIMD, heat T3 + elasticity T6 alone 127).
+template <typename R, bool CSC, int BLOCK, int KINDS>
+__global__ __launch_bounds__(BLOCK) void k_reuse_batch(ReuseArgs H3, ReuseArgs Ev, ReuseArgs H4, Coef ch3, Coef ce,
+                                                       Coef ch4, int n0, int n1, int nt, int lds_vals)
 {
     extern __shared__ double lds[];
-    constexpr QTable th = qt_fixed<3, 1>(), te = qt_fixed<6, 3>();
+    constexpr QTable t3 = qt_fixed<3, 1>(), t6 = qt_fixed<6, 3>(), t4 = qt_fixed<4, 4>();
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
-        if (t < nH) reuse_tile<ELFEL_HEAT, float, 3, 1, 1, CSC, BLOCK, 2>(B->H, B->ch, th, lds, lds_vals, t);
-        else reuse_tile<ELFEL_ELAST, float, 6, 2, 3, CSC, BLOCK, 1>(B->Ev, B->ce, te, lds, lds_vals, t - nH);
+        if (t < n0) {
+            if constexpr ((KINDS & 1) != 0) reuse_tile<ELFEL_HEAT, R, 3, 1, 1, CSC, BLOCK, 2>(H3, ch3, t3, lds, lds_vals, t);
+        } else if (t < n1) {
+            if constexpr ((KINDS & 2) != 0)
+                reuse_tile<ELFEL_HEAT, R, 4, 1, 4, CSC, BLOCK, 2>(H4, ch4, t4, lds, lds_vals, t - n0);
+        } else {
+            if constexpr ((KINDS & 4) != 0)
+                reuse_tile<ELFEL_ELAST, R, 6, 2, 3, CSC, BLOCK, 1>(Ev, ce, t6, lds, lds_vals, t - n1);
+        }
         __syncthreads();
     }
+}
+
+// Launch of the batched kernel (batch.hip: mixed, batch64.hip: FP64).
+struct BatchLaunch {
+    ReuseArgs A[3];  // heat T3, elasticity T6, heat Q4
+    Coef cf[3];
+    int n0, n1, nt, lds_vals, block, grid;
+    size_t lds;
+    bool csc;
+};
+int launch_batch32(hipStream_t st, const BatchLaunch &L);
+int launch_batch64(hipStream_t st, const BatchLaunch &L);
+
+// Instantiations: heat T3 + elasticity T6 (KINDS 5; also batches of one of
+// them), and all three kinds (7) for batches with heat Q4.
+template <typename R>
+int launch_batch_t(hipStream_t st, const BatchLaunch &L)
+{
+    const bool q4 = L.n1 > L.n0;
+#define KB(C, BL, K) hipLaunchKernelGGL((k_reuse_batch<R, C, BL, K>), dim3(L.grid), dim3(BL), L.lds, st, L.A[0], \
+                                        L.A[1], L.A[2], L.cf[0], L.cf[1], L.cf[2], L.n0, L.n1, L.nt, L.lds_vals)
+#define KB2(C, BL) do { if (q4) KB(C, BL, 7); else KB(C, BL, 5); } while (0)
+    if (L.block == 128) { if (L.csc) KB2(true, 128); else KB2(false, 128); }
+    else { if (L.csc) KB2(true, 256); else KB2(false, 256); }
+#undef KB2
+#undef KB
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // Geometry gather for the reuse kernels: per element, FP64 coordinates of its
@@ -695,13 +884,16 @@ struct ReuseTables {
     std::vector<uint32_t> info;
     std::vector<uint8_t> pos;
     std::vector<int32_t> telem;  // heat: distinct elements per tile (padded to the slot count)
+    std::vector<uint32_t> rrec;  // row-owner heat tiles: HEAT_ROWS_BLOCK records of rw words per tile
+    int rw = 0;                  // words per row record (0: task tables)
+    int tstride = 0;             // entries of telem per tile
     int ncolp = 0, maxv = 0, maxrank = 0, maxu = 0;
 };
 int pattern_to_host(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf,
                     std::vector<int32_t> &hptr, std::vector<int32_t> &hind);
 int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1, int64_t n,
                 const std::vector<int32_t> &hptr, const std::vector<int32_t> &hind, int cap, int ucap,
-                ReuseTables &T, bool patch = false);
+                ReuseTables &T, bool patch = false, bool rows = false);
 int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P);
 // Sorts the tiles by their first element and lays them out so that workgroup b
 // (dispatched to XCD b mod nxcd) walks a contiguous chunk of the sorted list
@@ -712,6 +904,9 @@ int env_int(const char *name, int dflt);
 typedef int (*reuse_launch_fn)(elfel_hip_ctx *, const ReuseArgs &, const Coef &, const QTable &, int);
 reuse_launch_fn pick_reuse64(int kind, int fe, int nq, int csc, int block);
 reuse_launch_fn pick_reuse32(int kind, int fe, int nq, int csc, int block);
+// row-owner heat kernels (T3 and Q4, workgroups of HEAT_ROWS_BLOCK threads)
+reuse_launch_fn pick_heat_rows64(int fe, int nq, int csc);
+reuse_launch_fn pick_heat_rows32(int fe, int nq, int csc);
 int gather_geom64(elfel_hip_ctx *c, int nen, int64_t E, const int32_t *conn, const double2 *xy, double *geom);
 int gather_geom_rel(elfel_hip_ctx *c, int nen, int64_t E, const int32_t *conn, const double2 *xy, double *geom);
 
@@ -731,6 +926,29 @@ int launch_reuse(elfel_hip_ctx *c, const ReuseArgs &A, const Coef &cf, const QTa
     HIPCHK(c, hipGetLastError());
     return 0;
 }
+
+template <typename R, int NEN, int NQ, bool CSC>
+int launch_heat_rows(elfel_hip_ctx *c, const ReuseArgs &A, const Coef &cf, const QTable &tab, int lds_vals)
+{
+    if (A.ntiles <= 0) return 0;
+    if (A.maxu > HEAT_ROWS_BLOCK) return set_err(c, ELFEL_EINVAL, "internal: heat tile exceeds one element per thread");
+    lds_vals = lds_span(lds_vals);
+    // element records alias the value sums
+    lds_vals = std::max(lds_vals, (int)(((size_t)A.maxu * heat_rec<NEN>() * sizeof(R) + 7) / 8));
+    lds_vals = (lds_vals + 1) & ~1;
+    const size_t lds = sizeof(double) * ((size_t)lds_vals + REUSE_MAX_TILE_ROWS);
+    hipLaunchKernelGGL((k_heat_rows<R, NEN, NQ, CSC, HEAT_ROWS_BLOCK>), dim3(A.ntiles), dim3(HEAT_ROWS_BLOCK), lds,
+                       c->stream, A, cf, tab, lds_vals);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+#define HEAT_ROWS_PICK_BODY(R)                                                                        \
+    if (fe == ELFEL_T3 && nq == 1) return csc ? launch_heat_rows<R, 3, 1, true> : launch_heat_rows<R, 3, 1, false>; \
+    if (fe == ELFEL_T3 && nq == 3) return csc ? launch_heat_rows<R, 3, 3, true> : launch_heat_rows<R, 3, 3, false>; \
+    if (fe == ELFEL_Q4 && nq == 1) return csc ? launch_heat_rows<R, 4, 1, true> : launch_heat_rows<R, 4, 1, false>; \
+    if (fe == ELFEL_Q4 && nq == 4) return csc ? launch_heat_rows<R, 4, 4, true> : launch_heat_rows<R, 4, 4, false>; \
+    return nullptr;
 
 #define REUSE_PICK_BODY(R, B)                                                                           \
     if (kind == ELFEL_HEAT) {                                                                        \

@@ -42,8 +42,11 @@
 
 #include "common.h"
 
+int env_int(const char *name, int dflt);  // reuse.hip
+
 #define SV_BLOCK 256
 #define CHECK 8
+#define SELL_UNROLL 12  // slices at most this wide are read with every load in flight at once
 #ifndef RU_DROP
 #define RU_DROP 1e-2   // reliable update when the recurrence residual has dropped by this factor
 #endif
@@ -147,18 +150,85 @@ __global__ void k_sell_fill(int64_t nfree, const int32_t *__restrict__ ptr, cons
     }
 }
 
+// Slices whose every slot holds one column offset for all its rows (each row
+// of the slice has its k-th entry at row + d_k: the stencil of a structured
+// grid away from its boundary) are flagged, and their offsets d_k are kept
+// once per slot; such a slice reads no per-entry column at all.  One wave per
+// slice.
+template <typename C>
+__global__ void k_sell_uniform(int64_t nfree, int64_t nslices, const uint32_t *__restrict__ off,
+                               const C *__restrict__ col, uint32_t *__restrict__ uflag, int32_t *__restrict__ uoff)
+{
+    const int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (s >= nslices) return;
+    const int64_t i = s * 64 + lane;
+    const uint32_t o = off[s], w = (off[s + 1] - o) >> 6, sb = o >> 6;
+    bool uniform = true;
+    for (uint32_t k = 0; k < w; k++) {
+        const int32_t d = sell_col<C>(col[(size_t)o + (size_t)k * 64 + lane], i) - (int32_t)i;
+        const int32_t d0 = __shfl(d, 0, 64);  // row s*64 < nfree: lane 0 is a real row
+        if (__ballot((i < nfree) && (d != d0)) != 0ull) uniform = false;
+        if (lane == 0) uoff[sb + k] = d0;
+    }
+    if (lane == 0) uflag[s] = uniform ? 1u : 0u;
+}
+
+struct SellU {
+    const uint32_t *uflag;  // per slice: 1 when every slot has one column offset for all rows
+    const int32_t *uoff;    // per slot: that offset
+};
+
 template <typename T, typename C>
 __device__ __forceinline__ T sell_row(int64_t i, const uint32_t *__restrict__ off, const C *__restrict__ col,
-                                      const T *__restrict__ val, const T *__restrict__ x)
+                                      const T *__restrict__ val, const T *__restrict__ x, SellU U)
 {
-    const int64_t s = i >> 6;
+    // one wave holds one slice: s is the same on every lane
+    const int64_t s = __builtin_amdgcn_readfirstlane((int)(i >> 6));
     const int lane = (int)(i & 63);
     const uint32_t o = off[s], w = (off[s + 1] - o) >> 6;
     T acc = T(0);
-    const C *c = col + o + lane;
     const T *v = val + o + lane;
+    const C *c = col + o + lane;
+    const bool uni = U.uflag[s] != 0u;
+    const int32_t *d = U.uoff + (o >> 6);
+    const int32_t ri = (int32_t)i;
+    if (w <= SELL_UNROLL) {
+        // every load of the slice is issued before the first product: the
+        // values and (uniform slices) the gathers at once, else the columns,
+        // then the gathers.  Slots past w read slot 0 again (in bounds) and
+        // add nothing.
+        T vv[SELL_UNROLL], xx[SELL_UNROLL];
+        if (uni) {
+#pragma unroll
+            for (int k = 0; k < SELL_UNROLL; k++) {
+                const int kk = k < (int)w ? k : 0;
+                vv[k] = v[(size_t)kk * 64];
+                xx[k] = x[ri + d[kk]];
+            }
+        } else {
+            int32_t cj[SELL_UNROLL];
+#pragma unroll
+            for (int k = 0; k < SELL_UNROLL; k++) {
+                const int kk = k < (int)w ? k : 0;
+                vv[k] = v[(size_t)kk * 64];
+                cj[k] = sell_col<C>(c[(size_t)kk * 64], i);
+            }
+#pragma unroll
+            for (int k = 0; k < SELL_UNROLL; k++) xx[k] = x[cj[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < SELL_UNROLL; k++)
+            if (k < (int)w) acc += vv[k] * xx[k];
+        return acc;
+    }
+    if (uni) {
 #pragma unroll 4
-    for (uint32_t k = 0; k < w; k++) acc += v[(size_t)k * 64] * x[sell_col<C>(c[(size_t)k * 64], i)];
+        for (uint32_t k = 0; k < w; k++) acc += v[(size_t)k * 64] * x[ri + d[k]];
+    } else {
+#pragma unroll 4
+        for (uint32_t k = 0; k < w; k++) acc += v[(size_t)k * 64] * x[sell_col<C>(c[(size_t)k * 64], i)];
+    }
     return acc;
 }
 
@@ -170,13 +240,13 @@ __device__ __forceinline__ T sell_row(int64_t i, const uint32_t *__restrict__ of
 template <typename T, typename C>
 __global__ __launch_bounds__(SV_BLOCK) void k_cg_spmv(int64_t n, const uint32_t *__restrict__ off,
                                                       const C *__restrict__ col, const T *__restrict__ val,
-                                                      const T *__restrict__ p, T *__restrict__ q,
+                                                      SellU U, const T *__restrict__ p, T *__restrict__ q,
                                                       double *__restrict__ part_pq)
 {
     __shared__ double sh[SV_BLOCK / 64];
     double d = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * SV_BLOCK) {
-        const T y = sell_row<T, C>(i, off, col, val, p);
+        const T y = sell_row<T, C>(i, off, col, val, p, U);
         q[i] = y;
         d += (double)p[i] * (double)y;
     }
@@ -285,14 +355,14 @@ __global__ void k_flush(int64_t n, float *__restrict__ x32, double *__restrict__
 template <typename C>
 __global__ __launch_bounds__(SV_BLOCK) void k_sell_resid(int64_t n, const uint32_t *__restrict__ off,
                                                          const C *__restrict__ col,
-                                                         const double *__restrict__ val,
+                                                         const double *__restrict__ val, SellU U,
                                                          const double *__restrict__ x, const double *__restrict__ b,
                                                          double *__restrict__ r, double *__restrict__ part)
 {
     __shared__ double sh[SV_BLOCK / 64];
     double rr = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * SV_BLOCK) {
-        const double v = b[i] - sell_row<double, C>(i, off, col, val, x);
+        const double v = b[i] - sell_row<double, C>(i, off, col, val, x, U);
         r[i] = v;
         rr += v * v;
     }
@@ -356,6 +426,7 @@ struct Sell {
     bool c16;
     const double *v64;
     const float *v32;
+    SellU U;
 };
 
 template <typename T>
@@ -377,11 +448,11 @@ void cg_step(hipStream_t st, int64_t n, const Sell &S, const Vecs<T> &v, const P
 {
     const dim3 g(P.G), b(SV_BLOCK);
     if (S.c16)
-        hipLaunchKernelGGL((k_cg_spmv<T, int16_t>), g, b, 0, st, n, S.off, (const int16_t *)S.col, sell_vals<T>(S), v.p,
-                           v.q, P.pq);
+        hipLaunchKernelGGL((k_cg_spmv<T, int16_t>), g, b, 0, st, n, S.off, (const int16_t *)S.col, sell_vals<T>(S), S.U,
+                           v.p, v.q, P.pq);
     else
-        hipLaunchKernelGGL((k_cg_spmv<T, int32_t>), g, b, 0, st, n, S.off, (const int32_t *)S.col, sell_vals<T>(S), v.p,
-                           v.q, P.pq);
+        hipLaunchKernelGGL((k_cg_spmv<T, int32_t>), g, b, 0, st, n, S.off, (const int32_t *)S.col, sell_vals<T>(S), S.U,
+                           v.p, v.q, P.pq);
     hipLaunchKernelGGL((k_cg_update<T>), g, b, 0, st, n, v.dinv, v.x, v.r, v.z, v.p, v.q, P.pq, P.rz[par ^ 1],
                        P.rz[par], P.rr);
     hipLaunchKernelGGL((k_cg_dir<T>), g, b, 0, st, n, P.rz[par], P.rz[par ^ 1], P.rr, v.z, v.p, sc);
@@ -463,21 +534,36 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
     HIPCHK(c, hipMemcpyAsync(off + nslices, total, 4, hipMemcpyDeviceToDevice, st));
     // SELL arrays in a buffer of their own (the size is known only now)
     const size_t cbytes = (size_t)nnzp * (c16 ? 2 : 4);
-    const size_t sbytes = ((cbytes + 15) & ~(size_t)15) + (size_t)nnzp * (8 + (mixed ? 4 : 0)) + 256;
+    const size_t nslots = (size_t)nnzp / 64;
+    const size_t vbytes = (size_t)nnzp * (8 + (mixed ? 4 : 0));
+    const size_t sbytes = ((cbytes + 15) & ~(size_t)15) + vbytes + 4 * (size_t)(nslices + 1) + 4 * nslots + 256;
     if ((rc = ensure(c, c->pcg_sell, sbytes))) return rc;
     void *scol = c->pcg_sell.p;
     double *sv64 = (double *)((char *)c->pcg_sell.p + ((cbytes + 15) & ~(size_t)15));
     float *sv32 = mixed ? (float *)(sv64 + nnzp) : nullptr;
+    uint32_t *uflag = (uint32_t *)((char *)sv64 + vbytes);
+    int32_t *uoff = (int32_t *)(uflag + nslices + (nslices & 1));
     if (c16)
         hipLaunchKernelGGL((k_sell_fill<int16_t>), dim3(ceil_div(nslices * 64, 256)), dim3(256), 0, st, n, ptr, ind, val,
                            off, (int16_t *)scol, sv64, sv32);
     else
         hipLaunchKernelGGL((k_sell_fill<int32_t>), dim3(ceil_div(nslices * 64, 256)), dim3(256), 0, st, n, ptr, ind, val,
                            off, (int32_t *)scol, sv64, sv32);
+    const bool uniform_slots = env_int("ELFEL_HIP_SELL_UNIFORM", 1) != 0;
+    if (uniform_slots) {
+        if (c16)
+            hipLaunchKernelGGL((k_sell_uniform<int16_t>), dim3(ceil_div(nslices * 64, 256)), dim3(256), 0, st, n,
+                               nslices, (const uint32_t *)off, (const int16_t *)scol, uflag, uoff);
+        else
+            hipLaunchKernelGGL((k_sell_uniform<int32_t>), dim3(ceil_div(nslices * 64, 256)), dim3(256), 0, st, n,
+                               nslices, (const uint32_t *)off, (const int32_t *)scol, uflag, uoff);
+    } else {
+        hipLaunchKernelGGL(k_fill<uint32_t>, dim3(ceil_div(nslices, 256)), dim3(256), 0, st, nslices, uflag, 0u);
+    }
     hipLaunchKernelGGL(k_diag_inv, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, ptr, ind, val, dinv64,
                        mixed ? v32.dinv : nullptr);
     HIPCHK(c, hipGetLastError());
-    const Sell S{off, scol, c16, sv64, sv32};
+    const Sell S{off, scol, c16, sv64, sv32, SellU{uflag, uoff}};
 
     auto read_sc = [&](double *h) -> int {
         HIPCHK(c, hipMemcpyAsync(h, sc, sizeof(double) * S_NS, hipMemcpyDeviceToHost, st));
@@ -498,10 +584,12 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
     auto true_residual = [&](double *out_norm) -> int {
         if (S.c16)
             hipLaunchKernelGGL((k_sell_resid<int16_t>), dim3(G), dim3(SV_BLOCK), 0, st, n, S.off,
-                               (const int16_t *)S.col, S.v64, (const double *)xfull, (const double *)b, r64, P.aux);
+                               (const int16_t *)S.col, S.v64, S.U, (const double *)xfull, (const double *)b, r64,
+                               P.aux);
         else
             hipLaunchKernelGGL((k_sell_resid<int32_t>), dim3(G), dim3(SV_BLOCK), 0, st, n, S.off,
-                               (const int32_t *)S.col, S.v64, (const double *)xfull, (const double *)b, r64, P.aux);
+                               (const int32_t *)S.col, S.v64, S.U, (const double *)xfull, (const double *)b, r64,
+                               P.aux);
         hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, st, (const double *)P.aux, G, sc, (int)S_RR64,
                            (const double *)nullptr, 0);
         HIPCHK(c, hipGetLastError());

@@ -404,9 +404,9 @@ class Context:
         self._chk(self.lib.elfel_hip_batch_begin(self.h), "batch_begin")
 
     def batch_add(self, kind, fe, rule, conn, xy, dofnums, ndofs, coef_e=None, kappa=1.0, Q=0.0, D=None,
-                  layout=CSC):
+                  layout=CSC, precision=MIXED):
         pb = Problem()
-        pb.kind, pb.fe, pb.rule, pb.precision, pb.method, pb.layout = kind, fe, rule, MIXED, REUSE, layout
+        pb.kind, pb.fe, pb.rule, pb.precision, pb.method, pb.layout = kind, fe, rule, precision, REUSE, layout
         pb.kappa, pb.Q, pb.mu = kappa, Q, 1.0
         if D is not None:
             Dc = np.asarray(D, np.float64).reshape(3, 3).ravel(order="F")

@@ -270,16 +270,20 @@ int elfel_hip_copy_mesh(elfel_hip_ctx *ctx, int field, int64_t E, int64_t *conn,
 int elfel_hip_colliding_errors(elfel_hip_ctx *ctx, int64_t ndofs, const double *U, double *ep, double *ev);
 
 /* ---- batched problems ------------------------------------------------------
- * Many independent problems of uneven size (heat T3 1-point and plane
- * elasticity T6 3-point, mixed precision) assembled by one persistent fused
- * launch replayed from a hipGraph.  Each problem is one reference driver's
- * integrate! + finish! (test/test_heat.jl:35-71, test/test_elasticity.jl:
- * 149-181); problems of one kind share their uniform coefficients (kappa, Q,
- * D); coef_e is the per-element conductivity (heat) or scale of D
- * (elasticity), or NULL.  batch_build runs the symbolic phase of every
- * problem, picks the launch shape (autotune = 1 times every variant on the
- * batch; 0 uses the on-disk cache when it has an entry) and captures the
- * graph.  No counterpart in the reference beyond the per-problem loops. */
+ * Many independent problems of uneven size (heat T3 1-point, heat Q4 2x2 and
+ * plane elasticity T6 3-point) assembled by one persistent fused launch
+ * replayed from a hipGraph.  Each problem is one reference driver's
+ * integrate! + finish! (test/test_heat.jl:35-71, examples/heat/poisson/q4.jl:
+ * 31-54, test/test_elasticity.jl:149-181); the problems of a batch share their
+ * precision (ELFEL_MIXED, or ELFEL_FP64: the FP64 REUSE method's bits) and
+ * layout, problems of one kind their uniform coefficients (kappa, Q, D);
+ * coef_e is the per-element conductivity (heat) or scale of D (elasticity),
+ * or NULL.  batch_build runs the symbolic phase of every problem (a mesh the
+ * REUSE tiles do not hold is refused with ELFEL_ENOTSUP; the build uses field
+ * 0 of the context, whose mesh and dof map it replaces), picks the launch
+ * shape (autotune = 1 times every variant on the batch; 0 uses the on-disk
+ * cache when it has an entry) and captures the graph.  No counterpart in the
+ * reference beyond the per-problem loops. */
 int elfel_hip_batch_begin(elfel_hip_ctx *ctx);
 int elfel_hip_batch_add(elfel_hip_ctx *ctx, const elfel_hip_problem *pb, int32_t nen, int64_t E, const int64_t *conn,
                         int64_t Nn, const double *xy, int32_t ncomp, const int64_t *dofnums, int64_t ndofs,
