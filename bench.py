@@ -239,6 +239,9 @@ def bench_single(args, L, W):
         extra["pcg"] = bench_pcg(ctx, L, cs, res, args.pcg_maxit)
     if args.config == "c5":
         extra["device_setup"] = device_setup_c5(L, cs)
+    if args.config == "c2":
+        extra["element_matrices"] = bench_element_matrices(ctx, L, cs)
+        progress("element matrices timed")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -269,6 +272,13 @@ def bench_single(args, L, W):
         "errors": errs,
         "geometry": {"min_det": mindet, "elements_over_quality_threshold": nbad},
         "setup": {"host_mesh_and_upload_s": t_setup, "symbolic_ms": t_sym},
+        # the first mixed REUSE call: symbolic phase (pattern, tiles, tables; host
+        # C++ and the plain pipeline) + the numeric kernel, against the plain FP64
+        # method, which has no symbolic phase
+        "first_call": {"ms": t_sym, "elements_per_s": E / (t_sym * 1e-3),
+                       "assemblies_to_break_even_vs_fp64_plain":
+                           (t_sym / (out["fp64"]["ms_per_step"] - m["ms_per_step"]))
+                           if "fp64" in out and out["fp64"]["ms_per_step"] > m["ms_per_step"] else None},
         "device": {"name": devname, "cus": ncu},
     }
     line.update(extra)
@@ -367,6 +377,26 @@ def device_setup_c5(L, cs):
     c2.close()
     return {"mesh_ms": (t1 - t0) * 1e3, "boundary_ms": (t2 - t1) * 1e3, "numbering_ms": (t3 - t2) * 1e3,
             "dofs_equal_host": bool(np.array_equal(dn, f.dofnums)), "elements": E, "nodes": Nn}
+
+
+def bench_element_matrices(ctx, L, cs, reps=20):
+    """The element matrices of the C2 mesh (T6 plane elasticity, 3-point rule,
+    mixed) from the vector kernel and from the matrix-core kernel
+    (v_mfma_f32_4x4x1f32), left on the device: median kernel time of `reps`
+    calls each.  Both write 576 bytes per element (144 FP32 entries), so
+    both are bound by the write of the blocks; the matrix-core form does the
+    B^T D B contraction in 81 MFMA instructions per 16 elements."""
+    E, D = cs["E"], cs["kw"]["D"]
+    out = {"elements": E, "bytes_written": 576 * E}
+    for name, meth in (("valu", L.EM_VALU), ("mfma", L.EM_MFMA)):
+        ts = []
+        for _ in range(reps + 3):
+            ts.append(ctx.element_matrices(E, D, meth, copy=False)[1])
+        ms = float(np.median(ts[3:]))
+        out[f"{name}_ms"] = ms
+        out[f"{name}_elements_per_s"] = E / (ms * 1e-3)
+        out[f"{name}_write_gbs"] = 576 * E / (ms * 1e-3) / 1e9
+    return out
 
 
 def bench_pcg(ctx, L, cs, res, args_maxit=60000):

@@ -36,6 +36,7 @@ static_assert(QT_T3 == ELFEL_T3 && QT_T6 == ELFEL_T6 && QT_Q4 == ELFEL_Q4, "elem
 #define PAD_LOG2 12
 enum { COEF_GAP = 16, CONN_GAP = 24, XY_GAP = 40 };
 __host__ __device__ __forceinline__ int64_t padded(int64_t i, int gap) { return i + (i >> PAD_LOG2) * gap; }
+__host__ __device__ __forceinline__ int32_t padded32(int32_t i, int gap) { return i + (i >> PAD_LOG2) * gap; }
 // entries to allocate for n padded entries
 __host__ __device__ __forceinline__ int64_t padded_len(int64_t n, int gap) { return padded(n, gap) + 1; }
 

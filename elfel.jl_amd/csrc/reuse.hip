@@ -231,12 +231,19 @@ int tile_range(const TileInput &I, int64_t r0, int64_t r1, ReuseTables &T, std::
                 }
             }
         }
-        // pad to cap tasks, so that tile t's tasks start at t * cap (the kernel
-        // computes a thread's task index without reading the tile first)
-        for (int k = cur.y; k < cap; k++) {
-            T.elem.push_back(0);
-            T.info.push_back(TASK_PAD);
-            T.pos.insert(T.pos.end(), ncolp, (uint8_t)0);
+        if (I.rw) {
+            // row-owner tiles: the records hold everything; the tasks are dropped
+            T.elem.resize((size_t)cur.x);
+            T.info.resize((size_t)cur.x);
+            T.pos.resize((size_t)cur.x * ncolp);
+        } else {
+            // pad to cap tasks, so that tile t's tasks start at t * cap (the kernel
+            // computes a thread's task index without reading the tile first)
+            for (int k = cur.y; k < cap; k++) {
+                T.elem.push_back(0);
+                T.info.push_back(TASK_PAD);
+                T.pos.insert(T.pos.end(), ncolp, (uint8_t)0);
+            }
         }
         T.tiles.push_back(cur);
         T.trows.push_back(cr);
@@ -338,8 +345,12 @@ int tile_range(const TileInput &I, int64_t r0, int64_t r1, ReuseTables &T, std::
     return 0;
 }
 
-// Locality order of the nodes of a 2D mesh: bands about four node spacings
+// Locality order of the nodes of a 2D mesh: bands about eight node spacings
 // high, bottom to top, each traversed in x (then y); ties keep node order.
+// With row-owner tiles (at most 256 rows and 256 elements each) on heat Q4
+// 4096^2, band heights 3 / 4 / 6 / 8 / 10 / 12 / 16 measured 0.599 / 0.573 /
+// 0.563 / 0.545 / 0.558 / 0.559 / 0.582 ms: an 8 x 27 node tile holds 216
+// rows in 252 elements (1.17 elements per row, against 1.28 for 4 x 48).
 // Packed greedily in this order, a tile's rows cover a few short stretches of
 // neighbouring grid lines instead of one stretch of one line: each element of
 // a heat Q4 4096^2 mesh is integrated by 1.3 tiles instead of 2, while a tile
@@ -366,7 +377,7 @@ void band_order(const std::vector<double2> &xy, std::vector<int32_t> &order)
     // key = (band, x, y), x and y quantised to 16 bits; the node spacing is
     // estimated from the bounding box and the node count
     const double spacing = std::sqrt(std::max((x1 - x0) * (y1 - y0), 1e-300) / std::max<double>((double)nn, 1.0));
-    const double bh = spacing * std::max(1, env_int("ELFEL_HIP_BAND_ROWS", 4));
+    const double bh = spacing * std::max(1, env_int("ELFEL_HIP_BAND_ROWS", 8));
     for (size_t i = 0; i < nn; i++) {
         order[i] = (int32_t)i;
         const uint32_t qx = (uint32_t)((xy[i].x - x0) * sx), qy = (uint32_t)((xy[i].y - y0) * sy);
@@ -504,6 +515,7 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
         nranges, std::min<int64_t>(16, std::max(1u, std::thread::hardware_concurrency()))));
     std::atomic<int64_t> next{0};
     auto worker = [&]() {
+        // the last tile each element joined (element-staged tiles)
         std::vector<int32_t> stamp(I.staged ? F0.E : 0, -1);
         int tile_id = 0;
         for (;;) {
@@ -568,7 +580,9 @@ void order_tiles(ReuseTables &T, int nxcd, int cap, int run)
         // 0 and they keep the row order (measured faster for heat Q4 4096^2
         // than sorting them by their first element: 0.91 against 1.00 ms)
         int32_t m = INT32_MAX;
-        for (int k = 0; k < T.tiles[t].y; k++) m = std::min(m, T.elem[(size_t)T.tiles[t].x + k]);
+        if (!T.telem.empty()) m = 0;  // (row-owner tiles keep no tasks at all)
+        else
+            for (int k = 0; k < T.tiles[t].y; k++) m = std::min(m, T.elem[(size_t)T.tiles[t].x + k]);
         key[t] = {m, (int32_t)t};
     }
     std::stable_sort(key.begin(), key.end());
@@ -605,9 +619,11 @@ void order_tiles(ReuseTables &T, int nxcd, int cap, int run)
         trows[b] = T.trows[o];
         tiles[b].x = (int)(b * cap);  // the tasks move with their tile
         if (!T.truns.empty()) truns[b] = T.truns[o];  // the runs stay, their ranges move
-        std::copy(T.elem.begin() + o * cap, T.elem.begin() + (o + 1) * cap, elem.begin() + b * cap);
-        std::copy(T.info.begin() + o * cap, T.info.begin() + (o + 1) * cap, info.begin() + b * cap);
-        std::copy(T.pos.begin() + o * cap * ncolp, T.pos.begin() + (o + 1) * cap * ncolp, pos.begin() + b * cap * ncolp);
+        if (!T.rw) {  // row-owner tiles keep no tasks
+            std::copy(T.elem.begin() + o * cap, T.elem.begin() + (o + 1) * cap, elem.begin() + b * cap);
+            std::copy(T.info.begin() + o * cap, T.info.begin() + (o + 1) * cap, info.begin() + b * cap);
+            std::copy(T.pos.begin() + o * cap * ncolp, T.pos.begin() + (o + 1) * cap * ncolp, pos.begin() + b * cap * ncolp);
+        }
         if (!T.telem.empty())
             std::copy(T.telem.begin() + o * ts, T.telem.begin() + (o + 1) * ts, telem.begin() + b * ts);
         if (!T.rrec.empty())
@@ -724,9 +740,20 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
     const Field &F0 = c->f[0];
     const Field *F1 = stokes ? &c->f[1] : nullptr;
     const int64_t n = c->nrow;
+    // ELFEL_HIP_SYMBOLIC_TIMES=1: the phase's stages on stderr
+    static const bool say = env_int("ELFEL_HIP_SYMBOLIC_TIMES", 0) != 0;
+    auto lap = [&](const char *what) {
+        if (!say) return;
+        static thread_local std::chrono::steady_clock::time_point last;
+        const auto now = std::chrono::steady_clock::now();
+        if (what) fprintf(stderr, "[symbolic] %-28s %8.1f ms\n", what, std::chrono::duration<double, std::milli>(now - last).count());
+        last = now;
+    };
+    lap(nullptr);
     std::vector<int32_t> hptr, hind;
     int rc = pattern_to_host(c, pb, tab, cf, hptr, hind);
     if (rc) return rc;
+    lap("pattern (device) + copy out");
     const int64_t nnz = (int64_t)hind.size();
     if ((rc = ensure(c, P.ptr, 4 * (n + 1))) || (rc = ensure(c, P.ind, 4 * std::max<int64_t>(nnz, 1)))) return rc;
     HIPCHK(c, hipMemcpyAsync(P.ptr.p, c->ptr.p, 4 * (n + 1), hipMemcpyDeviceToDevice, c->stream));
@@ -745,9 +772,12 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
     } else if (rc) {
         return rc;
     }
+    lap("tiles and tables (host)");
     if (!P.fallback) {
         if (env_int("ELFEL_HIP_TILE_ORDER", 1)) order_tiles(T, 8, cap, env_int("ELFEL_HIP_XCD_RUN", 0));
+        lap("tile order (host)");
         if ((rc = upload_tables(c, T, P))) return rc;
+        lap("upload");
         // the numeric kernel gathers from padded copies of the mesh arrays
         if ((rc = ensure(c, P.conn_pad, 4 * (size_t)F0.nen * padded_len(F0.E, CONN_GAP))) ||
             (rc = ensure(c, P.xy_pad, 16 * (size_t)padded_len(F0.Nn, XY_GAP))))

@@ -709,15 +709,21 @@ __device__ __forceinline__ void heat_rows_tile(const ReuseArgs &A, const Coef &c
         }
     }
     if (tid < nU) {
+        // 32-bit index arithmetic: element and node counts stay below 2^31 / NEN
         double kap = cf.kappa, Qv = cf.Q;
-        if (cf.kappa_e) kap = cf.kappa_e[padded(eg, cf.cgap)];
-        if (cf.Q_e) Qv = cf.Q_e[padded(eg, cf.cgap)];
+        const uint32_t ec = (uint32_t)padded32(eg, cf.cgap);
+        if (cf.kappa_e) kap = cf.kappa_e[ec];
+        if (cf.Q_e) Qv = cf.Q_e[ec];
         int32_t nd[NEN];
+        const uint32_t en = (uint32_t)padded32(eg, A.ngap) * NEN;
 #pragma unroll
-        for (int k = 0; k < NEN; k++) nd[k] = A.conn[padded(eg, A.ngap) * NEN + k];
+        for (int k = 0; k < NEN; k++) nd[k] = A.conn[en + k];
         double X[NEN], Y[NEN];
 #pragma unroll
-        for (int k = 0; k < NEN; k++) { const double2 p = A.xy[padded(nd[k], A.xgap)]; X[k] = p.x; Y[k] = p.y; }
+        for (int k = 0; k < NEN; k++) {
+            const double2 p = A.xy[(uint32_t)padded32(nd[k], A.xgap)];
+            X[k] = p.x; Y[k] = p.y;
+        }
         if constexpr (sizeof(R) == 4) {
             const double x0 = X[0], y0 = Y[0];
 #pragma unroll
@@ -827,6 +833,27 @@ __global__ __launch_bounds__(BLOCK) void k_reuse_batch(ReuseArgs H3, ReuseArgs E
     }
 }
 
+// The same tiles, one workgroup per tile (the launch shape the autotuner
+// picks on 256 problems): without the grid-stride loop each branch loads only
+// its own kind's arguments.
+template <typename R, bool CSC, int BLOCK, int KINDS>
+__global__ __launch_bounds__(BLOCK) void k_reuse_batch1(ReuseArgs H3, ReuseArgs Ev, ReuseArgs H4, Coef ch3, Coef ce,
+                                                        Coef ch4, int n0, int n1, int lds_vals)
+{
+    extern __shared__ double lds[];
+    constexpr QTable t3 = qt_fixed<3, 1>(), t6 = qt_fixed<6, 3>(), t4 = qt_fixed<4, 4>();
+    const int t = blockIdx.x;
+    if (t < n0) {
+        if constexpr ((KINDS & 1) != 0) reuse_tile<ELFEL_HEAT, R, 3, 1, 1, CSC, BLOCK, 2>(H3, ch3, t3, lds, lds_vals, t);
+    } else if (t < n1) {
+        if constexpr ((KINDS & 2) != 0)
+            reuse_tile<ELFEL_HEAT, R, 4, 1, 4, CSC, BLOCK, 2>(H4, ch4, t4, lds, lds_vals, t - n0);
+    } else {
+        if constexpr ((KINDS & 4) != 0)
+            reuse_tile<ELFEL_ELAST, R, 6, 2, 3, CSC, BLOCK, 1>(Ev, ce, t6, lds, lds_vals, t - n1);
+    }
+}
+
 // Launch of the batched kernel (batch.hip: mixed, batch64.hip: FP64).
 struct BatchLaunch {
     ReuseArgs A[3];  // heat T3, elasticity T6, heat Q4
@@ -844,8 +871,15 @@ template <typename R>
 int launch_batch_t(hipStream_t st, const BatchLaunch &L)
 {
     const bool q4 = L.n1 > L.n0;
-#define KB(C, BL, K) hipLaunchKernelGGL((k_reuse_batch<R, C, BL, K>), dim3(L.grid), dim3(BL), L.lds, st, L.A[0], \
-                                        L.A[1], L.A[2], L.cf[0], L.cf[1], L.cf[2], L.n0, L.n1, L.nt, L.lds_vals)
+#define KB(C, BL, K)                                                                                           \
+    do {                                                                                                       \
+        if (L.grid >= L.nt)                                                                                    \
+            hipLaunchKernelGGL((k_reuse_batch1<R, C, BL, K>), dim3(L.nt), dim3(BL), L.lds, st, L.A[0], L.A[1], \
+                               L.A[2], L.cf[0], L.cf[1], L.cf[2], L.n0, L.n1, L.lds_vals);                     \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_reuse_batch<R, C, BL, K>), dim3(L.grid), dim3(BL), L.lds, st, L.A[0], L.A[1], \
+                               L.A[2], L.cf[0], L.cf[1], L.cf[2], L.n0, L.n1, L.nt, L.lds_vals);               \
+    } while (0)
 #define KB2(C, BL) do { if (q4) KB(C, BL, 7); else KB(C, BL, 5); } while (0)
     if (L.block == 128) { if (L.csc) KB2(true, 128); else KB2(false, 128); }
     else { if (L.csc) KB2(true, 256); else KB2(false, 256); }

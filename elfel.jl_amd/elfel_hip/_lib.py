@@ -36,8 +36,9 @@ EXPORTED = [
     "elfel_hip_colliding_errors",
     "elfel_hip_block_mesh", "elfel_hip_t6_to_t3", "elfel_hip_boundary_nodes", "elfel_hip_copy_mesh",
     "elfel_hip_load_mesh", "elfel_hip_set_mesh_sdim", "elfel_hip_set_element_dofs",
-    "elfel_hip_assembly_info", "elfel_hip_sparse_noalloc",
+    "elfel_hip_assembly_info", "elfel_hip_sparse_noalloc", "elfel_hip_element_matrices",
 ]
+EM_VALU, EM_MFMA = 0, 1
 NOALLOC_MAX_COO = 1 << 20  # ELFEL_NOALLOC_MAX_COO
 
 
@@ -95,6 +96,8 @@ def load():
     lib.elfel_hip_set_coefficients.argtypes = [P, ctypes.c_int64, f64p, f64p, f64p]
     lib.elfel_hip_assemble.argtypes = [P, ctypes.POINTER(Problem), i64p]
     lib.elfel_hip_copy_vector.argtypes = [P, f64p, ctypes.c_int64]
+    lib.elfel_hip_element_matrices.argtypes = [P, ctypes.POINTER(Problem), ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                                               ctypes.c_int64, f64p]
     lib.elfel_hip_assembly_info.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                             ctypes.c_char_p, ctypes.c_int]
     lib.elfel_hip_synchronize.argtypes = [P]
@@ -280,6 +283,22 @@ class Context:
         self._chk(self.lib.elfel_hip_assemble(self.h, ctypes.byref(pb), ctypes.byref(nnz) if wait else None),
                   "assemble")
         return nnz.value if wait else None
+
+    def element_matrices(self, E, D, method=EM_VALU, copy=True):
+        """ke of the E elements of field 0 (T6 plane elasticity, 3-point rule,
+        mixed): (E, 144) float32, column-major per element, and the kernel ms
+        (copy False: the kernel alone, returns (None, ms))."""
+        pb = Problem()
+        pb.kind, pb.fe, pb.rule, pb.precision, pb.method, pb.layout = ELAST, T6, 3, MIXED, REUSE, CSC
+        Dc = np.asarray(D, np.float64).reshape(3, 3).ravel(order="F")
+        for i in range(9):
+            pb.D[i] = Dc[i]
+        ke = np.empty((E, 144), np.float32) if copy else None
+        ms = ctypes.c_double()
+        kp = ke.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if copy else None
+        self._chk(self.lib.elfel_hip_element_matrices(self.h, ctypes.byref(pb), method, kp, ke.size if copy else 0,
+                                                      ctypes.byref(ms)), "element_matrices")
+        return ke, ms.value
 
     def assembly_info(self):
         """(method, precision, note) of what the last assemble ran: the REUSE

@@ -186,6 +186,19 @@ int elfel_hip_assemble(elfel_hip_ctx *ctx, const elfel_hip_problem *pb, int64_t 
  * null) the reason when they differ from the request.  No counterpart in the
  * reference, which has one method. */
 int elfel_hip_assembly_info(elfel_hip_ctx *ctx, int *method, int *precision, char *note, int len);
+/* Element matrices: the ke block of every element of field 0 (what
+ * LocalMatrixAssembler holds after the element loop of test/test_elasticity.jl:
+ * 150-172, src/LocalAssemblers.jl:68-83), plane elasticity on T6 with the
+ * 3-point rule in mixed arithmetic, as ke[e * 144 + i + 12 j] (column-major per
+ * element, FP32).  method ELFEL_EM_VALU: vector FMAs, one work-item per
+ * element; ELFEL_EM_MFMA: the B^T D B contraction on the matrix cores
+ * (v_mfma_f32_4x4x1f32, 16 elements per wavefront).  *ms receives the kernel
+ * time.  n = 144 E; ke NULL with n 0 runs and times the kernel and leaves the
+ * blocks on the device.  The assembly kernels use the vector form; this entry
+ * point measures the matrix cores on the path's one dense contraction. */
+enum { ELFEL_EM_VALU = 0, ELFEL_EM_MFMA = 1 };
+int elfel_hip_element_matrices(elfel_hip_ctx *ctx, const elfel_hip_problem *pb, int method, float *ke, int64_t n,
+                               double *ms);
 /* Load vector of the last heat assembly (finish!(SysvecAssembler), :228-230). */
 int elfel_hip_copy_vector(elfel_hip_ctx *ctx, double *F, int64_t n);
 /* Blocks until the context's stream is idle. */
