@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--no-pcg", action="store_true", help="c5: assembly only")
     ap.add_argument("--pcg-maxit", type=int, default=60000)
     ap.add_argument("--only", choices=["all", "mixed", "fp64", "fp64-reuse"], default="all")
+    ap.add_argument("--kinds", choices=["all", "heat", "elast"], default="all",
+                    help="c4: only the problems of this kind (a diagnostic split of the batch)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MiB",
                     help="only run a device copy of this size (calibrates rocprof FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -469,7 +471,8 @@ def bench_batch(args, L, W):
     devname, ncu = ctx.device_name()
     specs = W.batch_specs()
     t0 = time.time()
-    probs = [W.batch_problem(k, kind, nn) for k, (kind, nn) in enumerate(specs)]
+    probs = [W.batch_problem(k, kind, nn) for k, (kind, nn) in enumerate(specs)
+             if args.kinds in ("all", kind)]
     t_host = time.time() - t0
     E = Nn = conn_b = coef_b = F_b = 0
     for p in probs:
@@ -535,9 +538,10 @@ def bench_batch(args, L, W):
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 geometry, f32 element products, f64 sums",
         "data": "synthetic (seeded SplitMix64), inputs resident in HBM",
-        "config": {"workload": "256 uneven heat-T3 (1-pt) / elasticity-T6 (3-pt) problems, E_k log-uniform "
-                               "[1e3, 2e5], lognormal(0, 0.5) coefficients, +-20% h jitter, one persistent "
-                               "launch replayed from a hipGraph",
+        "config": {"workload": ("256 uneven heat-T3 (1-pt) / elasticity-T6 (3-pt) problems, E_k log-uniform "
+                                "[1e3, 2e5], lognormal(0, 0.5) coefficients, +-20% h jitter, one persistent "
+                                "launch replayed from a hipGraph")
+                   + ("" if args.kinds == "all" else f"; only its {len(probs)} {args.kinds} problems"),
                    "problems": len(probs), "elements": E, "nnz": st["nnz"], "tiles": st["ntiles"],
                    "block": info0["block"], "grid": info0["grid"]},
         "roofline": {"bound": "hbm", "achieved": round(kernel_bytes / (ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
@@ -555,7 +559,8 @@ def bench_batch(args, L, W):
         "setup": {"host_meshes_s": t_host, "build_and_autotune_ms": t_build, "tuned_ms": st["tuned_ms"]},
         "device": {"name": devname, "cus": ncu},
     }
-    line["roofline"].update(measured_traffic("c4", 0, 0, block=info0["block"]))
+    if args.kinds == "all":
+        line["roofline"].update(measured_traffic("c4", 0, 0, block=info0["block"]))
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_batch(probs, sample)
     print(json.dumps(line))

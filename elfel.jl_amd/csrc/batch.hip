@@ -84,7 +84,7 @@ void batch_free(elfel_hip_ctx *c)
         release(k.conn); release(k.xy); release(k.coef);
         for (auto &t : k.tab) {
             release(t.tile); release(t.tile_rows); release(t.task_elem); release(t.task_info); release(t.task_pos);
-            release(t.tile_elem); release(t.rrec);
+            release(t.tile_elem); release(t.rrec); release(t.tconn); release(t.txy);
         }
     }
     release(b->ptr); release(b->ind); release(b->val); release(b->F);
@@ -181,6 +181,9 @@ static void tune_store(const std::string &key, int block, int gmult)
     fclose(f);
 }
 
+// Row-owner heat tiles in the batch (ELFEL_HIP_BATCH_ROWS=0: task tiles).
+static bool batch_rows_enabled() { return env_int("ELFEL_HIP_BATCH_ROWS", 1) != 0; }
+
 // Task slots per tile: heat tiles hold two light tasks per thread,
 // elasticity tiles one.
 static int batch_slots(int kind, int block) { return KS[kind].heat ? 2 * block : block; }
@@ -207,6 +210,10 @@ static int batch_launch(elfel_hip_ctx *c, Batch *b, int block, int grid)
         A.maxu = P.maxu;
         A.task_info = (const uint32_t *)P.task_info.p;
         A.task_pos = (const uint8_t *)P.task_pos.p;
+        A.rrec = P.rw ? (const uint32_t *)P.rrec.p : nullptr;  // row-owner heat tiles (block 256)
+        A.tstride = P.tstride;
+        A.tconn = P.rw && P.tgeom ? (const ushort4 *)P.tconn.p : nullptr;
+        A.txy = P.rw && P.tgeom ? (const double2 *)P.txy.p : nullptr;
         A.ncolp = P.ncolmax;
         A.ntiles = nk[k];
         A.geom = nullptr;  // the kernel gathers conn and xy itself
@@ -296,9 +303,12 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
         Coef cf{};
         for (int i = 0; i < 9; i++) cf.D[i] = pb.D[i];
         if ((rc = pattern_to_host(c, &pb, tab, cf, hptrs[k], hinds[k]))) return rc;
+        // heat problems get row-owner tiles for workgroups of 256 where their
+        // mesh allows (then every problem of the kind must: below)
         for (int bi = 0; bi < 2; bi++)
             if ((rc = tile_tables(c, false, c->f[0], nullptr, p.ndofs, hptrs[k], hinds[k],
-                                  batch_slots(p.kind, bi ? 256 : 128), bi ? 256 : 128, tabs[bi][k])))
+                                  batch_slots(p.kind, bi ? 256 : 128), bi ? 256 : 128, tabs[bi][k], false,
+                                  bi == 1 && KS[p.kind].heat && batch_rows_enabled())))
                 return rc;
         p.nnz = (int64_t)hinds[k].size();
         p.elem_base = koff_E[p.kind];
@@ -311,6 +321,25 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
         val_base += p.nnz;
         ptr_base += p.ndofs + 1;
         if (KS[p.kind].heat) row_base += p.ndofs;
+    }
+    // a kind's tiles are all row-owner tiles or all task tiles: a problem whose
+    // mesh the row records do not hold (a node in more elements, a longer row)
+    // puts every problem of its kind back on task tiles
+    for (int kind = 0; kind < BK_N; kind++) {
+        bool any_task = false;
+        for (size_t k = 0; k < b->probs.size(); k++)
+            if (b->probs[k].kind == kind && !tabs[1][k].rw) any_task = true;
+        if (!any_task) continue;
+        for (size_t k = 0; k < b->probs.size(); k++) {
+            BatchProb &p = b->probs[k];
+            if (p.kind != kind || !tabs[1][k].rw) continue;
+            const int nen = KS[p.kind].nen, nc = KS[p.kind].nc;
+            int rc = elfel_hip_set_mesh(c, 0, nen, p.E, p.conn.data(), p.Nn, p.xy.data());
+            if (!rc) rc = elfel_hip_set_dofs(c, 0, nc, p.Nn, p.dofs.data(), p.ndofs);
+            if (!rc) rc = tile_tables(c, false, c->f[0], nullptr, p.ndofs, hptrs[k], hinds[k],
+                                      batch_slots(p.kind, 256), 256, tabs[1][k]);
+            if (rc) return rc;
+        }
     }
     b->total_nnz = val_base;
     b->total_ptr = ptr_base;
@@ -362,9 +391,14 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
                 const BatchProb &p = b->probs[k];
                 if (p.kind != kind) continue;
                 const ReuseTables &t = tabs[bi][k];
-                const int task0 = (int)cat.elem.size();
+                // tile .x: the first task, or in row-owner tiles the first tile-local node
+                const int task0 = (int)(t.rw ? cat.txy.size() : cat.elem.size());
                 cat.ncolp = t.ncolp;
                 cat.tstride = t.tstride;
+                cat.rw = t.rw;
+                cat.rrec.insert(cat.rrec.end(), t.rrec.begin(), t.rrec.end());
+                cat.tconn.insert(cat.tconn.end(), t.tconn.begin(), t.tconn.end());
+                cat.txy.insert(cat.txy.end(), t.txy.begin(), t.txy.end());
                 cat.maxv = std::max(cat.maxv, t.maxv);
                 cat.maxrank = std::max(cat.maxrank, t.maxrank);
                 for (size_t i = 0; i < t.tiles.size(); i++) {
@@ -419,6 +453,15 @@ extern "C" int elfel_hip_batch_build(elfel_hip_ctx *c, int autotune)
     key += ":nnz" + std::to_string((long long)b->total_nnz);
     int block = 256, gmult = 0;
     bool cached = tune_lookup(key, &block, &gmult);
+    // ELFEL_HIP_BATCH_BLOCK=128 or 256: that workgroup size, one workgroup
+    // per tile, no timing (tests of each tile form)
+    const int forced = env_int("ELFEL_HIP_BATCH_BLOCK", 0);
+    if (forced == 128 || forced == 256) {
+        block = forced;
+        gmult = 0;
+        cached = true;
+        autotune = 0;
+    }
     if (autotune || !cached) {
         const int blocks[2] = {128, 256};
         const int gms[5] = {2, 4, 8, 16, 0};

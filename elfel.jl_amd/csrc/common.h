@@ -113,6 +113,8 @@ struct Pattern {
     // row-owner heat tiles: one record of rw words per row (rw 0: task tables),
     // tile_elem with tstride entries per tile
     Buf rrec;
+    Buf tconn, txy;              // row-owner heat tiles: tile-local nodes (ushort4 per slot) and coordinates
+    bool tgeom = false;          // tconn / txy hold the tile-local geometry
     int rw = 0, tstride = 0;
 };
 

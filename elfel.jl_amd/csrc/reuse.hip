@@ -92,6 +92,7 @@ struct TileInput {
     const int32_t *seq;  // patch tiles: the rows in packing order (null: dof order)
     int nen, nc, ncol, ncolp, cap, ucap, vcap;
     int rw, kmax;  // row-owner heat tiles: words per row record, most elements per row (rw 0: task tables)
+    const double2 *hxy;  // row-owner heat tiles: host coordinates for the tile-local geometry (null: none)
     int64_t n;
     const Field *F0, *F1;
     const Adj *A0, *A1;
@@ -128,6 +129,7 @@ int tile_range(const TileInput &I, int64_t r0, int64_t r1, ReuseTables &T, std::
     std::vector<int> rrow, rcnt, rvb;
     bool row_err = false;
     std::vector<int> ord, newidx, newvb;
+    std::vector<int32_t> lnodes;
     auto close = [&]() {
         if (!open) return;
         if (I.seq) {
@@ -236,6 +238,29 @@ int tile_range(const TileInput &I, int64_t r0, int64_t r1, ReuseTables &T, std::
             T.elem.resize((size_t)cur.x);
             T.info.resize((size_t)cur.x);
             T.pos.resize((size_t)cur.x * ncolp);
+            if (I.hxy) {
+                // tile-local geometry: the distinct nodes of the tile's elements
+                // (ascending), their coordinates from txy[cur.x], and per element
+                // slot its nodes' indices in that list
+                const size_t e0 = T.telem.size() - (size_t)T.tstride;
+                lnodes.clear();
+                for (int u = 0; u < cr.w; u++)
+                    for (int k = 0; k < I.nen; k++)
+                        lnodes.push_back(F0.h_conn[(int64_t)T.telem[e0 + u] * I.nen + k]);
+                std::sort(lnodes.begin(), lnodes.end());
+                lnodes.erase(std::unique(lnodes.begin(), lnodes.end()), lnodes.end());
+                cur.x = (int)T.txy.size();
+                for (int32_t nd : lnodes) T.txy.push_back(I.hxy[nd]);
+                for (int u = 0; u < T.tstride; u++) {
+                    uint16_t l[4] = {0, 0, 0, 0};
+                    if (u < cr.w)
+                        for (int k = 0; k < I.nen; k++)
+                            l[k] = (uint16_t)(std::lower_bound(lnodes.begin(), lnodes.end(),
+                                                               F0.h_conn[(int64_t)T.telem[e0 + u] * I.nen + k]) -
+                                              lnodes.begin());
+                    T.tconn.push_back(make_ushort4(l[0], l[1], l[2], l[3]));
+                }
+            }
         } else {
             // pad to cap tasks, so that tile t's tasks start at t * cap (the kernel
             // computes a thread's task index without reading the tile first)
@@ -465,6 +490,14 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     // (heat_rows_tile) in place of its tasks' tables; the tiles hold up to
     // HEAT_ROWS_BLOCK rows and as many distinct elements
     I.rw = I.kmax = 0;
+    I.hxy = nullptr;
+    std::vector<double2> hxy;  // host copy of the coordinates (tile-local geometry, locality order)
+    auto fetch_xy = [&]() -> int {
+        if (hxy.size() == (size_t)F0.Nn) return 0;
+        hxy.resize((size_t)F0.Nn);
+        if (F0.Nn > 0) HIPCHK(c, hipMemcpy(hxy.data(), F0.xy.p, 16 * (size_t)F0.Nn, hipMemcpyDeviceToHost));
+        return 0;
+    };
     if (rows && I.staged && (nen == 3 || nen == 4)) {
         const int kmax = nen == 4 ? hrow_kmax<4>() : hrow_kmax<3>();
         int maxdeg = 0;
@@ -477,6 +510,12 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
             I.cap = HEAT_ROWS_BLOCK * kmax;
             I.ucap = HEAT_ROWS_BLOCK;
             I.vcap = REUSE_VCAP_PER_THREAD * HEAT_ROWS_BLOCK;
+            // ELFEL_HIP_TILE_GEOM=0: the kernel gathers through conn and xy
+            if (F0.sdim == 2 && env_int("ELFEL_HIP_TILE_GEOM", 1) != 0) {
+                int rc = fetch_xy();
+                if (rc) return rc;
+                I.hxy = hxy.data();
+            }
         }
     }
     // patch tiles for the element-staged (scalar) problems of 2^18 rows or
@@ -484,8 +523,8 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     std::vector<int32_t> seq;
     I.seq = nullptr;
     if (patch && I.staged && nc == 1 && F0.sdim == 2 && n >= (int64_t(1) << 18)) {
-        std::vector<double2> hxy((size_t)F0.Nn);
-        if (F0.Nn > 0) HIPCHK(c, hipMemcpy(hxy.data(), F0.xy.p, 16 * (size_t)F0.Nn, hipMemcpyDeviceToHost));
+        int rc = fetch_xy();
+        if (rc) return rc;
         std::vector<int32_t> node_order;
         band_order(hxy, node_order);
         // the rows in the order of their nodes; rows without a node last
@@ -543,8 +582,11 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     T.tiles.reserve(ntiles); T.trows.reserve(ntiles);
     T.elem.reserve(nslots); T.info.reserve(nslots); T.pos.reserve(nslots * I.ncolp); T.telem.reserve(ntel);
     for (const ReuseTables &p : part) {
-        const int off = (int)T.elem.size();
+        // tile .x: the first task, or in row-owner tiles the first tile-local node
+        const int off = (int)(T.rw ? T.txy.size() : T.elem.size());
         for (int4 td : p.tiles) { td.x += off; T.tiles.push_back(td); }
+        T.tconn.insert(T.tconn.end(), p.tconn.begin(), p.tconn.end());
+        T.txy.insert(T.txy.end(), p.txy.begin(), p.txy.end());
         T.trows.insert(T.trows.end(), p.trows.begin(), p.trows.end());
         T.elem.insert(T.elem.end(), p.elem.begin(), p.elem.end());
         T.info.insert(T.info.end(), p.info.begin(), p.info.end());
@@ -611,13 +653,14 @@ void order_tiles(ReuseTables &T, int nxcd, int cap, int run)
     std::vector<uint8_t> pos(T.pos.size());
     std::vector<int32_t> telem(T.telem.size());
     std::vector<uint32_t> rrec(T.rrec.size());
+    std::vector<ushort4> tconn(T.tconn.size());
     std::vector<int2> truns(T.truns.size());
     const size_t ncolp = (size_t)T.ncolp, ts = (size_t)T.tstride, rs = (size_t)HEAT_ROWS_BLOCK * (size_t)T.rw;
     for (size_t b = 0; b < nt; b++) {
         const size_t o = (size_t)key[src[b]].second;
         tiles[b] = T.tiles[o];
         trows[b] = T.trows[o];
-        tiles[b].x = (int)(b * cap);  // the tasks move with their tile
+        if (!T.rw) tiles[b].x = (int)(b * cap);  // the tasks move with their tile (row-owner: .x stays)
         if (!T.truns.empty()) truns[b] = T.truns[o];  // the runs stay, their ranges move
         if (!T.rw) {  // row-owner tiles keep no tasks
             std::copy(T.elem.begin() + o * cap, T.elem.begin() + (o + 1) * cap, elem.begin() + b * cap);
@@ -628,6 +671,8 @@ void order_tiles(ReuseTables &T, int nxcd, int cap, int run)
             std::copy(T.telem.begin() + o * ts, T.telem.begin() + (o + 1) * ts, telem.begin() + b * ts);
         if (!T.rrec.empty())
             std::copy(T.rrec.begin() + o * rs, T.rrec.begin() + (o + 1) * rs, rrec.begin() + b * rs);
+        if (!T.tconn.empty())
+            std::copy(T.tconn.begin() + o * ts, T.tconn.begin() + (o + 1) * ts, tconn.begin() + b * ts);
     }
     T.tiles.swap(tiles);
     T.trows.swap(trows);
@@ -636,6 +681,7 @@ void order_tiles(ReuseTables &T, int nxcd, int cap, int run)
     T.pos.swap(pos);
     T.telem.swap(telem);
     T.rrec.swap(rrec);
+    T.tconn.swap(tconn);
     T.truns.swap(truns);
 }
 
@@ -652,6 +698,18 @@ int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P)
             (rc = ensure(c, P.rrec, 4 * std::max<size_t>(T.rrec.size(), 1))) ||
             (rc = ensure(c, P.tile_elem, 4 * std::max<size_t>(T.telem.size(), 1))))
             return rc;
+        if (!T.tconn.empty()) {
+            if ((rc = ensure(c, P.tconn, sizeof(ushort4) * T.tconn.size())) ||
+                (rc = ensure(c, P.txy, sizeof(double2) * std::max<size_t>(T.txy.size(), 1))))
+                return rc;
+            HIPCHK(c, hipMemcpy(P.tconn.p, T.tconn.data(), sizeof(ushort4) * T.tconn.size(), hipMemcpyHostToDevice));
+            if (!T.txy.empty())
+                HIPCHK(c, hipMemcpy(P.txy.p, T.txy.data(), sizeof(double2) * T.txy.size(), hipMemcpyHostToDevice));
+            P.tgeom = true;
+        } else {
+            release(P.tconn); release(P.txy);
+            P.tgeom = false;
+        }
         P.has_runs = !T.truns.empty();
         if (P.has_runs) {
             if ((rc = ensure(c, P.truns, sizeof(int2) * T.truns.size())) ||
@@ -679,7 +737,8 @@ int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P)
         P.maxu = T.maxu;
         return 0;
     }
-    release(P.rrec);
+    release(P.rrec); release(P.tconn); release(P.txy);
+    P.tgeom = false;
     const int64_t ntasks = (int64_t)T.elem.size();
     int rc;
     if ((rc = ensure(c, P.tile, sizeof(int4) * std::max<size_t>(T.tiles.size(), 1))) ||
@@ -874,6 +933,8 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
         A.runs = (const int4 *)P.runs.p;
     }
     A.rrec = P.rw ? (const uint32_t *)P.rrec.p : nullptr;
+    A.tconn = P.rw && P.tgeom ? (const ushort4 *)P.tconn.p : nullptr;
+    A.txy = P.rw && P.tgeom ? (const double2 *)P.txy.p : nullptr;
     A.tstride = P.tstride;
     tm_begin(c, ELFEL_PH_NUMERIC, &ev);
     rc = fn(c, A, cf, tab, P.lds_vals);

@@ -69,6 +69,13 @@ struct ReuseArgs {
     // BLOCK records per tile; tile_elem holds tstride entries per tile
     const uint32_t *rrec = nullptr;
     int tstride = 0;
+    // row-owner heat tiles with tile-local geometry: the coordinates of the
+    // nodes of each tile's elements, copied once per mesh into one contiguous
+    // range per tile (txy, from tiles[t].x), and each element slot's nodes as
+    // 16-bit indices into that range (tconn, tstride entries per tile; null:
+    // the kernel gathers through conn and xy)
+    const ushort4 *tconn = nullptr;
+    const double2 *txy = nullptr;
 };
 
 // The owner's entries of one element block.  The owner node a and its
@@ -714,15 +721,27 @@ __device__ __forceinline__ void heat_rows_tile(const ReuseArgs &A, const Coef &c
         const uint32_t ec = (uint32_t)padded32(eg, cf.cgap);
         if (cf.kappa_e) kap = cf.kappa_e[ec];
         if (cf.Q_e) Qv = cf.Q_e[ec];
-        int32_t nd[NEN];
-        const uint32_t en = (uint32_t)padded32(eg, A.ngap) * NEN;
-#pragma unroll
-        for (int k = 0; k < NEN; k++) nd[k] = A.conn[en + k];
         double X[NEN], Y[NEN];
+        if (A.tconn) {
+            // tile-local geometry: no dependent load through the element number
+            const ushort4 ln = A.tconn[(int64_t)tile * A.tstride + tid];
+            const double2 *lxy = A.txy + td.x;
+            const uint32_t l[4] = {ln.x, ln.y, ln.z, ln.w};
 #pragma unroll
-        for (int k = 0; k < NEN; k++) {
-            const double2 p = A.xy[(uint32_t)padded32(nd[k], A.xgap)];
-            X[k] = p.x; Y[k] = p.y;
+            for (int k = 0; k < NEN; k++) {
+                const double2 p = lxy[l[k]];
+                X[k] = p.x; Y[k] = p.y;
+            }
+        } else {
+            int32_t nd[NEN];
+            const uint32_t en = (uint32_t)padded32(eg, A.ngap) * NEN;
+#pragma unroll
+            for (int k = 0; k < NEN; k++) nd[k] = A.conn[en + k];
+#pragma unroll
+            for (int k = 0; k < NEN; k++) {
+                const double2 p = A.xy[(uint32_t)padded32(nd[k], A.xgap)];
+                X[k] = p.x; Y[k] = p.y;
+            }
         }
         if constexpr (sizeof(R) == 4) {
             const double x0 = X[0], y0 = Y[0];
@@ -804,7 +823,8 @@ void k_reuse(ReuseArgs A, Coef cf, QTable tab, int lds_vals)
 // kinds: heat T3 1-point (tiles 0 .. n0-1), heat Q4 2x2 (n0 .. n1-1) and
 // plane elasticity T6 3-point (n1 .. nt-1); each workgroup walks the tile list
 // with stride gridDim.x, and the tile kind is uniform per workgroup.  Heat
-// tiles hold two tasks per thread, elasticity tiles one.  R = float: the
+// tiles are row-owner tiles in workgroups of 256 (two tasks per thread in
+// workgroups of 128), elasticity tiles hold one task per thread.  R = float: the
 // mixed arithmetic; R = double: the FP64 arithmetic (built without
 // contraction, equal to the single-problem FP64 kernels bit for bit).  The
 // arguments are passed by value: kept in device memory and loaded at each
@@ -813,6 +833,22 @@ void k_reuse(ReuseArgs A, Coef cf, QTable tab, int lds_vals)
 // KINDS: the tile kinds compiled in (1 heat T3, 2 heat Q4, 4 elasticity T6);
 // a kind left out takes no registers (with all three the mixed kernel needs
 // 129 VGPRs and drops to 3 waves per SIMD, heat T3 + elasticity T6 alone 127).
+// A heat tile of the batch: row-owner records (heat_rows_tile) when the
+// kind's tables carry them (workgroups of HEAT_ROWS_BLOCK threads only), else
+// two tasks per thread.
+template <typename R, int NEN, int NQ, bool CSC, int BLOCK>
+__device__ __forceinline__ void batch_heat_tile(const ReuseArgs &A, const Coef &cf, const QTable &tab, double *lds,
+                                                int lds_vals, int tile)
+{
+    if constexpr (BLOCK == HEAT_ROWS_BLOCK) {
+        if (A.rrec) {
+            heat_rows_tile<R, NEN, NQ, CSC, BLOCK>(A, cf, tab, lds, lds_vals, tile);
+            return;
+        }
+    }
+    reuse_tile<ELFEL_HEAT, R, NEN, 1, NQ, CSC, BLOCK, 2>(A, cf, tab, lds, lds_vals, tile);
+}
+
 template <typename R, bool CSC, int BLOCK, int KINDS>
 __global__ __launch_bounds__(BLOCK) void k_reuse_batch(ReuseArgs H3, ReuseArgs Ev, ReuseArgs H4, Coef ch3, Coef ce,
                                                        Coef ch4, int n0, int n1, int nt, int lds_vals)
@@ -821,10 +857,9 @@ __global__ __launch_bounds__(BLOCK) void k_reuse_batch(ReuseArgs H3, ReuseArgs E
     constexpr QTable t3 = qt_fixed<3, 1>(), t6 = qt_fixed<6, 3>(), t4 = qt_fixed<4, 4>();
     for (int t = blockIdx.x; t < nt; t += gridDim.x) {
         if (t < n0) {
-            if constexpr ((KINDS & 1) != 0) reuse_tile<ELFEL_HEAT, R, 3, 1, 1, CSC, BLOCK, 2>(H3, ch3, t3, lds, lds_vals, t);
+            if constexpr ((KINDS & 1) != 0) batch_heat_tile<R, 3, 1, CSC, BLOCK>(H3, ch3, t3, lds, lds_vals, t);
         } else if (t < n1) {
-            if constexpr ((KINDS & 2) != 0)
-                reuse_tile<ELFEL_HEAT, R, 4, 1, 4, CSC, BLOCK, 2>(H4, ch4, t4, lds, lds_vals, t - n0);
+            if constexpr ((KINDS & 2) != 0) batch_heat_tile<R, 4, 4, CSC, BLOCK>(H4, ch4, t4, lds, lds_vals, t - n0);
         } else {
             if constexpr ((KINDS & 4) != 0)
                 reuse_tile<ELFEL_ELAST, R, 6, 2, 3, CSC, BLOCK, 1>(Ev, ce, t6, lds, lds_vals, t - n1);
@@ -844,10 +879,9 @@ __global__ __launch_bounds__(BLOCK) void k_reuse_batch1(ReuseArgs H3, ReuseArgs 
     constexpr QTable t3 = qt_fixed<3, 1>(), t6 = qt_fixed<6, 3>(), t4 = qt_fixed<4, 4>();
     const int t = blockIdx.x;
     if (t < n0) {
-        if constexpr ((KINDS & 1) != 0) reuse_tile<ELFEL_HEAT, R, 3, 1, 1, CSC, BLOCK, 2>(H3, ch3, t3, lds, lds_vals, t);
+        if constexpr ((KINDS & 1) != 0) batch_heat_tile<R, 3, 1, CSC, BLOCK>(H3, ch3, t3, lds, lds_vals, t);
     } else if (t < n1) {
-        if constexpr ((KINDS & 2) != 0)
-            reuse_tile<ELFEL_HEAT, R, 4, 1, 4, CSC, BLOCK, 2>(H4, ch4, t4, lds, lds_vals, t - n0);
+        if constexpr ((KINDS & 2) != 0) batch_heat_tile<R, 4, 4, CSC, BLOCK>(H4, ch4, t4, lds, lds_vals, t - n0);
     } else {
         if constexpr ((KINDS & 4) != 0)
             reuse_tile<ELFEL_ELAST, R, 6, 2, 3, CSC, BLOCK, 1>(Ev, ce, t6, lds, lds_vals, t - n1);
@@ -919,6 +953,8 @@ struct ReuseTables {
     std::vector<uint8_t> pos;
     std::vector<int32_t> telem;  // heat: distinct elements per tile (padded to the slot count)
     std::vector<uint32_t> rrec;  // row-owner heat tiles: HEAT_ROWS_BLOCK records of rw words per tile
+    std::vector<ushort4> tconn;  // row-owner heat tiles: tile-local nodes per element slot (tstride per tile)
+    std::vector<double2> txy;    // row-owner heat tiles: tile-local coordinates (tile t's from tiles[t].x)
     int rw = 0;                  // words per row record (0: task tables)
     int tstride = 0;             // entries of telem per tile
     int ncolp = 0, maxv = 0, maxrank = 0, maxu = 0;

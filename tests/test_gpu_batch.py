@@ -103,6 +103,68 @@ def test_batch_fp64_with_q4_equals_oracle_bitwise(layout):
     ctx.close()
 
 
+@pytest.mark.parametrize("block", [128, 256])
+def test_batch_fp64_each_tile_form_bitwise(monkeypatch, block):
+    """The batch at each workgroup size, no timing: heat tiles are row-owner
+    tiles at 256 and task tiles at 128; both equal the oracle bit for bit."""
+    monkeypatch.setenv("ELFEL_HIP_BATCH_BLOCK", str(block))
+    ctx = eh._lib.Context(0)
+    probs = _small_batch(ctx, MIXED_SPECS, precision=eh._lib.FP64)
+    ctx.batch_build(autotune=False)
+    ctx.batch_assemble()
+    assert ctx.batch_info(0)["block"] == block
+    for k, p in enumerate(probs):
+        ptr, ind, val, F = ctx.batch_copy(k)
+        rptr, rind, rval, rF = _oracle(p)
+        assert np.array_equal(ptr, rptr) and np.array_equal(ind, rind), k
+        assert np.array_equal(val, rval), (k, p["kind"])
+        if p["kind"] != "elast":
+            assert np.array_equal(F, rF), k
+    ctx.close()
+
+
+def _fan(K):
+    """T3 fan: a centre node in K triangles inside two rings of K nodes."""
+    xy = [(0.0, 0.0)]
+    for r in (1, 2):
+        for k in range(K):
+            t = 2 * np.pi * (k + 0.3 * (r % 2)) / K
+            xy.append((r * np.cos(t), r * np.sin(t)))
+    node = lambda r, k: 1 + (r - 1) * K + (k % K)  # noqa: E731
+    conn = [(0, node(1, k), node(1, k + 1)) for k in range(K)]
+    for k in range(K):
+        a, b, c, d = node(1, k), node(1, k + 1), node(2, k + 1), node(2, k)
+        conn += [(a, d, c), (a, c, b)]
+    return np.array(conn, np.int64) + 1, np.array(xy)
+
+
+@pytest.mark.parametrize("precision", [eh._lib.FP64, eh._lib.MIXED])
+def test_batch_heat_kind_falls_back_to_task_tiles(monkeypatch, precision):
+    """A heat T3 problem with a node in 12 elements (more than a row record
+    holds, fewer than the task tiles' 16) puts every heat T3 problem of the
+    batch on task tiles; all still match the oracle (FP64: bit for bit)."""
+    monkeypatch.setenv("ELFEL_HIP_BATCH_BLOCK", "256")
+    ctx = eh._lib.Context(0)
+    probs = _small_batch(ctx, [("heat", 9), ("heatq4", 7), ("elast", 4)], precision=precision)
+    conn, xy = _fan(12)
+    dn = np.arange(1, xy.shape[0] + 1, dtype=np.int64).reshape(-1, 1)
+    Q = probs[0]["Q"]  # problems of one kind share their uniform coefficients
+    ctx.batch_add(eh._lib.HEAT, eh._lib.T3, 1, conn, xy, dn, xy.shape[0], kappa=1.0, Q=Q, precision=precision,
+                  layout=eh._lib.CSC)
+    ctx.batch_build(autotune=False)
+    ctx.batch_assemble()
+    refs = [_oracle(p) for p in probs]
+    refs.append(O.assemble(O.make_problem(O.HEAT, O.T3, 1, conn, xy, dn, kappa=1.0, Q=Q), xy.shape[0], csc=True))
+    for k, ref in enumerate(refs):
+        ptr, ind, val, F = ctx.batch_copy(k)
+        assert np.array_equal(ptr, ref[0]) and np.array_equal(ind, ref[1]), k
+        if precision == eh._lib.FP64:
+            assert np.array_equal(val, ref[2]), k
+        else:
+            assert frob_rel(val, ref[2]) <= 1e-6, k
+    ctx.close()
+
+
 def test_batch_mixed_with_q4_and_replays():
     """A mixed batch with heat Q4 problems: within 1e-6 of the oracle, and
     graph replays give the first result's bits."""
