@@ -809,6 +809,29 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
         last = now;
     };
     lap(nullptr);
+    // vector problems (elasticity, Stokes): the whole phase on the device
+    // (ELFEL_HIP_DEVICE_SYMBOLIC=0: the host builder below)
+    if ((pb->kind == ELFEL_ELAST || stokes) && F0.ncomp == 2 && env_int("ELFEL_HIP_DEVICE_SYMBOLIC", 1) != 0) {
+        int rc = device_symbolic(c, pb, block);
+        if (rc) return rc;
+        lap("device symbolic");
+        if (!P.fallback) {
+            if ((rc = ensure(c, P.conn_pad, 4 * (size_t)F0.nen * padded_len(F0.E, CONN_GAP))) ||
+                (rc = ensure(c, P.xy_pad, 16 * (size_t)padded_len(F0.Nn, XY_GAP))))
+                return rc;
+            if ((rc = copy_padded(c, P.conn_pad.p, F0.conn.p, F0.E, 4 * (size_t)F0.nen, CONN_GAP)) ||
+                (rc = copy_padded(c, P.xy_pad.p, F0.xy.p, F0.Nn, 16, XY_GAP)))
+                return rc;
+        }
+        P.block = block * reuse_tpt(pb->kind);
+        P.kind = pb->kind; P.fe = pb->fe; P.layout = pb->layout;
+        P.key_mesh0 = F0.mesh_version; P.key_dof0 = F0.dof_version;
+        P.key_mesh1 = stokes ? F1->mesh_version : 0; P.key_dof1 = stokes ? F1->dof_version : 0;
+        P.nrow = P.ncol = n;
+        P.valid = true;
+        P.symbolic_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return 0;
+    }
     std::vector<int32_t> hptr, hind;
     int rc = pattern_to_host(c, pb, tab, cf, hptr, hind);
     if (rc) return rc;

@@ -965,6 +965,9 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
                 const std::vector<int32_t> &hptr, const std::vector<int32_t> &hind, int cap, int ucap,
                 ReuseTables &T, bool patch = false, bool rows = false);
 int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P);
+// symbolic.hip: the pattern and task tables of an elasticity or Stokes problem
+// built on the device (c->pat; fallback set when the mesh does not fit)
+int device_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int block);
 // Sorts the tiles by their first element and lays them out so that workgroup b
 // (dispatched to XCD b mod nxcd) walks a contiguous chunk of the sorted list
 // (run = 0), or runs of `run` tiles dealt to the XCDs in turn.
