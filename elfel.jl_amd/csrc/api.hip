@@ -208,7 +208,7 @@ extern "C" int elfel_hip_free(elfel_hip_ctx *c)
                    &c->fvals[1], &c->stage64, &c->segpos, &c->ptr, &c->ind, &c->val, &c->F, &c->dscal, &c->pcg, &c->pcg_sell,
                    &c->pat.ptr, &c->pat.ind, &c->pat.tile, &c->pat.tile_rows, &c->pat.task_elem,
                    &c->pat.task_info, &c->pat.task_pos, &c->pat.tile_elem, &c->pat.conn_pad, &c->pat.xy_pad,
-                   &c->pat.truns, &c->pat.runs, &c->pat.rrec, &c->pat.tconn, &c->pat.txy, &c->pat.sym};
+                   &c->pat.truns, &c->pat.runs, &c->pat.rrec, &c->pat.tconn, &c->pat.txy, &c->pat.sym, &c->pat.sym2};
     for (Buf *b : bufs) release(*b);
     batch_free(c);
     for (auto &p : c->tm.pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }

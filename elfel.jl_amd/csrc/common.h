@@ -114,7 +114,7 @@ struct Pattern {
     // tile_elem with tstride entries per tile
     Buf rrec;
     Buf tconn, txy;              // row-owner heat tiles: tile-local nodes (ushort4 per slot) and coordinates
-    Buf sym;                     // scratch of the device symbolic phase (symbolic.hip)
+    Buf sym, sym2;               // scratch of the device symbolic phase (symbolic.hip)
     bool tgeom = false;          // tconn / txy hold the tile-local geometry
     int rw = 0, tstride = 0;
 };

@@ -14,22 +14,25 @@
 //      distinct dofs;
 //   3. each node's sorted column list: the dofs of every element at the node
 //      (plus the element's pressure dofs for Stokes velocity rows), sorted
-//      and made unique by one wavefront in LDS; the rows of a node share it.
-//      The pattern of sparse() on the element blocks is this union, so ptr is
-//      the scan of the row lengths and ind the concatenated lists.  The blocks
+//      and made unique by one wavefront in registers (a bitonic network over
+//      shuffles); the rows of a node share it.  The pattern of sparse() on
+//      the element blocks is this union, so ptr is the scan of the row
+//      lengths; a second sort writes the lists into ind and, for every
+//      (element, column) at the node, its position in the list.  The blocks
 //      are symmetric (elasticity K; Stokes K, B and B^T), so the CSC arrays
 //      are the CSR arrays;
 //   4. greedy tile packing of row groups (a node's two rows together) in dof
-//      order, one thread per range of 2048 rows: the same limits as the host
-//      builder (tasks <= slots, values <= the LDS cap, rows <= 256), a tile
-//      never spans two ranges;
-//   5. the task tables, one workgroup per tile: within a tile the tasks are
-//      laid out rank by rank (each rank step's threads in as few waves as
-//      possible), each task's column positions found by binary search in its
-//      row and the first contribution to each entry flagged;
-//   6. tiles sorted by their first element (the host builder's order) and
+//      order, one wavefront per range of 2048 rows (wave prefix sums find
+//      where each tile overflows): the same limits as the host builder
+//      (tasks <= slots, values <= the LDS cap, rows <= 256), a tile never
+//      spans two ranges;
+//   5. tiles sorted by their first element (the host builder's order) and
 //      placed so that launch slot b (XCD b mod 8) walks a contiguous eighth
-//      of the sorted list.
+//      of the sorted list;
+//   6. the task tables, one workgroup per tile: within a tile the tasks are
+//      laid out rank by rank (each rank step's threads in as few waves as
+//      possible), with their column positions from step 3 and the first
+//      contribution to each entry flagged.
 //
 // Nothing depends on the tile packing but speed: every value is the sum of
 // its contributions in element order, whatever tile owns it, so the FP64
@@ -41,7 +44,7 @@ namespace {
 constexpr int SY_CMAX = 128;  // longest row (entries): the tasks' 7-bit positions
 constexpr int SY_SEG = 2048;  // rows per packing range
 
-enum { SYF_DEG = 1, SYF_LONG = 2, SYF_SHARED = 4, SYF_REPEAT = 8, SYF_PAIR = 16 };
+enum { SYF_DEG = 1, SYF_LONG = 2, SYF_SHARED = 4, SYF_REPEAT = 8 };
 enum { SYS_FLAGS = 0, SYS_MAXV = 1, SYS_MAXRANK = 2, SYS_N = 4 };
 
 struct SymIn {
@@ -130,80 +133,203 @@ __global__ void k_sy_inv(SymIn S, int32_t *__restrict__ inv_node, uint8_t *__res
 }
 
 // an element whose columns repeat a dof (a task reads all of its partial
-// sums before it writes any, so its columns must be distinct)
-__global__ void k_sy_elem_check(SymIn S, uint32_t *__restrict__ scal)
+// sums before it writes any, so its columns must be distinct); edof keeps
+// every element's columns (field 0 order) for the node lists
+__global__ void k_sy_elem_check(SymIn S, int pstride, int32_t *__restrict__ edof, uint32_t *__restrict__ scal)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= S.E) return;
     const int nc0 = sy_ncol(S, 0);
     int32_t c[16];
-    for (int j = 0; j < nc0; j++) c[j] = sy_col(S, e, j);
+    for (int j = 0; j < nc0; j++) {
+        c[j] = sy_col(S, e, j);
+        edof[e * pstride + j] = c[j];  // the element's columns, for the node lists
+    }
     bool rep = false;
     for (int j = 1; j < nc0; j++)
         for (int i = 0; i < j; i++) rep |= c[i] == c[j];
     if (rep) atomicOr(&scal[SYS_FLAGS], (uint32_t)SYF_REPEAT);
 }
 
-// One wavefront (a workgroup of 64) per node: the columns of every element
-// at the node, bitonic-sorted in LDS, duplicates dropped; the list goes to
-// lists[node * SY_CMAX ..] and its length to nodeL[node].
-__global__ __launch_bounds__(64) void k_sy_node_cols(SymIn S, int64_t N, const uint32_t *__restrict__ aptr,
-                                                     const int32_t *__restrict__ aelem, int32_t *__restrict__ lists,
-                                                     uint32_t *__restrict__ nodeL, uint32_t *__restrict__ scal)
+// Bitonic sort of 64 R values across one wavefront, value i = r * 64 + lane
+// in register r of lane i mod 64: exchanges between lanes by shuffles,
+// between registers in place; no LDS, no barrier.
+__device__ __forceinline__ uint64_t sy_shfl_xor(uint64_t v, int m)
 {
-    __shared__ int32_t s[256];
-    const int64_t nd = blockIdx.x;
-    const int lane = threadIdx.x;
-    const uint32_t p0 = aptr[nd], deg = aptr[nd + 1] - p0;
-    if (deg > 16) {  // flagged by k_sy_adj_sort
-        if (lane == 0) nodeL[nd] = 0;
-        return;
-    }
-    const int field = nd >= S.Nn0 ? 1 : 0;
-    const int ncol = sy_ncol(S, field);
-    const int cnt = (int)deg * ncol;  // <= 16 x 15
-    int size = 64;
-    while (size < cnt) size <<= 1;
-    for (int i = lane; i < size; i += 64) {
-        int32_t v = INT32_MAX;
-        if (i < cnt) v = sy_col(S, aelem[p0 + i / ncol], i % ncol);
-        s[i] = v;
-    }
-    __syncthreads();
-    for (int k = 2; k <= size; k <<= 1)
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, m, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t sy_shfl(uint64_t v, int src)
+{
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)v, src, 64);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(v >> 32), src, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t sy_shfl_up1(uint64_t v)
+{
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)v, 1, 64);
+    const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(v >> 32), 1, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t sy_shfl_xor(uint32_t v, int m) { return (uint32_t)__shfl_xor((int)v, m, 64); }
+__device__ __forceinline__ uint32_t sy_shfl(uint32_t v, int src) { return (uint32_t)__shfl((int)v, src, 64); }
+__device__ __forceinline__ uint32_t sy_shfl_up1(uint32_t v) { return (uint32_t)__shfl_up((int)v, 1, 64); }
+
+template <int R, typename K>
+__device__ __forceinline__ void sy_wave_sort(K (&v)[R], int lane)
+{
+#pragma unroll
+    for (int k = 2; k <= 64 * R; k <<= 1) {
+#pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int p = lane; p < (size >> 1); p += 64) {
-                const int i = (p / j) * 2 * j + (p % j), q = i + j;
-                const int32_t a = s[i], b = s[q];
-                const bool up = (i & k) == 0;
-                if ((a > b) == up) { s[i] = b; s[q] = a; }
+            if (j >= 64) {
+                const int rj = j >> 6;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if ((r & rj) != 0) continue;
+                    const bool up = ((r * 64 + lane) & k) == 0;
+                    const K a = v[r], b = v[r | rj];
+                    const bool sw = (a > b) == up;
+                    v[r] = sw ? b : a;
+                    v[r | rj] = sw ? a : b;
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const K p = sy_shfl_xor(v[r], j);
+                    const bool up = ((r * 64 + lane) & k) == 0;
+                    const bool takemin = ((lane & j) == 0) == up;
+                    v[r] = takemin ? (v[r] < p ? v[r] : p) : (v[r] < p ? p : v[r]);
+                }
             }
-            __syncthreads();
         }
-    int32_t *out = lists + nd * SY_CMAX;
-    int run = 0;
-    for (int base = 0; base < size; base += 64) {
-        const int i = base + lane;
-        const int32_t v = s[i];
-        const bool keep = v != INT32_MAX && (i == 0 || v != s[i - 1]);
-        const uint64_t m = __ballot(keep);
-        const int pos = run + __popcll(m & ((1ull << lane) - 1ull));
-        if (keep && pos < SY_CMAX) out[pos] = v;
-        run += __popcll(m);
-    }
-    if (lane == 0) {
-        nodeL[nd] = (uint32_t)run;
-        if (run > SY_CMAX) atomicOr(&scal[SYS_FLAGS], (uint32_t)SYF_LONG);
     }
 }
 
-// Row r: length of its node's list; rinfo packs what the packing walk reads:
+// One wavefront per node.  Pass 0: the length of the node's column list
+// (nodeL).  Pass 1: the columns of every element at the node (key =
+// dof << 8 | candidate index, candidate = rank * ncol + column), sorted,
+// duplicates of a dof dropped; the list goes into ind at each of the node's
+// rows, and each candidate's position in it into npos (one byte, at
+// (aptr[node] + rank) * pstride + column).
+template <int R, typename K>
+__device__ __forceinline__ void sy_node_cols_body(const SymIn &S, int64_t nd, int lane, uint32_t p0, int cnt, int ncol,
+                                                  const int32_t *__restrict__ aelem, const int32_t *__restrict__ edof,
+                                                  int pass, uint32_t *__restrict__ nodeL,
+                                                  const int32_t *__restrict__ ptr, int32_t *__restrict__ ind,
+                                                  uint8_t *__restrict__ npos, int pstride, uint32_t *__restrict__ scal)
+{
+    // keys: pass 0 node numbers, pass 1 dof << 8 | candidate (32-bit keys
+    // when every dof is below 2^23)
+    constexpr K PAD = (K)~(K)0;
+    K v[R];
+    // pass 0 counts the distinct nodes of the elements at the node (no dof is
+    // shared -- that case falls back -- so each velocity node brings nc dofs
+    // and each pressure node one); pass 1 sorts the dofs themselves
+    const int nnv = S.nen0 + ((S.stokes && nd < S.Nn0) ? 3 : 0);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int i = r * 64 + lane;
+        v[r] = PAD;
+        if (pass == 0) {
+            if (i < cnt) {
+                const int64_t e = aelem[p0 + i / nnv];
+                const int k = i % nnv;
+                v[r] = k < S.nen0 ? (K)S.conn0[e * S.nen0 + k] : (K)(S.Nn0 + S.conn1[e * 3 + (k - S.nen0)]);
+            }
+        } else if (i < cnt) {
+            v[r] = ((K)(uint32_t)edof[(int64_t)aelem[p0 + i / ncol] * pstride + i % ncol] << 8) | (K)i;
+        }
+    }
+    sy_wave_sort<R, K>(v, lane);
+    // a value is kept when it differs from the one before; every candidate's
+    // position is the number of kept values up to it, less one
+    int run = 0;
+    int32_t rows[2] = {-1, -1};
+    int nrows = 0;
+    if (pass == 1) {
+        if (nd < S.Nn0) {
+            nrows = S.nc;
+            for (int c = 0; c < S.nc; c++) rows[c] = S.dof0[nd * S.nc + c];
+        } else {
+            nrows = 1;
+            rows[0] = S.dof1[nd - S.Nn0];
+        }
+    }
+    const int sh = pass == 0 ? 0 : 8;
+    int nl = 0;  // pass 0: the list length
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const K up = sy_shfl_up1(v[r]);
+        const K last = r > 0 ? sy_shfl(v[r > 0 ? r - 1 : 0], 63) : PAD;
+        const K prev = lane == 0 ? last : up;
+        const bool real = v[r] != PAD;
+        const bool keep = real && (r * 64 + lane == 0 || (v[r] >> sh) != (prev >> sh));
+        if (pass == 0) {
+            const uint64_t mv = __ballot(keep && v[r] < (K)S.Nn0), mp = __ballot(keep && v[r] >= (K)S.Nn0);
+            nl += S.nc * __popcll(mv) + __popcll(mp);
+            continue;
+        }
+        const uint64_t m = __ballot(keep);
+        const int incl = run + __popcll(m & ((2ull << lane) - 1ull));  // lanes 0..lane
+        if (pass == 1 && real) {
+            const int pos = incl - 1;
+            if (keep && pos < SY_CMAX)
+                for (int k = 0; k < nrows; k++) ind[ptr[rows[k]] + pos] = (int32_t)(v[r] >> 8);
+            const int cand = (int)(v[r] & 255u);
+            npos[((int64_t)p0 + cand / ncol) * pstride + cand % ncol] = (uint8_t)min(pos, 255);
+        }
+        run += __popcll(m);
+    }
+    if (pass == 0 && lane == 0) {
+        nodeL[nd] = (uint32_t)nl;
+        if (nl > SY_CMAX) atomicOr(&scal[SYS_FLAGS], (uint32_t)SYF_LONG);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sy_node_cols(SymIn S, int64_t N, const uint32_t *__restrict__ aptr,
+                                                      const int32_t *__restrict__ aelem,
+                                                      const int32_t *__restrict__ edof, int pass,
+                                                      uint32_t *__restrict__ nodeL, const int32_t *__restrict__ ptr,
+                                                      int32_t *__restrict__ ind, uint8_t *__restrict__ npos,
+                                                      int pstride, uint32_t *__restrict__ scal)
+{
+    // four nodes per workgroup, one per wavefront (the waves share nothing)
+    const int64_t nd = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (nd >= N) return;
+    const uint32_t p0 = aptr[nd], deg = aptr[nd + 1] - p0;
+    if (deg > 16) {  // flagged by k_sy_adj_sort
+        if (lane == 0 && pass == 0) nodeL[nd] = 0;
+        return;
+    }
+    const int ncol = sy_ncol(S, nd >= S.Nn0 ? 1 : 0);
+    // candidates: pass 0 the nodes, pass 1 the columns of each element (<= 16 x 15)
+    const int cnt = (int)deg * (pass == 0 ? S.nen0 + ((S.stokes && nd < S.Nn0) ? 3 : 0) : ncol);
+#define SY_BODY(RR, KK) \
+    sy_node_cols_body<RR, KK>(S, nd, lane, p0, cnt, ncol, aelem, edof, pass, nodeL, ptr, ind, npos, pstride, scal)
+    if (pass == 0 || S.n <= (1 << 23)) {
+        if (cnt <= 64) SY_BODY(1, uint32_t);
+        else if (cnt <= 128) SY_BODY(2, uint32_t);
+        else SY_BODY(4, uint32_t);
+    } else {
+        if (cnt <= 64) SY_BODY(1, uint64_t);
+        else if (cnt <= 128) SY_BODY(2, uint64_t);
+        else SY_BODY(4, uint64_t);
+    }
+#undef SY_BODY
+}
+
+// Row r: length of its node's list; rinfo packs what the packing walk reads
+// (with gmin, the group's lowest element):
 // 0xFFFFFFFF for the second row of a node's pair, else deg (bits 0-4), rows
 // of the group (bits 5-6), length (bits 8-15), row code (bits 16-17),
 // field (bit 18).
 __global__ void k_sy_rows(SymIn S, const int32_t *__restrict__ inv_node, const uint8_t *__restrict__ inv_fc,
-                          const uint32_t *__restrict__ aptr, const uint32_t *__restrict__ nodeL,
-                          uint32_t *__restrict__ rowL, uint32_t *__restrict__ rinfo, uint32_t *__restrict__ scal)
+                          const uint32_t *__restrict__ aptr, const int32_t *__restrict__ aelem,
+                          const uint32_t *__restrict__ nodeL, uint32_t *__restrict__ rowL, uint32_t *__restrict__ rinfo,
+                          int32_t *__restrict__ gmin)
 {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r > S.n) return;
@@ -219,6 +345,7 @@ __global__ void k_sy_rows(SymIn S, const int32_t *__restrict__ inv_node, const u
     if (second(r)) { rinfo[r] = 0xFFFFFFFFu; return; }
     const int nr = second(r + 1) ? 2 : 1;
     const uint32_t deg = node >= 0 ? aptr[node + 1] - aptr[node] : 0u;
+    gmin[r] = deg > 0 ? aelem[aptr[node]] : INT32_MAX;  // the group's lowest element
     const int field = fc >> 4, comp = fc & 15;
     const int rowcode = nr == 2 ? 2 : (comp == 1 ? 1 : 0);
     rinfo[r] = min(deg, 31u) | ((uint32_t)nr << 5) | (min(L, 255u) << 8) | ((uint32_t)rowcode << 16) |
@@ -234,61 +361,134 @@ __device__ __forceinline__ int64_t sy_seg_start(int64_t k, int64_t n, const uint
     return r;
 }
 
-// Greedy packing of each range's row groups (one thread per range).  Pass 0
-// counts the range's tiles; pass 1 writes each group's tile and LDS offset and
-// each tile's rows and sizes (dof-ordered tile numbers from segbase).
-__global__ void k_sy_pack(int64_t n, int64_t nseg, const uint32_t *__restrict__ rinfo,
-                          const int32_t *__restrict__ ptr, int cap, int vcap, int rowcap, int pass,
-                          uint32_t *__restrict__ segtiles, int32_t *__restrict__ gtile, uint32_t *__restrict__ ginfo,
-                          int4 *__restrict__ tdo, int4 *__restrict__ tro, const int32_t *__restrict__ inv_node,
-                          const uint32_t *__restrict__ aptr, const int32_t *__restrict__ aelem,
-                          uint64_t *__restrict__ tkey, uint64_t *__restrict__ tval, uint32_t *__restrict__ scal)
+// inclusive sums over the wavefront
+__device__ __forceinline__ int sy_wave_incl(int v, int lane)
 {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nseg) return;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+__device__ __forceinline__ int sy_wave_min(int v)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d, 64));
+    return v;
+}
+__device__ __forceinline__ int sy_wave_max(int v)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+// Greedy packing of each range's row groups, one wavefront per range: the
+// rows are read 64 at a time (one per lane); prefix sums of the groups'
+// tasks, values and rows say where the open tile overflows, the tile is
+// closed before that group and the rest of the chunk goes on from it, so the
+// result is the row-by-row greedy walk's.  Each group gets its LDS offset and
+// tile-local row (ginfo); each tile's rows, sizes and first element go to
+// sparse index k * SY_SEG + l (a range of SY_SEG rows has at most SY_SEG
+// tiles); segtiles[k] receives the range's tile count.
+__global__ __launch_bounds__(64) void k_sy_pack(int64_t n, int64_t nseg, const uint32_t *__restrict__ rinfo,
+                                                const int32_t *__restrict__ gmin, const int32_t *__restrict__ ptr,
+                                                int cap, int vcap, int rowcap, uint32_t *__restrict__ segtiles,
+                                                uint32_t *__restrict__ ginfo, int4 *__restrict__ tds,
+                                                int4 *__restrict__ trs, int32_t *__restrict__ tmin,
+                                                uint32_t *__restrict__ scal)
+{
+    const int64_t k = blockIdx.x;
+    const int lane = threadIdx.x;
     const int64_t r0 = sy_seg_start(k, n, rinfo), r1 = sy_seg_start(k + 1, n, rinfo);
-    int64_t t = pass ? (int64_t)segtiles[k] : 0;
+    int64_t t = k * SY_SEG;
     const int64_t tfirst = t;
+    // the open tile (wave-uniform)
     bool open = false;
-    int ty = 0, tw = 0, trows = 0, nranks = 0;
+    int ty = 0, tw = 0, trows = 0, nranks = 0, emin = INT32_MAX;
     int64_t row0 = 0;
-    int32_t emin = INT32_MAX;  // the tile's first element: its sort key
     bool bad = false;
-    auto close = [&]() {
-        if (!open) return;
-        if (pass) {
-            tdo[t] = make_int4((int)row0, ty, ptr[row0], tw);
-            tro[t] = make_int4((int)row0, trows, nranks, 0);
-            tkey[t] = ((uint64_t)(uint32_t)emin << 32) | (uint64_t)t;
-            tval[t] = (uint64_t)t;
+    for (int64_t c0 = r0; c0 < r1; c0 += 64) {
+        const int64_t r = c0 + lane;
+        const uint32_t ri = r < r1 ? rinfo[r] : 0xFFFFFFFFu;
+        const int32_t gm = r < r1 ? gmin[r] : INT32_MAX;
+        int deg = (int)(ri & 31u), nr = (int)((ri >> 5) & 3u), L = (int)((ri >> 8) & 255u);
+        bool isg = ri != 0xFFFFFFFFu;
+        if (isg && (deg > cap || nr * L > vcap || deg > 16 || L > SY_CMAX)) { bad = true; isg = false; }
+        if (!isg) deg = nr = L = 0;
+        int pos = 0;  // first lane of the chunk not yet placed
+        for (;;) {
+            const bool mine = isg && lane >= pos;
+            const int ad = mine ? deg : 0, av = mine ? nr * L : 0, ar = mine ? nr : 0;
+            const int sd = sy_wave_incl(ad, lane), sv = sy_wave_incl(av, lane), sr = sy_wave_incl(ar, lane);
+            const bool fail = mine && (ty + sd > cap || tw + sv > vcap || trows + sr > rowcap);
+            const uint64_t fm = __ballot(fail);
+            // the group that overflows an open tile closes it; a group alone
+            // always fits (checked above), so a fresh tile's first group never fails
+            const int j = fm ? __ffsll((long long)fm) - 1 : 64;  // lanes pos .. j-1 join the tile
+            const bool join = mine && lane < j;
+            const uint64_t jm = __ballot(join);
+            if (jm) {
+                if (!open) {
+                    open = true;
+                    row0 = c0 + (__ffsll((long long)jm) - 1);
+                }
+                if (join) {
+                    const int l8 = nr == 2 ? L : trows + (sr - ar);
+                    ginfo[r] = (uint32_t)(tw + (sv - av)) | ((uint32_t)l8 << 16) | (((ri >> 16) & 3u) << 24) |
+                               (((ri >> 18) & 1u) << 26);
+                }
+                const int last = 63 - __clzll((long long)jm);  // the last joining lane
+                ty += __shfl(sd, last, 64);
+                tw += __shfl(sv, last, 64);
+                trows += __shfl(sr, last, 64);
+                nranks = max(nranks, sy_wave_max(join ? deg : 0));
+                emin = min(emin, sy_wave_min(join ? gm : INT32_MAX));
+            }
+            if (!fm) break;
+            // close the tile before lane j's group; lane j's group opens the next
+            if (open) {
+                if (lane == 0) {
+                    tds[t] = make_int4((int)row0, ty, ptr[row0], tw);
+                    trs[t] = make_int4((int)row0, trows, nranks, 0);
+                    tmin[t] = emin;
+                }
+                t++;
+            }
+            open = false;
+            ty = tw = trows = nranks = 0;
+            emin = INT32_MAX;
+            pos = j;
+        }
+    }
+    if (open) {
+        if (lane == 0) {
+            tds[t] = make_int4((int)row0, ty, ptr[row0], tw);
+            trs[t] = make_int4((int)row0, trows, nranks, 0);
+            tmin[t] = emin;
         }
         t++;
-        open = false;
-    };
-    for (int64_t r = r0; r < r1; r++) {
-        const uint32_t ri = rinfo[r];
-        if (ri == 0xFFFFFFFFu) {
-            if (pass) gtile[r] = -1;
-            continue;
-        }
-        const int deg = (int)(ri & 31u), nr = (int)((ri >> 5) & 3u), L = (int)((ri >> 8) & 255u);
-        if (deg > cap || nr * L > vcap || deg > 16 || L > SY_CMAX) { bad = true; continue; }
-        if (open && (ty + deg > cap || tw + nr * L > vcap || trows + nr > rowcap)) close();
-        if (!open) { open = true; row0 = r; ty = tw = trows = nranks = 0; emin = INT32_MAX; }
-        if (pass) {
-            if (deg > 0) emin = min(emin, aelem[aptr[inv_node[r]]]);  // the group's lowest element
-            const int l8 = nr == 2 ? L : trows;
-            gtile[r] = (int32_t)t;
-            ginfo[r] = (uint32_t)tw | ((uint32_t)l8 << 16) | (((ri >> 16) & 3u) << 24) | (((ri >> 18) & 1u) << 26);
-        }
-        ty += deg;
-        tw += nr * L;
-        trows += nr;
-        nranks = max(nranks, deg);
     }
-    close();
-    if (!pass) segtiles[k] = (uint32_t)(t - tfirst);
-    if (bad) atomicOr(&scal[SYS_FLAGS], (uint32_t)SYF_DEG);
+    if (lane == 0) segtiles[k] = (uint32_t)(t - tfirst);
+    if (__ballot(bad) && lane == 0) atomicOr(&scal[SYS_FLAGS], (uint32_t)SYF_DEG);
+}
+
+// Dense tile numbers: range k's tiles follow base[k] (the scanned counts);
+// the sort key is the first element, then the tile number.
+__global__ void k_sy_dense(int64_t nseg, const uint32_t *__restrict__ base, const int4 *__restrict__ tds,
+                           const int4 *__restrict__ trs, const int32_t *__restrict__ tmin, int4 *__restrict__ tdo,
+                           int4 *__restrict__ tro, uint64_t *__restrict__ tkey, uint64_t *__restrict__ tval)
+{
+    const int64_t k = blockIdx.x;
+    const uint32_t b0 = base[k], cnt = base[k + 1] - b0;
+    for (uint32_t l = threadIdx.x; l < cnt; l += blockDim.x) {
+        const int64_t src = k * SY_SEG + l, t = (int64_t)b0 + l;
+        tdo[t] = tds[src];
+        tro[t] = trs[src];
+        tkey[t] = ((uint64_t)(uint32_t)tmin[src] << 32) | (uint64_t)t;
+        tval[t] = (uint64_t)t;
+    }
 }
 
 // sorted position of each tile (tiles sorted by their first element, so that
@@ -319,8 +519,8 @@ __global__ __launch_bounds__(256) void k_sy_tasks(SymIn S, int64_t nt, int cap, 
                                                   const uint32_t *__restrict__ rinfo, const uint32_t *__restrict__ ginfo,
                                                   const int32_t *__restrict__ inv_node,
                                                   const uint32_t *__restrict__ aptr, const int32_t *__restrict__ aelem,
-                                                  const uint8_t *__restrict__ aloc, const int32_t *__restrict__ ptr,
-                                                  const int32_t *__restrict__ ind, int4 *__restrict__ tiles,
+                                                  const uint8_t *__restrict__ aloc, const uint8_t *__restrict__ npos,
+                                                  int pstride, int4 *__restrict__ tiles,
                                                   int4 *__restrict__ tile_rows, int32_t *__restrict__ task_elem,
                                                   uint32_t *__restrict__ task_info, uint32_t *__restrict__ task_pos,
                                                   uint32_t *__restrict__ scal)
@@ -375,10 +575,9 @@ __global__ __launch_bounds__(256) void k_sy_tasks(SymIn S, int64_t nt, int cap, 
     const int ncol = sy_ncol(S, field);
     const int32_t node = inv_node[r];
     const uint32_t a0 = aptr[node];
-    const int32_t *row = ind + ptr[r];
-    const int L = ptr[r + 1] - ptr[r];
+    // positions of each task's columns in the row (from the node's sort),
+    // flagged where the task is the first (lowest rank) to add into the entry
     uint32_t seen[4] = {0u, 0u, 0u, 0u};
-    bool bad = false;
     for (int s = 0; s < deg; s++) {
         const int32_t e = aelem[a0 + s];
         const int a = aloc[a0 + s];
@@ -387,33 +586,15 @@ __global__ __launch_bounds__(256) void k_sy_tasks(SymIn S, int64_t nt, int cap, 
         task_info[slot] = (uint32_t)a | ((uint32_t)s << 3) | ((uint32_t)rowcode << 7) | ((uint32_t)field << 9) |
                           ((uint32_t)vbase << 10) | ((uint32_t)l8 << 24);
         uint32_t pw[4] = {0u, 0u, 0u, 0u};
+        const uint8_t *pp = npos + (int64_t)(a0 + s) * pstride;
         for (int j = 0; j < ncol; j++) {
-            const int32_t col = sy_col(S, e, j);
-            int lo = 0, hi = L;  // first position with row[pos] >= col
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (row[mid] < col) lo = mid + 1;
-                else hi = mid;
-            }
-            if (lo >= L || row[lo] != col) { bad = true; continue; }
-            uint32_t pv = (uint32_t)lo;
+            const uint32_t lo = pp[j] & 127u;
+            uint32_t pv = lo;
             if (!((seen[lo >> 5] >> (lo & 31)) & 1u)) { seen[lo >> 5] |= 1u << (lo & 31); pv |= 0x80u; }
             pw[j >> 2] |= pv << (8 * (j & 3));
         }
         for (int q = 0; q < nw; q++) task_pos[slot * nw + q] = pw[q];
     }
-    if (bad) atomicOr(&scal[SYS_FLAGS], (uint32_t)SYF_PAIR);
-}
-
-__global__ void k_sy_fill_ind(int64_t n, const int32_t *__restrict__ inv_node, const int32_t *__restrict__ ptr,
-                              const int32_t *__restrict__ lists, int32_t *__restrict__ ind)
-{
-    const int64_t r = blockIdx.x;
-    const int32_t node = inv_node[r];
-    if (node < 0) return;
-    const int L = ptr[r + 1] - ptr[r];
-    const int32_t *src = lists + (int64_t)node * SY_CMAX;
-    for (int i = threadIdx.x; i < L; i += 64) ind[ptr[r] + i] = src[i];
 }
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -450,31 +631,38 @@ int device_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int block)
     const int vcap = std::min(std::max(env_int("ELFEL_HIP_VCAP", stokes ? 20 : REUSE_VCAP_PER_THREAD), 8), 24) * cap;
     const int rowcap = std::min(cap, REUSE_MAX_TILE_ROWS);
     const int ncolp = (S.nen0 * S.nc + (stokes ? 3 : 0) + 3) & ~3;
+    const int pstride = S.nen0 * S.nc + (stokes ? 3 : 0);  // npos bytes per adjacency entry
     const int64_t nseg = std::max<int64_t>(1, (n + SY_SEG - 1) / SY_SEG);
-    // scratch, one allocation: adjacency, inverse map, node lists, row data,
-    // packing ranges, dof-ordered tiles (at most one per row)
+    // scratch, one allocation: adjacency, inverse map, per-candidate
+    // positions, row data, packing ranges, dof-ordered tiles (at most one per
+    // row), the tile sort
     size_t off = 0;
     auto take = [&](size_t bytes) { const size_t o = off; off += align256(bytes); return o; };
     const size_t o_scal = take(4 * SYS_N), o_aptr = take(4 * (N + 1)), o_fill = take(4 * N), o_aelem = take(4 * nent),
                  o_aloc = take(nent), o_inv = take(4 * n), o_fc = take(n), o_nodeL = take(4 * N),
-                 o_lists = take(4 * (size_t)N * SY_CMAX), o_rinfo = take(4 * n), o_ginfo = take(4 * n),
-                 o_gtile = take(4 * n), o_seg = take(4 * (nseg + 1)), o_tdo = take(16 * (size_t)std::max<int64_t>(n, 1)),
-                 o_tro = take(16 * (size_t)std::max<int64_t>(n, 1)), o_k0 = take(8 * (size_t)std::max<int64_t>(n, 1)),
-                 o_k1 = take(8 * (size_t)std::max<int64_t>(n, 1)), o_v0 = take(8 * (size_t)std::max<int64_t>(n, 1)),
-                 o_v1 = take(8 * (size_t)std::max<int64_t>(n, 1)), o_tpos = take(4 * (size_t)std::max<int64_t>(n, 1));
+                 o_npos = take((size_t)nent * pstride), o_edof = take(4 * (size_t)S.E * pstride), o_gmin = take(4 * n),
+                 o_rinfo = take(4 * n), o_ginfo = take(4 * n), o_seg = take(4 * (nseg + 1)),
+                 o_tds = take(16 * (size_t)nseg * SY_SEG), o_trs = take(16 * (size_t)nseg * SY_SEG),
+                 o_tmin = take(4 * (size_t)nseg * SY_SEG),
+                 o_scan = take(4 * (size_t)scan_scratch_entries(std::max(N, n) + 1)),
+                 o_hist = take(4 * (size_t)radix_hist_entries((int64_t)nseg * SY_SEG));
     int rc;
     if ((rc = ensure(c, P.sym, off))) return rc;
     unsigned char *base = (unsigned char *)P.sym.p;
     uint32_t *scal = (uint32_t *)(base + o_scal), *aptr = (uint32_t *)(base + o_aptr), *fill = (uint32_t *)(base + o_fill);
-    int32_t *aelem = (int32_t *)(base + o_aelem), *inv_node = (int32_t *)(base + o_inv), *lists = (int32_t *)(base + o_lists);
+    int32_t *aelem = (int32_t *)(base + o_aelem), *inv_node = (int32_t *)(base + o_inv), *gmin = (int32_t *)(base + o_gmin);
+    uint8_t *npos = base + o_npos;
     uint8_t *aloc = base + o_aloc, *inv_fc = base + o_fc;
     uint32_t *nodeL = (uint32_t *)(base + o_nodeL), *rinfo = (uint32_t *)(base + o_rinfo), *ginfo = (uint32_t *)(base + o_ginfo);
-    int32_t *gtile = (int32_t *)(base + o_gtile);
+    int32_t *edof = (int32_t *)(base + o_edof), *tmin = (int32_t *)(base + o_tmin);
     uint32_t *segtiles = (uint32_t *)(base + o_seg);
-    int4 *tdo = (int4 *)(base + o_tdo), *tro = (int4 *)(base + o_tro);
-    uint64_t *k0 = (uint64_t *)(base + o_k0), *k1 = (uint64_t *)(base + o_k1);
-    uint64_t *v0 = (uint64_t *)(base + o_v0), *v1 = (uint64_t *)(base + o_v1);
-    uint32_t *tpos = (uint32_t *)(base + o_tpos);
+    int4 *tds = (int4 *)(base + o_tds), *trs = (int4 *)(base + o_trs);
+    // the scans' and the tile sort's scratch come from the same allocation
+    SortScratch ss;
+    ss.scan = (uint32_t *)(base + o_scan);
+    ss.scan_cap = scan_scratch_entries(std::max(N, n) + 1);
+    ss.hist = (uint32_t *)(base + o_hist);
+    ss.hist_cap = radix_hist_entries((int64_t)nseg * SY_SEG);
     hipStream_t st = c->stream;
     const int g = 256;
     const int grid_ent = std::max(1, std::min(ceil_div(nent, g), 8192));
@@ -484,7 +672,7 @@ int device_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int block)
     HIPCHK(c, hipMemsetAsync(inv_node, 0xFF, 4 * n, st));
     // 1. adjacency
     hipLaunchKernelGGL(k_sy_adj_count, dim3(grid_ent), dim3(g), 0, st, S, aptr);
-    if ((rc = exclusive_scan_u32(c, aptr, N + 1, nullptr))) return rc;
+    if ((rc = exclusive_scan_u32(c, aptr, N + 1, nullptr, &ss))) return rc;
     hipLaunchKernelGGL(k_sy_adj_fill, dim3(grid_ent), dim3(g), 0, st, S, (const uint32_t *)aptr, fill, aelem, aloc);
     if (N > 0)
         hipLaunchKernelGGL(k_sy_adj_sort, dim3(ceil_div(N, g)), dim3(g), 0, st, N, (const uint32_t *)aptr, aelem, aloc,
@@ -493,17 +681,19 @@ int device_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int block)
     const int64_t ndofref = S.Nn0 * S.nc + S.Np;
     hipLaunchKernelGGL(k_sy_inv, dim3(std::max(1, std::min(ceil_div(ndofref, g), 8192))), dim3(g), 0, st, S, inv_node,
                        inv_fc, scal);
-    if (S.E > 0) hipLaunchKernelGGL(k_sy_elem_check, dim3(ceil_div(S.E, g)), dim3(g), 0, st, S, scal);
-    // 3. node column lists, row lengths, ptr
+    if (S.E > 0) hipLaunchKernelGGL(k_sy_elem_check, dim3(ceil_div(S.E, g)), dim3(g), 0, st, S, pstride, edof, scal);
+    // 3. node column lists: lengths, row lengths, ptr; then the lists into ind
     if (N > 0)
-        hipLaunchKernelGGL(k_sy_node_cols, dim3((unsigned)N), dim3(64), 0, st, S, N, (const uint32_t *)aptr,
-                           (const int32_t *)aelem, lists, nodeL, scal);
+        hipLaunchKernelGGL(k_sy_node_cols, dim3(ceil_div(N, 4)), dim3(256), 0, st, S, N, (const uint32_t *)aptr,
+                           (const int32_t *)aelem, (const int32_t *)edof, 0, nodeL, (const int32_t *)nullptr,
+                           (int32_t *)nullptr,
+                           (uint8_t *)nullptr, pstride, scal);
     if ((rc = ensure(c, P.ptr, 4 * (n + 1)))) return rc;
     int32_t *ptr = (int32_t *)P.ptr.p;
     hipLaunchKernelGGL(k_sy_rows, dim3(ceil_div(n + 1, g)), dim3(g), 0, st, S, (const int32_t *)inv_node,
-                       (const uint8_t *)inv_fc, (const uint32_t *)aptr, (const uint32_t *)nodeL, (uint32_t *)ptr, rinfo,
-                       scal);
-    if ((rc = exclusive_scan_u32(c, (uint32_t *)ptr, n + 1, nullptr))) return rc;
+                       (const uint8_t *)inv_fc, (const uint32_t *)aptr, (const int32_t *)aelem, (const uint32_t *)nodeL,
+                       (uint32_t *)ptr, rinfo, gmin);
+    if ((rc = exclusive_scan_u32(c, (uint32_t *)ptr, n + 1, nullptr, &ss))) return rc;
     uint32_t hv[2] = {0, 0};
     HIPCHK(c, hipMemcpyAsync(&hv[0], ptr + n, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipMemcpyAsync(&hv[1], scal + SYS_FLAGS, 4, hipMemcpyDeviceToHost, st));
@@ -521,18 +711,16 @@ int device_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int block)
         return 0;
     }
     if ((rc = ensure(c, P.ind, 4 * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
-    if (n > 0)
-        hipLaunchKernelGGL(k_sy_fill_ind, dim3((unsigned)n), dim3(64), 0, st, n, (const int32_t *)inv_node,
-                           (const int32_t *)ptr, (const int32_t *)lists, (int32_t *)P.ind.p);
-    // 4. packing
-    hipLaunchKernelGGL(k_sy_pack, dim3(ceil_div(nseg, 64)), dim3(64), 0, st, n, nseg, (const uint32_t *)rinfo,
-                       (const int32_t *)ptr, cap, vcap, rowcap, 0, segtiles, gtile, ginfo, tdo, tro,
-                       (const int32_t *)inv_node, (const uint32_t *)aptr, (const int32_t *)aelem, k0, v0, scal);
+    if (N > 0)
+        hipLaunchKernelGGL(k_sy_node_cols, dim3(ceil_div(N, 4)), dim3(256), 0, st, S, N, (const uint32_t *)aptr,
+                           (const int32_t *)aelem, (const int32_t *)edof, 1, nodeL, (const int32_t *)ptr,
+                           (int32_t *)P.ind.p, npos, pstride, scal);
+    // 4. packing (sparse tile numbers), counts scanned, dense tiles and keys
+    hipLaunchKernelGGL(k_sy_pack, dim3((unsigned)nseg), dim3(64), 0, st, n, nseg, (const uint32_t *)rinfo,
+                       (const int32_t *)gmin, (const int32_t *)ptr, cap, vcap, rowcap, segtiles, ginfo, tds, trs, tmin,
+                       scal);
     HIPCHK(c, hipMemsetAsync(segtiles + nseg, 0, 4, st));
-    if ((rc = exclusive_scan_u32(c, segtiles, nseg + 1, nullptr))) return rc;
-    hipLaunchKernelGGL(k_sy_pack, dim3(ceil_div(nseg, 64)), dim3(64), 0, st, n, nseg, (const uint32_t *)rinfo,
-                       (const int32_t *)ptr, cap, vcap, rowcap, 1, segtiles, gtile, ginfo, tdo, tro,
-                       (const int32_t *)inv_node, (const uint32_t *)aptr, (const int32_t *)aelem, k0, v0, scal);
+    if ((rc = exclusive_scan_u32(c, segtiles, nseg + 1, nullptr, &ss))) return rc;
     uint32_t hn[2] = {0, 0};
     HIPCHK(c, hipMemcpyAsync(&hn[0], segtiles + nseg, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipMemcpyAsync(&hn[1], scal + SYS_FLAGS, 4, hipMemcpyDeviceToHost, st));
@@ -544,14 +732,28 @@ int device_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int block)
         return 0;
     }
     const int64_t nt = hn[0];
-    // 6. launch order: tiles sorted by first element (key: element << 32 | tile)
+    // the dense tiles, their sort keys and order: nt-sized, a second allocation
+    size_t off2 = 0;
+    auto take2 = [&](size_t bytes) { const size_t o = off2; off2 += align256(bytes); return o; };
+    const int64_t ntc = std::max<int64_t>(nt, 1);
+    const size_t o_tdo = take2(16 * ntc), o_tro = take2(16 * ntc), o_k0 = take2(8 * ntc), o_k1 = take2(8 * ntc),
+                 o_v0 = take2(8 * ntc), o_v1 = take2(8 * ntc), o_tpos = take2(4 * ntc);
+    if ((rc = ensure(c, P.sym2, off2))) return rc;
+    unsigned char *base2 = (unsigned char *)P.sym2.p;
+    int4 *tdo = (int4 *)(base2 + o_tdo), *tro = (int4 *)(base2 + o_tro);
+    uint64_t *k0 = (uint64_t *)(base2 + o_k0), *k1 = (uint64_t *)(base2 + o_k1);
+    uint64_t *v0 = (uint64_t *)(base2 + o_v0), *v1 = (uint64_t *)(base2 + o_v1);
+    uint32_t *tpos = (uint32_t *)(base2 + o_tpos);
+    hipLaunchKernelGGL(k_sy_dense, dim3((unsigned)nseg), dim3(64), 0, st, nseg, (const uint32_t *)segtiles,
+                       (const int4 *)tds, (const int4 *)trs, (const int32_t *)tmin, tdo, tro, k0, v0);
+    // 5. launch order: tiles sorted by first element (key: element << 32 | tile)
     if (nt > 1) {
-        if ((rc = radix_sort_pairs(c, k0, v0, k1, v1, nt, 32 + bits_for(std::max<int64_t>(S.E, 1))))) return rc;
+        if ((rc = radix_sort_pairs(c, k0, v0, k1, v1, nt, 32 + bits_for(std::max<int64_t>(S.E, 1)), &ss))) return rc;
         hipLaunchKernelGGL(k_sy_tpos, dim3(ceil_div(nt, g)), dim3(g), 0, st, nt, (const uint64_t *)v0, tpos);
     } else {
         HIPCHK(c, hipMemsetAsync(tpos, 0, 4, st));
     }
-    // 5. task tables in launch order
+    // 6. task tables in launch order
     const size_t slots = (size_t)nt * cap;
     if ((rc = ensure(c, P.tile, sizeof(int4) * std::max<int64_t>(nt, 1))) ||
         (rc = ensure(c, P.tile_rows, sizeof(int4) * std::max<int64_t>(nt, 1))) ||
@@ -563,15 +765,14 @@ int device_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int block)
         hipLaunchKernelGGL(k_sy_tasks, dim3((unsigned)nt), dim3(256), 0, st, S, nt, cap, ncolp, (const uint32_t *)tpos,
                            (const int4 *)tdo,
                            (const int4 *)tro, (const uint32_t *)rinfo, (const uint32_t *)ginfo, (const int32_t *)inv_node,
-                           (const uint32_t *)aptr, (const int32_t *)aelem, (const uint8_t *)aloc, (const int32_t *)ptr,
-                           (const int32_t *)P.ind.p, (int4 *)P.tile.p, (int4 *)P.tile_rows.p, (int32_t *)P.task_elem.p,
+                           (const uint32_t *)aptr, (const int32_t *)aelem, (const uint8_t *)aloc,
+                           (const uint8_t *)npos, pstride, (int4 *)P.tile.p, (int4 *)P.tile_rows.p, (int32_t *)P.task_elem.p,
                            (uint32_t *)P.task_info.p, (uint32_t *)P.task_pos.p, scal);
     HIPCHK(c, hipGetLastError());
     uint32_t hs[SYS_N] = {0, 0, 0, 0};
     HIPCHK(c, hipMemcpyAsync(hs, scal, 4 * SYS_N, hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
-    if (hs[SYS_FLAGS])
-        return set_err(c, ELFEL_EINVAL, "internal: an element column is missing from the device pattern");
+    if (hs[SYS_FLAGS]) return set_err(c, ELFEL_EINVAL, "internal: device symbolic phase flagged late");
     release(P.rrec); release(P.tconn); release(P.txy); release(P.truns); release(P.runs); release(P.tile_elem);
     P.rw = 0;
     P.tstride = cap;
