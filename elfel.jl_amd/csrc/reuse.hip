@@ -832,6 +832,24 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
         P.symbolic_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return 0;
     }
+    // scalar heat problems: row-owner tiles built on the device when the mesh
+    // fits the row records (else the host builder below makes task tiles)
+    if (pb->kind == ELFEL_HEAT && F0.ncomp == 1 && env_int("ELFEL_HIP_HEAT_ROWS", 1) != 0 &&
+        env_int("ELFEL_HIP_DEVICE_SYMBOLIC", 1) != 0 && pick_heat_rows32(pb->fe, tab.nq, pb->layout == ELFEL_CSC)) {
+        int rc = device_symbolic_heat(c, pb, env_int("ELFEL_HIP_PATCH_TILES", 1) != 0);
+        if (rc < 0) return rc;
+        if (rc == 0) {
+            lap("device symbolic (heat)");
+            P.block = block * reuse_tpt(pb->kind);
+            P.kind = pb->kind; P.fe = pb->fe; P.layout = pb->layout;
+            P.key_mesh0 = F0.mesh_version; P.key_dof0 = F0.dof_version;
+            P.key_mesh1 = P.key_dof1 = 0;
+            P.nrow = P.ncol = n;
+            P.valid = true;
+            P.symbolic_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            return 0;
+        }
+    }
     std::vector<int32_t> hptr, hind;
     int rc = pattern_to_host(c, pb, tab, cf, hptr, hind);
     if (rc) return rc;

@@ -968,6 +968,9 @@ int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P);
 // symbolic.hip: the pattern and task tables of an elasticity or Stokes problem
 // built on the device (c->pat; fallback set when the mesh does not fit)
 int device_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int block);
+// the same for the row-owner heat tiles of scalar T3 / Q4 problems; 1: the
+// mesh does not fit the row records (the host builder makes task tiles)
+int device_symbolic_heat(elfel_hip_ctx *c, const elfel_hip_problem *pb, bool patch_allowed);
 // Sorts the tiles by their first element and lays them out so that workgroup b
 // (dispatched to XCD b mod nxcd) walks a contiguous chunk of the sorted list
 // (run = 0), or runs of `run` tiles dealt to the XCDs in turn.

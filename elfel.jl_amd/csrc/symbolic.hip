@@ -45,7 +45,7 @@ constexpr int SY_CMAX = 128;  // longest row (entries): the tasks' 7-bit positio
 constexpr int SY_SEG = 2048;  // rows per packing range
 
 enum { SYF_DEG = 1, SYF_LONG = 2, SYF_SHARED = 4, SYF_REPEAT = 8 };
-enum { SYS_FLAGS = 0, SYS_MAXV = 1, SYS_MAXRANK = 2, SYS_N = 4 };
+enum { SYS_FLAGS = 0, SYS_MAXV = 1, SYS_MAXRANK = 2, SYS_MAXU = 3, SYS_MAXDEG = 4, SYS_MAXL = 5, SYS_N = 8 };
 
 struct SymIn {
     const int32_t *conn0, *dof0;  // velocity / displacement field: E x nen0, Nn0 x nc (0-based)
@@ -599,6 +599,430 @@ __global__ __launch_bounds__(256) void k_sy_tasks(SymIn S, int64_t nt, int cap, 
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+
+// ---- heat row-owner tiles ------------------------------------------------------
+// The tiles of heat_rows_tile (reuse_kernel.h) for scalar T3 / Q4 problems:
+// rows packed greedily in a locality order of their nodes (bands of about
+// eight node spacings, in x then y; the host builder's band_order), at most
+// 256 rows and 256 distinct elements a tile; inside a tile the rows' values
+// lie in dof order, written out in runs of consecutive rows; one record per
+// row; the tile's elements and the coordinates of their nodes in tile-local
+// form.  Distinct elements of a candidate tile come from prev: for each
+// (node, element) pair, the latest earlier position in the row order of
+// another node of the element -- the element is new to a tile starting at s
+// exactly when prev < s.
+
+// order-preserving integer image of a double (for min / max atomics)
+__device__ __forceinline__ unsigned long long sy_ord(double d)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(d);
+    return (u >> 63) ? ~u : (u | (1ull << 63));
+}
+__device__ __forceinline__ double sy_unord(unsigned long long u)
+{
+    u = (u >> 63) ? (u & ~(1ull << 63)) : ~u;
+    return __longlong_as_double((long long)u);
+}
+
+__global__ void k_sy_bbox(int64_t Nn, const double2 *__restrict__ xy, unsigned long long *__restrict__ bb)
+{
+    unsigned long long mnx = ~0ull, mxx = 0, mny = ~0ull, mxy = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < Nn; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 p = xy[i];
+        const unsigned long long ox = sy_ord(p.x), oy = sy_ord(p.y);
+        mnx = min(mnx, ox); mxx = max(mxx, ox); mny = min(mny, oy); mxy = max(mxy, oy);
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        mnx = min(mnx, (unsigned long long)__shfl_xor((long long)mnx, d, 64));
+        mxx = max(mxx, (unsigned long long)__shfl_xor((long long)mxx, d, 64));
+        mny = min(mny, (unsigned long long)__shfl_xor((long long)mny, d, 64));
+        mxy = max(mxy, (unsigned long long)__shfl_xor((long long)mxy, d, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&bb[0], mnx); atomicMax(&bb[1], mxx); atomicMin(&bb[2], mny); atomicMax(&bb[3], mxy);
+    }
+}
+
+// key = (band, x, y): x and y quantised to 16 bits in the bounding box, bands
+// band_rows node spacings high (spacing from the box and the node count)
+__global__ void k_sy_band_keys(int64_t Nn, const double2 *__restrict__ xy, const unsigned long long *__restrict__ bb,
+                               int band_rows, uint64_t *__restrict__ key, uint64_t *__restrict__ val)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nn) return;
+    const double x0 = sy_unord(bb[0]), x1 = sy_unord(bb[1]), y0 = sy_unord(bb[2]), y1 = sy_unord(bb[3]);
+    const double sx = x1 > x0 ? 65535.0 / (x1 - x0) : 0.0, sy = y1 > y0 ? 65535.0 / (y1 - y0) : 0.0;
+    const double spacing = sqrt(fmax((x1 - x0) * (y1 - y0), 1e-300) / fmax((double)Nn, 1.0));
+    const double bh = spacing * (double)band_rows;
+    const double2 p = xy[i];
+    const uint32_t qx = (uint32_t)((p.x - x0) * sx), qy = (uint32_t)((p.y - y0) * sy);
+    const uint64_t band = (uint64_t)((p.y - y0 + 0.5 * spacing) / bh);
+    key[i] = (band << 32) | ((uint64_t)qx << 16) | qy;
+    val[i] = (uint64_t)i;
+}
+
+// row order: the rows of the nodes in the given node order, then the rows
+// without a node in dof order (slot Nn + their rank, from the scan of noder);
+// without a node order, dof order
+__global__ void k_sy_seq(int64_t n, int64_t Nn, const uint64_t *__restrict__ node_order, const int32_t *__restrict__ dof,
+                         const int32_t *__restrict__ inv_node, const uint32_t *__restrict__ noder_rank,
+                         int32_t *__restrict__ seq, int32_t *__restrict__ P)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (!node_order) {
+        if (i < n) { seq[i] = (int32_t)i; P[i] = (int32_t)i; }
+        return;
+    }
+    if (i < Nn) {
+        const int32_t r = dof[node_order ? (int64_t)node_order[i] : i];
+        seq[i] = r;
+        P[r] = (int32_t)i;
+    }
+    if (i < n && inv_node[i] < 0) {
+        const int64_t q = Nn + noder_rank[i];
+        seq[q] = (int32_t)i;
+        P[i] = (int32_t)q;
+    }
+}
+__global__ void k_sy_noder(int64_t n, const int32_t *__restrict__ inv_node, uint32_t *__restrict__ f)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) f[i] = inv_node[i] < 0 ? 1u : 0u;
+    if (i == n) f[i] = 0u;
+}
+
+// prev of each (node, element) pair: the latest position before the node's
+// own of another node of the element (-1: none)
+__global__ void k_sy_prev(int64_t Nn, int nen, const int32_t *__restrict__ conn, const int32_t *__restrict__ dof,
+                          const uint32_t *__restrict__ aptr, const int32_t *__restrict__ aelem,
+                          const int32_t *__restrict__ P, int32_t *__restrict__ prev)
+{
+    const int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nd >= Nn) return;
+    const int32_t p = P[dof[nd]];
+    for (uint32_t k = aptr[nd]; k < aptr[nd + 1]; k++) {
+        const int64_t e = aelem[k];
+        int32_t best = -1;
+        for (int j = 0; j < nen; j++) {
+            const int32_t m = conn[e * nen + j];
+            if (m == nd) continue;
+            const int32_t q = P[dof[m]];
+            if (q < p) best = max(best, q);
+        }
+        prev[k] = best;
+    }
+}
+
+// the most elements at a node and the longest node list (row records hold
+// kmax elements and rows of 16 entries)
+__global__ void k_sy_hcheck(int64_t N, const uint32_t *__restrict__ aptr, const uint32_t *__restrict__ nodeL,
+                            uint32_t *__restrict__ scal)
+{
+    const int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t dg = 0, l = 0;
+    if (nd < N) { dg = aptr[nd + 1] - aptr[nd]; l = nodeL[nd]; }
+    for (int d = 32; d > 0; d >>= 1) {
+        dg = max(dg, (uint32_t)__shfl_xor((int)dg, d, 64));
+        l = max(l, (uint32_t)__shfl_xor((int)l, d, 64));
+    }
+    if ((threadIdx.x & 63) == 0) { atomicMax(&scal[SYS_MAXDEG], dg); atomicMax(&scal[SYS_MAXL], l); }
+}
+
+// Greedy packing of the row order, one wavefront per range of `seg`
+// positions (as k_sy_pack), with the distinct-element limit: a row's new
+// elements for a tile starting at s are those with prev < s.  Tile info at
+// sparse index k * seg + l: {start position, rows, tasks, values} and
+// {0, 0, most elements of a row, distinct elements}.
+template <int KMAX>
+__global__ __launch_bounds__(64) void k_sy_hpack(int64_t n, int64_t nseg, int seg, const int32_t *__restrict__ seq,
+                                                 const int32_t *__restrict__ inv_node, const uint32_t *__restrict__ aptr,
+                                                 const int32_t *__restrict__ prev, const int32_t *__restrict__ ptr,
+                                                 int vcap, uint32_t *__restrict__ segtiles, int4 *__restrict__ tds,
+                                                 int4 *__restrict__ trs)
+{
+    const int64_t k = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t r0 = k * seg, r1 = min((int64_t)(k + 1) * seg, n);
+    int64_t t = k * seg;
+    const int64_t tfirst = t;
+    bool open = false;
+    int ty = 0, tw = 0, trows = 0, tu = 0, nranks = 0;
+    int64_t s0 = 0;  // the open tile's first position
+    for (int64_t c0 = r0; c0 < r1; c0 += 64) {
+        const int64_t q = c0 + lane;
+        const bool valid = q < r1;
+        int deg = 0, L = 0;
+        int32_t pv[KMAX];
+#pragma unroll
+        for (int j = 0; j < KMAX; j++) pv[j] = INT32_MAX;
+        if (valid) {
+            const int32_t r = seq[q];
+            const int32_t nd = inv_node[r];
+            L = ptr[r + 1] - ptr[r];
+            if (nd >= 0) {
+                const uint32_t a0 = aptr[nd];
+                deg = (int)min(aptr[nd + 1] - a0, (uint32_t)KMAX);
+#pragma unroll
+                for (int j = 0; j < KMAX; j++)
+                    if (j < deg) pv[j] = prev[a0 + j];
+            }
+        }
+        int pos = 0;
+        for (;;) {
+            const bool mine = valid && lane >= pos;
+            const int64_t s = open ? s0 : c0 + pos;  // the start of the tile the lanes from pos join
+            int nu = 0;
+#pragma unroll
+            for (int j = 0; j < KMAX; j++) nu += (j < deg && pv[j] < s) ? 1 : 0;
+            const int ad = mine ? deg : 0, av = mine ? L : 0, ar = mine ? 1 : 0, au = mine ? nu : 0;
+            const int sd = sy_wave_incl(ad, lane), sv = sy_wave_incl(av, lane), sr = sy_wave_incl(ar, lane),
+                      su = sy_wave_incl(au, lane);
+            const bool fail = mine && (ty + sd > 256 * KMAX || tw + sv > vcap || trows + sr > 256 || tu + su > 256);
+            const uint64_t fm = __ballot(fail);
+            const int j = fm ? __ffsll((long long)fm) - 1 : 64;
+            const bool join = mine && lane < j;
+            const uint64_t jm = __ballot(join);
+            if (jm) {
+                if (!open) { open = true; s0 = c0 + pos; }
+                const int last = 63 - __clzll((long long)jm);
+                ty += __shfl(sd, last, 64);
+                tw += __shfl(sv, last, 64);
+                trows += __shfl(sr, last, 64);
+                tu += __shfl(su, last, 64);
+                nranks = max(nranks, sy_wave_max(join ? deg : 0));
+            }
+            if (!fm) break;
+            if (open) {
+                if (lane == 0) {
+                    tds[t] = make_int4((int)s0, trows, ty, tw);
+                    trs[t] = make_int4(0, 0, nranks, tu);
+                }
+                t++;
+            }
+            open = false;
+            ty = tw = trows = tu = nranks = 0;
+            pos = j;
+        }
+    }
+    if (open) {
+        if (lane == 0) {
+            tds[t] = make_int4((int)s0, trows, ty, tw);
+            trs[t] = make_int4(0, 0, nranks, tu);
+        }
+        t++;
+    }
+    if (lane == 0) segtiles[k] = (uint32_t)(t - tfirst);
+}
+
+__global__ void k_sy_hdense(int64_t nseg, int seg, const uint32_t *__restrict__ base, const int4 *__restrict__ tds,
+                            const int4 *__restrict__ trs, int4 *__restrict__ tdo, int4 *__restrict__ tro)
+{
+    const int64_t k = blockIdx.x;
+    const uint32_t b0 = base[k], cnt = base[k + 1] - b0;
+    for (uint32_t l = threadIdx.x; l < cnt; l += blockDim.x) {
+        tdo[b0 + l] = tds[k * seg + l];
+        tro[b0 + l] = trs[k * seg + l];
+    }
+}
+
+// bitonic sort of `size` (a power of two, <= 4 x 256) keys in LDS by a
+// workgroup of 256 threads
+template <typename T>
+__device__ __forceinline__ void sy_block_sort(T *a, int size)
+{
+    for (int k = 2; k <= size; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int p = threadIdx.x; p < (size >> 1); p += 256) {
+                const int i = (p / j) * 2 * j + (p % j), q = i + j;
+                const T x = a[i], y = a[q];
+                const bool up = (i & k) == 0;
+                if ((x > y) == up) { a[i] = y; a[q] = x; }
+            }
+            __syncthreads();
+        }
+}
+__device__ __forceinline__ int sy_block_excl(int v, int *wsum, int *total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int incl = sy_wave_incl(v, lane);
+    __syncthreads();
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int pre = 0, tot = 0;
+    for (int q = 0; q < 4; q++) { pre += q < w ? wsum[q] : 0; tot += wsum[q]; }
+    if (total) *total = tot;
+    return pre + incl - v;
+}
+template <typename T>
+__device__ __forceinline__ int sy_lower_bound(const T *a, int n, T v)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (a[m] < v) lo = m + 1;
+        else hi = m;
+    }
+    return lo;
+}
+
+// One tile per workgroup of 256.  Pass 0 counts its runs of consecutive rows
+// and the distinct nodes of its elements; pass 1 writes, at launch slot b,
+// the tile descriptors, runs, element list, row records, tile-local
+// coordinates and element nodes.
+template <int NEN>
+__global__ __launch_bounds__(256) void k_sy_htile(int64_t nt, int pass, int patch, const int32_t *__restrict__ seq,
+                                                  const int32_t *__restrict__ inv_node, const uint32_t *__restrict__ aptr,
+                                                  const int32_t *__restrict__ aelem, const uint8_t *__restrict__ aloc,
+                                                  const int32_t *__restrict__ prev, const uint8_t *__restrict__ npos,
+                                                  const int32_t *__restrict__ ptr, const int32_t *__restrict__ conn,
+                                                  const double2 *__restrict__ xy, const int4 *__restrict__ tdo,
+                                                  const int4 *__restrict__ tro, uint32_t *__restrict__ nruns,
+                                                  uint32_t *__restrict__ nnodes, int4 *__restrict__ tiles,
+                                                  int4 *__restrict__ tile_rows, int2 *__restrict__ truns,
+                                                  int4 *__restrict__ runs, int32_t *__restrict__ telem,
+                                                  uint32_t *__restrict__ rrec, double2 *__restrict__ txy,
+                                                  ushort4 *__restrict__ tconn, uint32_t *__restrict__ scal)
+{
+    constexpr int KMAX = hrow_kmax<NEN>();
+    constexpr int RW = hrow_words<NEN, KMAX>();
+    constexpr int FB = hrow_field_bits<NEN>();
+    __shared__ uint64_t rk[256];        // (row << 8 | tid), sorted: the tile's rows in dof order
+    __shared__ int vb[256];             // LDS value offset of sorted row q
+    __shared__ int32_t el[256];         // the tile's elements, sorted
+    __shared__ int32_t nds[1024];       // their nodes, sorted, then unique
+    __shared__ int32_t un[1024];
+    __shared__ int runend[256];
+    __shared__ int wsum[4];
+    const int64_t t = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int4 d = tdo[t], rr = tro[t];
+    const int s = d.x, nrows = d.y, nU = rr.w;
+    // this thread's row (row order position s + tid)
+    int32_t row = -1, nd = -1;
+    int deg = 0;
+    uint32_t a0 = 0;
+    if (tid < nrows) {
+        row = seq[s + tid];
+        nd = inv_node[row];
+        if (nd >= 0) { a0 = aptr[nd]; deg = (int)(aptr[nd + 1] - a0); }
+    }
+    // rows in dof order
+    rk[tid] = tid < nrows ? (((uint64_t)(uint32_t)row << 8) | (uint64_t)tid) : ~0ull;
+    __syncthreads();
+    sy_block_sort<uint64_t>(rk, 256);
+    // sorted position q = tid: its row's length -> value offsets
+    const int qtid = tid < nrows ? (int)(rk[tid] & 255u) : 0;  // the thread (row) at sorted position tid
+    const int32_t qrow = tid < nrows ? (int32_t)(rk[tid] >> 8) : -1;
+    const int qL = tid < nrows ? ptr[qrow + 1] - ptr[qrow] : 0;
+    int nvals = 0;
+    const int qvb = sy_block_excl(qL, wsum, &nvals);
+    vb[tid] = qvb;
+    // runs of consecutive dofs (patch tiles; otherwise the rows are one run)
+    const bool rstart = tid < nrows && (tid == 0 || qrow != (int32_t)(rk[tid - 1] >> 8) + 1);
+    int nr = 0;
+    const int ridx = sy_block_excl(rstart ? 1 : 0, wsum, &nr);
+    // the tile's elements: each row's elements new to the tile (prev < s)
+    int cnew = 0;
+    for (int k = 0; k < deg; k++) cnew += prev[a0 + k] < s ? 1 : 0;
+    int nu2 = 0;
+    const int eoff = sy_block_excl(cnew, wsum, &nu2);
+    el[tid] = INT32_MAX;
+    __syncthreads();
+    for (int k = 0, o = eoff; k < deg; k++)
+        if (prev[a0 + k] < s) el[o++] = aelem[a0 + k];
+    __syncthreads();
+    sy_block_sort<int32_t>(el, 256);
+    // their nodes, sorted and unique
+    for (int i = tid; i < 1024; i += 256) {
+        const int u = i / NEN, j = i % NEN;
+        nds[i] = (u < nU && i < NEN * 256) ? conn[(int64_t)el[u] * NEN + j] : INT32_MAX;
+    }
+    __syncthreads();
+    sy_block_sort<int32_t>(nds, 1024);
+    int fl[4], cu = 0;
+    for (int m = 0; m < 4; m++) {
+        const int i = tid * 4 + m;
+        fl[m] = nds[i] != INT32_MAX && (i == 0 || nds[i] != nds[i - 1]);
+        cu += fl[m];
+    }
+    int nn = 0;
+    const int uoff = sy_block_excl(cu, wsum, &nn);
+    if (pass == 0) {
+        if (tid == 0) {
+            nruns[t] = (uint32_t)(patch ? nr : 0);
+            nnodes[t] = (uint32_t)nn;
+            atomicMax(&scal[SYS_MAXV], (uint32_t)nvals);
+            atomicMax(&scal[SYS_MAXRANK], (uint32_t)rr.z);
+            atomicMax(&scal[SYS_MAXU], (uint32_t)nU);
+        }
+        return;
+    }
+    for (int m = 0, o = uoff; m < 4; m++)
+        if (fl[m]) un[o++] = nds[tid * 4 + m];
+    __syncthreads();
+    const int64_t b = sy_slot(t, nt);
+    const int xb = (int)nnodes[t];  // scanned: this tile's first tile-local coordinate
+    // descriptors
+    if (tid == 0) {
+        const int32_t row0 = (int32_t)(rk[0] >> 8);
+        tiles[b] = make_int4(xb, d.z, ptr[row0], nvals);
+        tile_rows[b] = make_int4(row0, nrows, rr.z, nU);
+        if (patch) truns[b] = make_int2((int)nruns[t], nr);
+    }
+    // runs: the last row of each run records where it ends (the run of sorted
+    // row tid is the number of run starts up to it, less one)
+    if (patch) {
+        const bool rend = tid < nrows && (tid == nrows - 1 || (int32_t)(rk[tid + 1] >> 8) != qrow + 1);
+        if (rend) runend[ridx + (rstart ? 1 : 0) - 1] = tid;
+    }
+    __syncthreads();
+    if (patch && rstart) {
+        // this run: sorted rows tid .. runend[ridx]
+        const int qe = runend[ridx];
+        const int nv = vb[qe] + (ptr[(int32_t)(rk[qe] >> 8) + 1] - ptr[(int32_t)(rk[qe] >> 8)]) - vb[tid];
+        runs[nruns[t] + ridx] = make_int4(ptr[qrow], qrow, vb[tid] | (nv << 16), tid | ((qe - tid + 1) << 16));
+    }
+    // element list (padded with the first element)
+    telem[b * 256 + tid] = tid < nU ? el[tid] : el[0];
+    // tile-local coordinates and element nodes
+    for (int i = tid; i < nn; i += 256) txy[xb + i] = xy[un[i]];
+    {
+        uint16_t l[4] = {0, 0, 0, 0};
+        if (tid < nU)
+            for (int j = 0; j < NEN; j++)
+                l[j] = (uint16_t)sy_lower_bound<int32_t>(un, nn, conn[(int64_t)el[tid] * NEN + j]);
+        tconn[b * 256 + tid] = make_ushort4(l[0], l[1], l[2], l[3]);
+    }
+    // row records, at the row's sorted (dof-order) position
+    uint32_t w[RW];
+    for (int i = 0; i < RW; i++) w[i] = 0u;
+    int myq = 0;
+    if (tid < nrows) {
+        // my sorted position: the q with rk[q] & 255 == tid
+        myq = sy_lower_bound<uint64_t>(rk, nrows, ((uint64_t)(uint32_t)row << 8) | (uint64_t)tid);
+        auto put = [&](int o, uint32_t v, int nb) {
+            for (int bb = 0; bb < nb; bb++)
+                if ((v >> bb) & 1u) w[(o + bb) >> 5] |= 1u << ((o + bb) & 31);
+        };
+        put(0, (uint32_t)deg, 4);
+        put(4, (uint32_t)vb[myq], 13);
+        for (int k = 0; k < deg && k < KMAX; k++) {
+            const int o = 17 + k * FB;
+            put(o, (uint32_t)sy_lower_bound<int32_t>(el, nU, aelem[a0 + k]), 8);
+            put(o + 8, (uint32_t)aloc[a0 + k], 2);
+            for (int j = 0; j < NEN; j++) put(o + 10 + 4 * j, (uint32_t)(npos[(int64_t)(a0 + k) * NEN + j] & 15u), 4);
+        }
+    }
+    __syncthreads();
+    // position q gets the record of its row; positions past the rows get zeros
+    __shared__ uint32_t recs[256 * RW];
+    if (tid < nrows)
+        for (int i = 0; i < RW; i++) recs[myq * RW + i] = w[i];
+    __syncthreads();
+    for (int i = 0; i < RW; i++) rrec[(b * 256 + tid) * RW + i] = tid < nrows ? recs[tid * RW + i] : 0u;
+    (void)qtid;
+}
+
 }  // namespace
 
 // The pattern and the task tables of a task-tile problem, built on the device
@@ -769,7 +1193,7 @@ int device_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int block)
                            (const uint8_t *)npos, pstride, (int4 *)P.tile.p, (int4 *)P.tile_rows.p, (int32_t *)P.task_elem.p,
                            (uint32_t *)P.task_info.p, (uint32_t *)P.task_pos.p, scal);
     HIPCHK(c, hipGetLastError());
-    uint32_t hs[SYS_N] = {0, 0, 0, 0};
+    uint32_t hs[SYS_N] = {};
     HIPCHK(c, hipMemcpyAsync(hs, scal, 4 * SYS_N, hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
     if (hs[SYS_FLAGS]) return set_err(c, ELFEL_EINVAL, "internal: device symbolic phase flagged late");
@@ -783,6 +1207,221 @@ int device_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int block)
     P.ntiles = nt;
     P.ntasks = (int64_t)slots;
     P.ncolmax = ncolp;
+    P.maxrank = (int)hs[SYS_MAXRANK];
+    P.lds_vals = (int)hs[SYS_MAXV];
+    P.nnz = nnz;
+    return 0;
+}
+
+// The row-owner heat tiles of a scalar T3 / Q4 problem built on the device
+// into c->pat.  Returns 1 (nothing built) when the mesh does not fit the row
+// records (a node in more than kmax elements, a row of more than 16 entries):
+// the host builder then makes task tiles.  Returns 0 with c->pat.fallback set
+// when the reuse method cannot take the mesh at all (a dof shared by two
+// nodes, an element that repeats a dof).
+int device_symbolic_heat(elfel_hip_ctx *c, const elfel_hip_problem *pb, bool patch_allowed)
+{
+    Pattern &P = c->pat;
+    const Field &F0 = c->f[0];
+    const int nen = F0.nen;
+    if (F0.ncomp != 1 || (nen != 3 && nen != 4) || F0.sdim != 2) return 1;
+    SymIn S;
+    S.conn0 = (const int32_t *)F0.conn.p;
+    S.dof0 = (const int32_t *)F0.dof.p;
+    S.conn1 = S.dof1 = nullptr;
+    S.E = F0.E;
+    S.Nn0 = F0.Nn;
+    S.Np = 0;
+    S.n = c->nrow;
+    S.nen0 = nen;
+    S.nc = 1;
+    S.stokes = 0;
+    const int64_t N = S.Nn0, n = S.n, nent = S.E * nen;
+    const int kmax = nen == 4 ? hrow_kmax<4>() : hrow_kmax<3>();
+    const int RW = nen == 4 ? hrow_words<4, hrow_kmax<4>()>() : hrow_words<3, hrow_kmax<3>()>();
+    const int vcap = REUSE_VCAP_PER_THREAD * HEAT_ROWS_BLOCK;
+    const int pstride = nen;
+    // packing ranges of 16384 positions: a tile never spans two, so longer
+    // ranges leave fewer part-filled tiles (heat tiles take about 220 rows)
+    const int seg = 16384;
+    const int64_t nseg = std::max<int64_t>(1, (n + seg - 1) / seg);
+    const bool patch = patch_allowed && n >= (int64_t(1) << 18);
+    size_t off = 0;
+    auto take = [&](size_t bytes) { const size_t o = off; off += align256(bytes); return o; };
+    const int64_t n1 = std::max<int64_t>(n, 1), N1 = std::max<int64_t>(N, 1);
+    const size_t o_scal = take(4 * SYS_N), o_aptr = take(4 * (N + 1)), o_fill = take(4 * N1), o_aelem = take(4 * nent),
+                 o_aloc = take(nent), o_inv = take(4 * n1), o_fc = take(n1), o_nodeL = take(4 * N1),
+                 o_npos = take((size_t)nent * pstride), o_edof = take(4 * (size_t)S.E * pstride),
+                 o_rinfo = take(4 * n1), o_gmin = take(4 * n1), o_noder = take(4 * (n + 1)), o_seq = take(4 * n1),
+                 o_P = take(4 * n1), o_prev = take(4 * nent), o_bb = take(32), o_k0 = take(8 * N1), o_k1 = take(8 * N1),
+                 o_v0 = take(8 * N1), o_v1 = take(8 * N1), o_seg = take(4 * (nseg + 1)),
+                 o_tds = take(16 * (size_t)nseg * seg), o_trs = take(16 * (size_t)nseg * seg),
+                 o_scan = take(4 * (size_t)scan_scratch_entries(std::max(N, n) + 1)),
+                 o_hist = take(4 * (size_t)radix_hist_entries(std::max<int64_t>(N, (int64_t)nseg * seg)));
+    int rc;
+    if ((rc = ensure(c, P.sym, off))) return rc;
+    unsigned char *base = (unsigned char *)P.sym.p;
+    uint32_t *scal = (uint32_t *)(base + o_scal), *aptr = (uint32_t *)(base + o_aptr), *fill = (uint32_t *)(base + o_fill);
+    int32_t *aelem = (int32_t *)(base + o_aelem), *inv_node = (int32_t *)(base + o_inv);
+    uint8_t *aloc = base + o_aloc, *inv_fc = base + o_fc, *npos = base + o_npos;
+    uint32_t *nodeL = (uint32_t *)(base + o_nodeL), *rinfo = (uint32_t *)(base + o_rinfo);
+    int32_t *edof = (int32_t *)(base + o_edof), *gmin = (int32_t *)(base + o_gmin);
+    uint32_t *noder = (uint32_t *)(base + o_noder);
+    int32_t *seq = (int32_t *)(base + o_seq), *Ppos = (int32_t *)(base + o_P), *prev = (int32_t *)(base + o_prev);
+    unsigned long long *bb = (unsigned long long *)(base + o_bb);
+    uint64_t *k0 = (uint64_t *)(base + o_k0), *k1 = (uint64_t *)(base + o_k1);
+    uint64_t *v0 = (uint64_t *)(base + o_v0), *v1 = (uint64_t *)(base + o_v1);
+    uint32_t *segtiles = (uint32_t *)(base + o_seg);
+    int4 *tds = (int4 *)(base + o_tds), *trs = (int4 *)(base + o_trs);
+    SortScratch ss;
+    ss.scan = (uint32_t *)(base + o_scan);
+    ss.scan_cap = scan_scratch_entries(std::max(N, n) + 1);
+    ss.hist = (uint32_t *)(base + o_hist);
+    ss.hist_cap = radix_hist_entries(std::max<int64_t>(N, (int64_t)nseg * seg));
+    hipStream_t st = c->stream;
+    const int g = 256;
+    const int grid_ent = std::max(1, std::min(ceil_div(nent, g), 8192));
+    HIPCHK(c, hipMemsetAsync(scal, 0, 4 * SYS_N, st));
+    HIPCHK(c, hipMemsetAsync(aptr, 0, 4 * (N + 1), st));
+    HIPCHK(c, hipMemsetAsync(fill, 0, 4 * N1, st));
+    HIPCHK(c, hipMemsetAsync(inv_node, 0xFF, 4 * n1, st));
+    // adjacency, inverse map, element check, node list lengths, ptr
+    hipLaunchKernelGGL(k_sy_adj_count, dim3(grid_ent), dim3(g), 0, st, S, aptr);
+    if ((rc = exclusive_scan_u32(c, aptr, N + 1, nullptr, &ss))) return rc;
+    hipLaunchKernelGGL(k_sy_adj_fill, dim3(grid_ent), dim3(g), 0, st, S, (const uint32_t *)aptr, fill, aelem, aloc);
+    if (N > 0)
+        hipLaunchKernelGGL(k_sy_adj_sort, dim3(ceil_div(N, g)), dim3(g), 0, st, N, (const uint32_t *)aptr, aelem, aloc,
+                           scal);
+    hipLaunchKernelGGL(k_sy_inv, dim3(std::max(1, std::min(ceil_div(N, g), 8192))), dim3(g), 0, st, S, inv_node, inv_fc,
+                       scal);
+    if (S.E > 0) hipLaunchKernelGGL(k_sy_elem_check, dim3(ceil_div(S.E, g)), dim3(g), 0, st, S, pstride, edof, scal);
+    if (N > 0) {
+        hipLaunchKernelGGL(k_sy_node_cols, dim3(ceil_div(N, 4)), dim3(256), 0, st, S, N, (const uint32_t *)aptr,
+                           (const int32_t *)aelem, (const int32_t *)edof, 0, nodeL, (const int32_t *)nullptr,
+                           (int32_t *)nullptr, (uint8_t *)nullptr, pstride, scal);
+        hipLaunchKernelGGL(k_sy_hcheck, dim3(ceil_div(N, g)), dim3(g), 0, st, N, (const uint32_t *)aptr,
+                           (const uint32_t *)nodeL, scal);
+    }
+    if ((rc = ensure(c, P.ptr, 4 * (n + 1)))) return rc;
+    int32_t *ptr = (int32_t *)P.ptr.p;
+    hipLaunchKernelGGL(k_sy_rows, dim3(ceil_div(n + 1, g)), dim3(g), 0, st, S, (const int32_t *)inv_node,
+                       (const uint8_t *)inv_fc, (const uint32_t *)aptr, (const int32_t *)aelem, (const uint32_t *)nodeL,
+                       (uint32_t *)ptr, rinfo, gmin);
+    if ((rc = exclusive_scan_u32(c, (uint32_t *)ptr, n + 1, nullptr, &ss))) return rc;
+    uint32_t hv[SYS_N + 1] = {};
+    HIPCHK(c, hipMemcpyAsync(&hv[0], ptr + n, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(&hv[1], scal, 4 * SYS_N, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    const int64_t nnz = hv[0];
+    const uint32_t flags = hv[1 + SYS_FLAGS];
+    P.fallback = false;
+    if (flags & (SYF_SHARED | SYF_REPEAT)) {
+        P.fallback = true;
+        P.fallback_note = std::string("plain FP64 method: ") +
+                          ((flags & SYF_SHARED) ? "a dof shared by two nodes or components" : "an element with a repeated dof");
+        P.nnz = nnz;
+        return 0;
+    }
+    if (flags || hv[1 + SYS_MAXDEG] > (uint32_t)kmax || hv[1 + SYS_MAXL] > 16u) return 1;  // task tiles (host)
+    HIPCHK(c, hipMemsetAsync(scal + SYS_MAXV, 0, 4 * (SYS_N - SYS_MAXV), st));
+    if ((rc = ensure(c, P.ind, 4 * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
+    if (N > 0)
+        hipLaunchKernelGGL(k_sy_node_cols, dim3(ceil_div(N, 4)), dim3(256), 0, st, S, N, (const uint32_t *)aptr,
+                           (const int32_t *)aelem, (const int32_t *)edof, 1, nodeL, (const int32_t *)ptr,
+                           (int32_t *)P.ind.p, npos, pstride, scal);
+    // row order: nodes in bands (patch tiles) or in dof order, then the rows without a node
+    const uint64_t *order = nullptr;
+    if (patch && N > 1) {
+        const unsigned long long init[4] = {~0ull, 0ull, ~0ull, 0ull};
+        HIPCHK(c, hipMemcpyAsync(bb, init, sizeof init, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_sy_bbox, dim3(std::max(1, std::min(ceil_div(N, g), 4096))), dim3(g), 0, st, N,
+                           (const double2 *)F0.xy.p, bb);
+        hipLaunchKernelGGL(k_sy_band_keys, dim3(ceil_div(N, g)), dim3(g), 0, st, N, (const double2 *)F0.xy.p,
+                           (const unsigned long long *)bb, std::max(1, env_int("ELFEL_HIP_BAND_ROWS", 8)), k0, v0);
+        if ((rc = radix_sort_pairs(c, k0, v0, k1, v1, N, 64, &ss))) return rc;
+        order = v0;
+    }
+    hipLaunchKernelGGL(k_sy_noder, dim3(ceil_div(n + 1, g)), dim3(g), 0, st, n, (const int32_t *)inv_node, noder);
+    if ((rc = exclusive_scan_u32(c, noder, n + 1, nullptr, &ss))) return rc;
+    hipLaunchKernelGGL(k_sy_seq, dim3(ceil_div(std::max(n, N), g)), dim3(g), 0, st, n, N, order,
+                       (const int32_t *)S.dof0, (const int32_t *)inv_node, (const uint32_t *)noder, seq, Ppos);
+    if (N > 0)
+        hipLaunchKernelGGL(k_sy_prev, dim3(ceil_div(N, g)), dim3(g), 0, st, N, nen, S.conn0, S.dof0,
+                           (const uint32_t *)aptr, (const int32_t *)aelem, (const int32_t *)Ppos, prev);
+    // packing, dense tiles
+    if (nen == 4)
+        hipLaunchKernelGGL((k_sy_hpack<hrow_kmax<4>()>), dim3((unsigned)nseg), dim3(64), 0, st, n, nseg, seg,
+                           (const int32_t *)seq, (const int32_t *)inv_node, (const uint32_t *)aptr,
+                           (const int32_t *)prev, (const int32_t *)ptr, vcap, segtiles, tds, trs);
+    else
+        hipLaunchKernelGGL((k_sy_hpack<hrow_kmax<3>()>), dim3((unsigned)nseg), dim3(64), 0, st, n, nseg, seg,
+                           (const int32_t *)seq, (const int32_t *)inv_node, (const uint32_t *)aptr,
+                           (const int32_t *)prev, (const int32_t *)ptr, vcap, segtiles, tds, trs);
+    HIPCHK(c, hipMemsetAsync(segtiles + nseg, 0, 4, st));
+    if ((rc = exclusive_scan_u32(c, segtiles, nseg + 1, nullptr, &ss))) return rc;
+    uint32_t hn = 0;
+    HIPCHK(c, hipMemcpyAsync(&hn, segtiles + nseg, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    const int64_t nt = hn, ntc = std::max<int64_t>(nt, 1);
+    size_t off2 = 0;
+    auto take2 = [&](size_t bytes) { const size_t o = off2; off2 += align256(bytes); return o; };
+    const size_t o_tdo = take2(16 * ntc), o_tro = take2(16 * ntc), o_nr = take2(4 * (ntc + 1)), o_nn = take2(4 * (ntc + 1));
+    if ((rc = ensure(c, P.sym2, off2))) return rc;
+    unsigned char *base2 = (unsigned char *)P.sym2.p;
+    int4 *tdo = (int4 *)(base2 + o_tdo), *tro = (int4 *)(base2 + o_tro);
+    uint32_t *nruns = (uint32_t *)(base2 + o_nr), *nnodes = (uint32_t *)(base2 + o_nn);
+    hipLaunchKernelGGL(k_sy_hdense, dim3((unsigned)nseg), dim3(64), 0, st, nseg, seg, (const uint32_t *)segtiles,
+                       (const int4 *)tds, (const int4 *)trs, tdo, tro);
+    // tile pass 0: runs and tile-local nodes per tile, scanned
+#define HTILE(NN, PASS)                                                                                                \
+    hipLaunchKernelGGL((k_sy_htile<NN>), dim3((unsigned)nt), dim3(256), 0, st, nt, PASS, patch ? 1 : 0,                \
+                       (const int32_t *)seq, (const int32_t *)inv_node, (const uint32_t *)aptr, (const int32_t *)aelem, \
+                       (const uint8_t *)aloc, (const int32_t *)prev, (const uint8_t *)npos, (const int32_t *)ptr,     \
+                       S.conn0, (const double2 *)F0.xy.p, (const int4 *)tdo, (const int4 *)tro, nruns, nnodes,        \
+                       (int4 *)P.tile.p, (int4 *)P.tile_rows.p, (int2 *)P.truns.p, (int4 *)P.runs.p,                   \
+                       (int32_t *)P.tile_elem.p, (uint32_t *)P.rrec.p, (double2 *)P.txy.p, (ushort4 *)P.tconn.p, scal)
+    if (nt > 0) {
+        if (nen == 4) HTILE(4, 0);
+        else HTILE(3, 0);
+    }
+    HIPCHK(c, hipMemsetAsync(nruns + nt, 0, 4, st));
+    HIPCHK(c, hipMemsetAsync(nnodes + nt, 0, 4, st));
+    if ((rc = exclusive_scan_u32(c, nruns, nt + 1, nullptr, &ss)) || (rc = exclusive_scan_u32(c, nnodes, nt + 1, nullptr, &ss)))
+        return rc;
+    uint32_t ht[2] = {0, 0}, hs[SYS_N] = {};
+    HIPCHK(c, hipMemcpyAsync(&ht[0], nruns + nt, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(&ht[1], nnodes + nt, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(hs, scal, 4 * SYS_N, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    const int64_t totruns = ht[0], totnodes = ht[1];
+    if ((rc = ensure(c, P.tile, sizeof(int4) * ntc)) || (rc = ensure(c, P.tile_rows, sizeof(int4) * ntc)) ||
+        (rc = ensure(c, P.tile_elem, 4 * (size_t)ntc * HEAT_ROWS_BLOCK)) ||
+        (rc = ensure(c, P.rrec, 4 * (size_t)ntc * HEAT_ROWS_BLOCK * RW)) ||
+        (rc = ensure(c, P.tconn, sizeof(ushort4) * (size_t)ntc * HEAT_ROWS_BLOCK)) ||
+        (rc = ensure(c, P.txy, sizeof(double2) * (size_t)std::max<int64_t>(totnodes, 1))))
+        return rc;
+    if (patch) {
+        if ((rc = ensure(c, P.truns, sizeof(int2) * ntc)) || (rc = ensure(c, P.runs, sizeof(int4) * std::max<int64_t>(totruns, 1))))
+            return rc;
+    } else {
+        release(P.truns); release(P.runs);
+    }
+    if (nt > 0) {
+        if (nen == 4) HTILE(4, 1);
+        else HTILE(3, 1);
+    }
+#undef HTILE
+    HIPCHK(c, hipGetLastError());
+    release(P.task_elem); release(P.task_info); release(P.task_pos);
+    P.rw = RW;
+    P.tstride = HEAT_ROWS_BLOCK;
+    P.tgeom = true;
+    P.has_runs = patch;
+    P.elem8 = false;
+    P.maxu = (int)hs[SYS_MAXU];
+    P.ntiles = nt;
+    P.ntasks = 0;
+    P.ncolmax = (nen + 3) & ~3;
     P.maxrank = (int)hs[SYS_MAXRANK];
     P.lds_vals = (int)hs[SYS_MAXV];
     P.nnz = nnz;

@@ -137,3 +137,53 @@ def test_device_symbolic_first_call_time():
     print(f"C2: FP64 plain {t_plain:.2f} ms, first mixed REUSE call {t_first:.2f} ms")
     assert t_first <= t_plain + 20.0
     ctx.close()
+
+
+@pytest.mark.parametrize("N,shuffle", [(60, False), (513, False), (513, True)])
+def test_device_symbolic_heat_q4(monkeypatch, N, shuffle):
+    """Row-owner heat tiles built on the device (band order and runs from
+    2^18 rows up) and by the host builder: FP64 REUSE equals the oracle bit
+    for bit (matrix and load vector) either way, mixed within 1e-6."""
+    cfg = W.heat_q4(N, sigma=1.0)
+    f, n = cfg["fesp"], cfg["ndofs"]
+    dn = f.dofnums.copy()
+    if shuffle:
+        dn = (np.random.default_rng(11).permutation(n) + 1)[dn - 1]
+    p = O.make_problem(O.HEAT, O.Q4, 2, f.mesh.conn, f.mesh.xy, dn, kappa_e=cfg["kappa_e"], Q_e=cfg["Q_e"])
+    ref = O.assemble(p, n, csc=False)
+    for dev in ("1", "0"):
+        monkeypatch.setenv("ELFEL_HIP_DEVICE_SYMBOLIC", dev)
+        ctx = L.Context(0)
+        ctx.set_mesh(0, f.mesh.conn, f.mesh.xy)
+        ctx.set_dofs(0, dn, n)
+        ctx.set_coefficients(f.mesh.nelements, cfg["kappa_e"], cfg["Q_e"], None)
+        for prec in (L.FP64, L.MIXED):
+            ctx.assemble(L.HEAT, L.Q4, 2, precision=prec, method=L.REUSE, layout=L.CSR)
+            assert ctx.assembly_info()[:2] == (L.REUSE, prec)
+            ptr, ind, val = ctx.copy_sparse(base=0)[:3]
+            F = ctx.copy_vector(n)
+            assert np.array_equal(ptr, ref[0]) and np.array_equal(ind, ref[1]), (dev, prec)
+            if prec == L.FP64:
+                assert np.array_equal(val, ref[2]) and np.array_equal(F, ref[3]), (dev, N, shuffle)
+            else:
+                assert frob_rel(val, ref[2]) <= MIXED_TOL and frob_rel(F, ref[3]) <= MIXED_TOL
+        ctx.close()
+
+
+def test_device_symbolic_heat_t3_fan_uses_task_tiles():
+    """A T3 node in 12 elements: more than a row record holds, so the host
+    builder makes task tiles; the result still equals the oracle."""
+    K = 12
+    xy = [(0.0, 0.0)] + [(np.cos(2 * np.pi * k / K), np.sin(2 * np.pi * k / K)) for k in range(K)]
+    conn = np.array([(1, 2 + k, 2 + (k + 1) % K) for k in range(K)], np.int64)
+    xy = np.array(xy)
+    dn = np.arange(1, K + 2, dtype=np.int64).reshape(-1, 1)
+    ref = O.assemble(O.make_problem(O.HEAT, O.T3, 1, conn, xy, dn, kappa=1.0, Q=1.0), K + 1, csc=False)
+    ctx = L.Context(0)
+    ctx.set_mesh(0, conn, xy)
+    ctx.set_dofs(0, dn, K + 1)
+    ctx.assemble(L.HEAT, L.T3, 1, precision=L.FP64, method=L.REUSE, layout=L.CSR, kappa=1.0, Q=1.0)
+    assert ctx.assembly_info()[:2] == (L.REUSE, L.FP64)
+    ptr, ind, val = ctx.copy_sparse(base=0)[:3]
+    assert np.array_equal(ptr, ref[0]) and np.array_equal(ind, ref[1]) and np.array_equal(val, ref[2])
+    ctx.close()
