@@ -176,11 +176,15 @@ __device__ __forceinline__ uint32_t sy_shfl_xor(uint32_t v, int m) { return (uin
 __device__ __forceinline__ uint32_t sy_shfl(uint32_t v, int src) { return (uint32_t)__shfl((int)v, src, 64); }
 __device__ __forceinline__ uint32_t sy_shfl_up1(uint32_t v) { return (uint32_t)__shfl_up((int)v, 1, 64); }
 
-template <int R, typename K>
+template <int R, typename K, int SIZE = 64 * R>
 __device__ __forceinline__ void sy_wave_sort(K (&v)[R], int lane)
 {
+    // SIZE < 64 (R = 1): each group of SIZE lanes sorts its own values
+    // (partners lane ^ j, j < SIZE, stay inside the group; the direction
+    // comes from the lane's index in its group)
+    const int dl = SIZE < 64 ? (lane & (SIZE - 1)) : lane;
 #pragma unroll
-    for (int k = 2; k <= 64 * R; k <<= 1) {
+    for (int k = 2; k <= SIZE; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
             if (j >= 64) {
@@ -188,7 +192,7 @@ __device__ __forceinline__ void sy_wave_sort(K (&v)[R], int lane)
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     if ((r & rj) != 0) continue;
-                    const bool up = ((r * 64 + lane) & k) == 0;
+                    const bool up = ((r * 64 + dl) & k) == 0;
                     const K a = v[r], b = v[r | rj];
                     const bool sw = (a > b) == up;
                     v[r] = sw ? b : a;
@@ -198,8 +202,8 @@ __device__ __forceinline__ void sy_wave_sort(K (&v)[R], int lane)
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     const K p = sy_shfl_xor(v[r], j);
-                    const bool up = ((r * 64 + lane) & k) == 0;
-                    const bool takemin = ((lane & j) == 0) == up;
+                    const bool up = ((r * 64 + dl) & k) == 0;
+                    const bool takemin = ((dl & j) == 0) == up;
                     v[r] = takemin ? (v[r] < p ? v[r] : p) : (v[r] < p ? p : v[r]);
                 }
             }
@@ -213,13 +217,18 @@ __device__ __forceinline__ void sy_wave_sort(K (&v)[R], int lane)
 // duplicates of a dof dropped; the list goes into ind at each of the node's
 // rows, and each candidate's position in it into npos (one byte, at
 // (aptr[node] + rank) * pstride + column).
-template <int R, typename K>
+template <int R, typename K, int W = 64>
 __device__ __forceinline__ void sy_node_cols_body(const SymIn &S, int64_t nd, int lane, uint32_t p0, int cnt, int ncol,
                                                   const int32_t *__restrict__ aelem, const int32_t *__restrict__ edof,
                                                   int pass, uint32_t *__restrict__ nodeL,
                                                   const int32_t *__restrict__ ptr, int32_t *__restrict__ ind,
                                                   uint8_t *__restrict__ npos, int pstride, uint32_t *__restrict__ scal)
 {
+    // W < 64: the node is one of 64 / W in the wavefront, on lanes
+    // [g W, g W + W); its candidate i sits on lane g W + i (R = 1)
+    static_assert(W == 64 || R == 1, "groups of fewer than 64 lanes hold one value a lane");
+    const int glane = lane % W;
+    const uint64_t gmask = W == 64 ? ~0ull : (((1ull << (W & 63)) - 1ull) << (lane - glane));
     // keys: pass 0 node numbers, pass 1 dof << 8 | candidate (32-bit keys
     // when every dof is below 2^23)
     constexpr K PAD = (K)~(K)0;
@@ -230,7 +239,7 @@ __device__ __forceinline__ void sy_node_cols_body(const SymIn &S, int64_t nd, in
     const int nnv = S.nen0 + ((S.stokes && nd < S.Nn0) ? 3 : 0);
 #pragma unroll
     for (int r = 0; r < R; r++) {
-        const int i = r * 64 + lane;
+        const int i = r * 64 + glane;
         v[r] = PAD;
         if (pass == 0) {
             if (i < cnt) {
@@ -242,7 +251,7 @@ __device__ __forceinline__ void sy_node_cols_body(const SymIn &S, int64_t nd, in
             v[r] = ((K)(uint32_t)edof[(int64_t)aelem[p0 + i / ncol] * pstride + i % ncol] << 8) | (K)i;
         }
     }
-    sy_wave_sort<R, K>(v, lane);
+    sy_wave_sort<R, K, W * R>(v, lane);
     // a value is kept when it differs from the one before; every candidate's
     // position is the number of kept values up to it, less one
     int run = 0;
@@ -265,14 +274,15 @@ __device__ __forceinline__ void sy_node_cols_body(const SymIn &S, int64_t nd, in
         const K last = r > 0 ? sy_shfl(v[r > 0 ? r - 1 : 0], 63) : PAD;
         const K prev = lane == 0 ? last : up;
         const bool real = v[r] != PAD;
-        const bool keep = real && (r * 64 + lane == 0 || (v[r] >> sh) != (prev >> sh));
+        const bool keep = real && (r * 64 + glane == 0 || (v[r] >> sh) != (prev >> sh));
         if (pass == 0) {
-            const uint64_t mv = __ballot(keep && v[r] < (K)S.Nn0), mp = __ballot(keep && v[r] >= (K)S.Nn0);
+            const uint64_t mv = __ballot(keep && v[r] < (K)S.Nn0) & gmask,
+                           mp = __ballot(keep && v[r] >= (K)S.Nn0) & gmask;
             nl += S.nc * __popcll(mv) + __popcll(mp);
             continue;
         }
-        const uint64_t m = __ballot(keep);
-        const int incl = run + __popcll(m & ((2ull << lane) - 1ull));  // lanes 0..lane
+        const uint64_t m = __ballot(keep) & gmask;
+        const int incl = run + __popcll(m & ((2ull << lane) - 1ull));  // lanes of the group up to this one
         if (pass == 1 && real) {
             const int pos = incl - 1;
             if (keep && pos < SY_CMAX)
@@ -282,7 +292,7 @@ __device__ __forceinline__ void sy_node_cols_body(const SymIn &S, int64_t nd, in
         }
         run += __popcll(m);
     }
-    if (pass == 0 && lane == 0) {
+    if (pass == 0 && glane == 0) {
         nodeL[nd] = (uint32_t)nl;
         if (nl > SY_CMAX) atomicOr(&scal[SYS_FLAGS], (uint32_t)SYF_LONG);
     }
@@ -290,26 +300,38 @@ __device__ __forceinline__ void sy_node_cols_body(const SymIn &S, int64_t nd, in
 
 __global__ __launch_bounds__(256) void k_sy_node_cols(SymIn S, int64_t N, const uint32_t *__restrict__ aptr,
                                                       const int32_t *__restrict__ aelem,
-                                                      const int32_t *__restrict__ edof, int pass,
+                                                      const int32_t *__restrict__ edof, int pass, int npw,
                                                       uint32_t *__restrict__ nodeL, const int32_t *__restrict__ ptr,
                                                       int32_t *__restrict__ ind, uint8_t *__restrict__ npos,
                                                       int pstride, uint32_t *__restrict__ scal)
 {
-    // four nodes per workgroup, one per wavefront (the waves share nothing)
-    const int64_t nd = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    // four wavefronts per workgroup (they share nothing), npw nodes per
+    // wavefront: 1, or 2 / 4 when every node has at most 32 / 16 candidates
     const int lane = threadIdx.x & 63;
+    const int64_t nd = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * npw + lane / (64 / npw);
     if (nd >= N) return;
     const uint32_t p0 = aptr[nd], deg = aptr[nd + 1] - p0;
     if (deg > 16) {  // flagged by k_sy_adj_sort
-        if (lane == 0 && pass == 0) nodeL[nd] = 0;
+        if (lane % (64 / npw) == 0 && pass == 0) nodeL[nd] = 0;
         return;
     }
     const int ncol = sy_ncol(S, nd >= S.Nn0 ? 1 : 0);
     // candidates: pass 0 the nodes, pass 1 the columns of each element (<= 16 x 15)
     const int cnt = (int)deg * (pass == 0 ? S.nen0 + ((S.stokes && nd < S.Nn0) ? 3 : 0) : ncol);
+    const bool k32 = pass == 0 || S.n <= (1 << 23);
+    if (npw == 4) {
+        if (k32) sy_node_cols_body<1, uint32_t, 16>(S, nd, lane, p0, cnt, ncol, aelem, edof, pass, nodeL, ptr, ind, npos, pstride, scal);
+        else sy_node_cols_body<1, uint64_t, 16>(S, nd, lane, p0, cnt, ncol, aelem, edof, pass, nodeL, ptr, ind, npos, pstride, scal);
+        return;
+    }
+    if (npw == 2) {
+        if (k32) sy_node_cols_body<1, uint32_t, 32>(S, nd, lane, p0, cnt, ncol, aelem, edof, pass, nodeL, ptr, ind, npos, pstride, scal);
+        else sy_node_cols_body<1, uint64_t, 32>(S, nd, lane, p0, cnt, ncol, aelem, edof, pass, nodeL, ptr, ind, npos, pstride, scal);
+        return;
+    }
 #define SY_BODY(RR, KK) \
     sy_node_cols_body<RR, KK>(S, nd, lane, p0, cnt, ncol, aelem, edof, pass, nodeL, ptr, ind, npos, pstride, scal)
-    if (pass == 0 || S.n <= (1 << 23)) {
+    if (k32) {
         if (cnt <= 64) SY_BODY(1, uint32_t);
         else if (cnt <= 128) SY_BODY(2, uint32_t);
         else SY_BODY(4, uint32_t);
@@ -600,6 +622,8 @@ __global__ __launch_bounds__(256) void k_sy_tasks(SymIn S, int64_t nt, int cap, 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 
+
+
 // ---- heat row-owner tiles ------------------------------------------------------
 // The tiles of heat_rows_tile (reuse_kernel.h) for scalar T3 / Q4 problems:
 // rows packed greedily in a locality order of their nodes (bands of about
@@ -638,7 +662,14 @@ __global__ void k_sy_bbox(int64_t Nn, const double2 *__restrict__ xy, unsigned l
         mny = min(mny, (unsigned long long)__shfl_xor((long long)mny, d, 64));
         mxy = max(mxy, (unsigned long long)__shfl_xor((long long)mxy, d, 64));
     }
-    if ((threadIdx.x & 63) == 0) {
+    __shared__ unsigned long long sm[4][4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sm[0][w] = mnx; sm[1][w] = mxx; sm[2][w] = mny; sm[3][w] = mxy; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) {
+            mnx = min(mnx, sm[0][q]); mxx = max(mxx, sm[1][q]); mny = min(mny, sm[2][q]); mxy = max(mxy, sm[3][q]);
+        }
         atomicMin(&bb[0], mnx); atomicMax(&bb[1], mxx); atomicMin(&bb[2], mny); atomicMax(&bb[3], mxy);
     }
 }
@@ -713,19 +744,30 @@ __global__ void k_sy_prev(int64_t Nn, int nen, const int32_t *__restrict__ conn,
     }
 }
 
-// the most elements at a node and the longest node list (row records hold
-// kmax elements and rows of 16 entries)
-__global__ void k_sy_hcheck(int64_t N, const uint32_t *__restrict__ aptr, const uint32_t *__restrict__ nodeL,
-                            uint32_t *__restrict__ scal)
+// The most elements at a node (scal[SYS_MAXDEG]) and, given nodeL, the
+// longest node list (scal[SYS_MAXL]): a grid-stride loop, wave and
+// workgroup reductions, one atomic per workgroup.
+__global__ __launch_bounds__(256) void k_sy_max(int64_t N, const uint32_t *__restrict__ aptr,
+                                                const uint32_t *__restrict__ nodeL, uint32_t *__restrict__ scal)
 {
-    const int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t sm[2][4];
     uint32_t dg = 0, l = 0;
-    if (nd < N) { dg = aptr[nd + 1] - aptr[nd]; l = nodeL[nd]; }
+    for (int64_t nd = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; nd < N; nd += (int64_t)gridDim.x * blockDim.x) {
+        dg = max(dg, aptr[nd + 1] - aptr[nd]);
+        if (nodeL) l = max(l, nodeL[nd]);
+    }
     for (int d = 32; d > 0; d >>= 1) {
         dg = max(dg, (uint32_t)__shfl_xor((int)dg, d, 64));
         l = max(l, (uint32_t)__shfl_xor((int)l, d, 64));
     }
-    if ((threadIdx.x & 63) == 0) { atomicMax(&scal[SYS_MAXDEG], dg); atomicMax(&scal[SYS_MAXL], l); }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sm[0][w] = dg; sm[1][w] = l; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < 4; q++) { dg = max(dg, sm[0][q]); l = max(l, sm[1][q]); }
+        atomicMax(&scal[SYS_MAXDEG], dg);
+        if (nodeL) atomicMax(&scal[SYS_MAXL], l);
+    }
 }
 
 // Greedy packing of the row order, one wavefront per range of `seg`
@@ -1023,6 +1065,21 @@ __global__ __launch_bounds__(256) void k_sy_htile(int64_t nt, int pass, int patc
     (void)qtid;
 }
 
+// nodes per wavefront of k_sy_node_cols for at most c candidates a node
+int sy_npw(int64_t c) { return c <= 16 ? 4 : (c <= 32 ? 2 : 1); }
+
+// the most elements at a node, read back (one synchronisation)
+int sy_maxdeg(elfel_hip_ctx *c, int64_t N, const uint32_t *aptr, uint32_t *scal, uint32_t *out)
+{
+    *out = 0;
+    if (N <= 0) return 0;
+    hipLaunchKernelGGL(k_sy_max, dim3(std::max(1, std::min(ceil_div(N, 256), 1024))), dim3(256), 0, c->stream, N, aptr,
+                       (const uint32_t *)nullptr, scal);
+    HIPCHK(c, hipMemcpyAsync(out, scal + SYS_MAXDEG, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 }  // namespace
 
 // The pattern and the task tables of a task-tile problem, built on the device
@@ -1107,11 +1164,13 @@ int device_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int block)
                        inv_fc, scal);
     if (S.E > 0) hipLaunchKernelGGL(k_sy_elem_check, dim3(ceil_div(S.E, g)), dim3(g), 0, st, S, pstride, edof, scal);
     // 3. node column lists: lengths, row lengths, ptr; then the lists into ind
+    uint32_t maxdeg = 0;
+    if ((rc = sy_maxdeg(c, N, aptr, scal, &maxdeg))) return rc;
+    const int npw0 = sy_npw((int64_t)maxdeg * (S.nen0 + (stokes ? 3 : 0))), npw1 = sy_npw((int64_t)maxdeg * pstride);
     if (N > 0)
-        hipLaunchKernelGGL(k_sy_node_cols, dim3(ceil_div(N, 4)), dim3(256), 0, st, S, N, (const uint32_t *)aptr,
-                           (const int32_t *)aelem, (const int32_t *)edof, 0, nodeL, (const int32_t *)nullptr,
-                           (int32_t *)nullptr,
-                           (uint8_t *)nullptr, pstride, scal);
+        hipLaunchKernelGGL(k_sy_node_cols, dim3(ceil_div(N, 4 * npw0)), dim3(256), 0, st, S, N, (const uint32_t *)aptr,
+                           (const int32_t *)aelem, (const int32_t *)edof, 0, npw0, nodeL, (const int32_t *)nullptr,
+                           (int32_t *)nullptr, (uint8_t *)nullptr, pstride, scal);
     if ((rc = ensure(c, P.ptr, 4 * (n + 1)))) return rc;
     int32_t *ptr = (int32_t *)P.ptr.p;
     hipLaunchKernelGGL(k_sy_rows, dim3(ceil_div(n + 1, g)), dim3(g), 0, st, S, (const int32_t *)inv_node,
@@ -1136,8 +1195,8 @@ int device_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int block)
     }
     if ((rc = ensure(c, P.ind, 4 * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
     if (N > 0)
-        hipLaunchKernelGGL(k_sy_node_cols, dim3(ceil_div(N, 4)), dim3(256), 0, st, S, N, (const uint32_t *)aptr,
-                           (const int32_t *)aelem, (const int32_t *)edof, 1, nodeL, (const int32_t *)ptr,
+        hipLaunchKernelGGL(k_sy_node_cols, dim3(ceil_div(N, 4 * npw1)), dim3(256), 0, st, S, N, (const uint32_t *)aptr,
+                           (const int32_t *)aelem, (const int32_t *)edof, 1, npw1, nodeL, (const int32_t *)ptr,
                            (int32_t *)P.ind.p, npos, pstride, scal);
     // 4. packing (sparse tile numbers), counts scanned, dense tiles and keys
     hipLaunchKernelGGL(k_sy_pack, dim3((unsigned)nseg), dim3(64), 0, st, n, nseg, (const uint32_t *)rinfo,
@@ -1295,12 +1354,16 @@ int device_symbolic_heat(elfel_hip_ctx *c, const elfel_hip_problem *pb, bool pat
     hipLaunchKernelGGL(k_sy_inv, dim3(std::max(1, std::min(ceil_div(N, g), 8192))), dim3(g), 0, st, S, inv_node, inv_fc,
                        scal);
     if (S.E > 0) hipLaunchKernelGGL(k_sy_elem_check, dim3(ceil_div(S.E, g)), dim3(g), 0, st, S, pstride, edof, scal);
+    uint32_t maxdeg = 0;
+    if ((rc = sy_maxdeg(c, N, aptr, scal, &maxdeg))) return rc;
+    if (maxdeg > (uint32_t)kmax) return 1;  // task tiles (host)
+    const int npw = sy_npw((int64_t)maxdeg * nen);  // both passes: nen candidates per element
     if (N > 0) {
-        hipLaunchKernelGGL(k_sy_node_cols, dim3(ceil_div(N, 4)), dim3(256), 0, st, S, N, (const uint32_t *)aptr,
-                           (const int32_t *)aelem, (const int32_t *)edof, 0, nodeL, (const int32_t *)nullptr,
+        hipLaunchKernelGGL(k_sy_node_cols, dim3(ceil_div(N, 4 * npw)), dim3(256), 0, st, S, N, (const uint32_t *)aptr,
+                           (const int32_t *)aelem, (const int32_t *)edof, 0, npw, nodeL, (const int32_t *)nullptr,
                            (int32_t *)nullptr, (uint8_t *)nullptr, pstride, scal);
-        hipLaunchKernelGGL(k_sy_hcheck, dim3(ceil_div(N, g)), dim3(g), 0, st, N, (const uint32_t *)aptr,
-                           (const uint32_t *)nodeL, scal);
+        hipLaunchKernelGGL(k_sy_max, dim3(std::max(1, std::min(ceil_div(N, g), 1024))), dim3(g), 0, st, N,
+                           (const uint32_t *)aptr, (const uint32_t *)nodeL, scal);
     }
     if ((rc = ensure(c, P.ptr, 4 * (n + 1)))) return rc;
     int32_t *ptr = (int32_t *)P.ptr.p;
@@ -1326,15 +1389,15 @@ int device_symbolic_heat(elfel_hip_ctx *c, const elfel_hip_problem *pb, bool pat
     HIPCHK(c, hipMemsetAsync(scal + SYS_MAXV, 0, 4 * (SYS_N - SYS_MAXV), st));
     if ((rc = ensure(c, P.ind, 4 * (size_t)std::max<int64_t>(nnz, 1)))) return rc;
     if (N > 0)
-        hipLaunchKernelGGL(k_sy_node_cols, dim3(ceil_div(N, 4)), dim3(256), 0, st, S, N, (const uint32_t *)aptr,
-                           (const int32_t *)aelem, (const int32_t *)edof, 1, nodeL, (const int32_t *)ptr,
+        hipLaunchKernelGGL(k_sy_node_cols, dim3(ceil_div(N, 4 * npw)), dim3(256), 0, st, S, N, (const uint32_t *)aptr,
+                           (const int32_t *)aelem, (const int32_t *)edof, 1, npw, nodeL, (const int32_t *)ptr,
                            (int32_t *)P.ind.p, npos, pstride, scal);
     // row order: nodes in bands (patch tiles) or in dof order, then the rows without a node
     const uint64_t *order = nullptr;
     if (patch && N > 1) {
         const unsigned long long init[4] = {~0ull, 0ull, ~0ull, 0ull};
         HIPCHK(c, hipMemcpyAsync(bb, init, sizeof init, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_sy_bbox, dim3(std::max(1, std::min(ceil_div(N, g), 4096))), dim3(g), 0, st, N,
+        hipLaunchKernelGGL(k_sy_bbox, dim3(std::max(1, std::min(ceil_div(N, g), 1024))), dim3(g), 0, st, N,
                            (const double2 *)F0.xy.p, bb);
         hipLaunchKernelGGL(k_sy_band_keys, dim3(ceil_div(N, g)), dim3(g), 0, st, N, (const double2 *)F0.xy.p,
                            (const unsigned long long *)bb, std::max(1, env_int("ELFEL_HIP_BAND_ROWS", 8)), k0, v0);
