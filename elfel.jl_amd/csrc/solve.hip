@@ -6,19 +6,21 @@
 // Alg. 9.1) with every vector and the matrix in FP64; when the recurrence
 // residual reaches the tolerance, the true residual b - A x is formed and
 // replaces it (the iteration goes on if the true one is not yet small enough).
-// Mixed: the same PCG with an FP32 matrix copy and FP32 vectors, FP64 dot
-// products, and reliable updates (Sleijpen & van der Vorst, Computing 56,
-// 1996): whenever the recurrence residual has fallen by RU_DROP since the last
-// update, the FP32 iterate is added into the FP64 solution, the FP64 residual
+// Mixed: the same PCG with an FP32 matrix copy, FP32 r, z, p and q, the FP64
+// iterate x (x += alpha p in FP64), FP64 dot products, and reliable updates
+// (Sleijpen & van der Vorst, Computing 56, 1996): whenever the recurrence
+// residual has fallen by RU_DROP since the last update, the FP64 residual
 // b - A x is formed with the FP64 matrix and replaces the FP32 one, and the
 // search direction is kept unless the recurrence has drifted below half the
-// true residual (then p restarts from z).  An FP32 iteration moves about 0.6
-// of the bytes of an FP64 one.  The FP32 recurrence loses accuracy as the
-// condition number grows, so the iteration count rises with the mesh: on heat
-// Q4 with lognormal(0, 1) conductivity, +9% over FP64 PCG at 256x256 and +56%
-// at 1024x1024 (numpy emulation of this code), +45% at 4096x4096 (measured:
-// 19224 against 13280 iterations).  Updates at every 10x drop, or never
-// restarting, fail to converge at 4096x4096.
+// true residual (then p restarts from z).  An iteration of the mixed solve
+// moves about 0.55 of the bytes of an FP64 one.  With an FP32 iterate (added
+// into the FP64 solution at each update) the iteration count rose with the
+// mesh: on heat Q4 with lognormal(0, 1) conductivity, +9 % over FP64 PCG at
+// 256x256 and +69 % at 1024x1024 (numpy emulation of this code), +45 % at
+// 4096x4096 (measured: 19224 against 13280); the FP64 iterate keeps the late,
+// small corrections that an FP32 one rounds away (+9 % at 1024x1024).
+// Updates at every 10x drop, or never restarting, failed to converge at
+// 4096x4096 (FP32 iterate).
 // Convergence is declared on the FP64 residual: ||b - A x|| / ||b|| <= tol.
 //
 // Storage: the free-free block in SELL-64 (one slice per 64 consecutive rows,
@@ -256,8 +258,12 @@ __global__ __launch_bounds__(SV_BLOCK) void k_cg_spmv(int64_t n, const uint32_t 
 
 // alpha = rz_old / pq; x += alpha p; r -= alpha q; z = M r; partials r.z (into
 // rz_new) and r.r
-template <typename T>
-__global__ __launch_bounds__(SV_BLOCK) void k_cg_update(int64_t n, const T *__restrict__ dinv, T *__restrict__ x,
+// The iterate x is FP64 in both solves (TX = double): an FP32 iterate
+// absorbs the late, small corrections (x + alpha p rounds back to x), and the
+// mixed solve then needed 69 % more iterations than FP64 PCG on heat Q4
+// 1024^2 against 9 % with the FP64 iterate (numpy emulation of this code).
+template <typename T, typename TX>
+__global__ __launch_bounds__(SV_BLOCK) void k_cg_update(int64_t n, const T *__restrict__ dinv, TX *__restrict__ x,
                                                         T *__restrict__ r, T *__restrict__ z,
                                                         const T *__restrict__ p, const T *__restrict__ q,
                                                         const double *__restrict__ part_pq,
@@ -268,10 +274,12 @@ __global__ __launch_bounds__(SV_BLOCK) void k_cg_update(int64_t n, const T *__re
     const int G = gridDim.x;
     const double pq = sum_partials(part_pq, G, sh);
     const double rz = sum_partials(rz_old, G, sh);
-    const T alpha = (T)(pq != 0.0 ? rz / pq : 0.0);
+    const double alpha64 = pq != 0.0 ? rz / pq : 0.0;
+    const T alpha = (T)alpha64;
+    const TX alphax = (TX)alpha64;
     double a = 0.0, b = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * SV_BLOCK + threadIdx.x; i < n; i += (int64_t)G * SV_BLOCK) {
-        x[i] += alpha * p[i];
+        x[i] += alphax * (TX)p[i];
         const T ri = r[i] - alpha * q[i];
         const T zi = dinv[i] * ri;
         r[i] = ri;
@@ -341,15 +349,6 @@ __global__ void k_reduce(const double *__restrict__ p, int g, double *__restrict
     }
 }
 
-// x64 += (double) x32, x32 = 0 (FP32 iterate flushed into the FP64 solution)
-__global__ void k_flush(int64_t n, float *__restrict__ x32, double *__restrict__ x64)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        x64[i] += (double)x32[i];
-        x32[i] = 0.f;
-    }
-}
 
 // r = b - A x over the free block (SELL, FP64); partial ||r||^2
 template <typename C>
@@ -417,7 +416,8 @@ __global__ void k_fill(int64_t n, T *__restrict__ a, T v)
 
 template <typename T>
 struct Vecs {
-    T *x, *r, *z, *p, *q, *dinv;
+    double *x;  // the iterate: FP64 in both solves
+    T *r, *z, *p, *q, *dinv;
 };
 
 struct Sell {
@@ -453,7 +453,7 @@ void cg_step(hipStream_t st, int64_t n, const Sell &S, const Vecs<T> &v, const P
     else
         hipLaunchKernelGGL((k_cg_spmv<T, int32_t>), g, b, 0, st, n, S.off, (const int32_t *)S.col, sell_vals<T>(S), S.U,
                            v.p, v.q, P.pq);
-    hipLaunchKernelGGL((k_cg_update<T>), g, b, 0, st, n, v.dinv, v.x, v.r, v.z, v.p, v.q, P.pq, P.rz[par ^ 1],
+    hipLaunchKernelGGL((k_cg_update<T, double>), g, b, 0, st, n, v.dinv, v.x, v.r, v.z, v.p, v.q, P.pq, P.rz[par ^ 1],
                        P.rz[par], P.rr);
     hipLaunchKernelGGL((k_cg_dir<T>), g, b, 0, st, n, P.rz[par], P.rz[par ^ 1], P.rr, v.z, v.p, sc);
 }
@@ -484,10 +484,10 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
     // ---- scratch layout (8-byte words first, then 4-byte) -----------------
     // doubles: xfull[N], b[n], r64[n], dinv64[n], parts[5 G], sc[S_NS],
     //          FP64 vectors r z p q [4n] (FP64 solve only)
-    // floats:  x r z p q dinv [6n] (mixed solve only)
+    // floats:  r z p q dinv [5n] (mixed solve only; its iterate is the FP64 xfull)
     // uint32:  off[nslices + 1], len[max(n, nslices)], total[4]
     const size_t nd = (size_t)N + 3 * (size_t)n + 5 * (size_t)G + S_NS + (mixed ? 0 : 4 * (size_t)n);
-    const size_t nf = mixed ? 6 * (size_t)n : 0;
+    const size_t nf = mixed ? 5 * (size_t)n : 0;
     const size_t nu32 = (size_t)nslices + 1 + (size_t)std::max<int64_t>(n, nslices) + (size_t)n + 4;
     int rc = ensure(c, c->pcg, 8 * nd + 4 * nf + 4 * nu32 + 256);
     if (rc) return rc;
@@ -509,7 +509,8 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
     float *fp = (float *)dp;
     Vecs<float> v32{};
     if (mixed) {
-        v32.x = fp; fp += n; v32.r = fp; fp += n; v32.z = fp; fp += n; v32.p = fp; fp += n; v32.q = fp; fp += n;
+        v32.x = xfull;  // the free part of the full vector is the iterate (FP64)
+        v32.r = fp; fp += n; v32.z = fp; fp += n; v32.p = fp; fp += n; v32.q = fp; fp += n;
         v32.dinv = fp; fp += n;
     }
     uint32_t *up = (uint32_t *)fp;
@@ -635,7 +636,6 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
             }
         }
     } else {
-        hipLaunchKernelGGL(k_fill<float>, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, v32.x, 0.f);
         reset(v32, 1);
         double rlast = rn;
         while (rn > target && it < maxit) {
@@ -649,8 +649,7 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
                 HIPCHK(c, hipGetLastError());
                 it += CHECK;
             }
-            // reliable update: FP32 iterate into the FP64 solution, FP64 residual replaces the FP32 one
-            hipLaunchKernelGGL(k_flush, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, v32.x, xfull);
+            // reliable update: the FP64 residual b - A x replaces the FP32 one
             if ((rc = true_residual(&rn))) return rc;
             nupd++;
             rlast = rn;
