@@ -214,8 +214,8 @@ int elfel_hip_geometry_stats(elfel_hip_ctx *ctx, double *min_det, int64_t *n_bad
  * K[1:nu,1:nu] \ (F - K*T)[1:nu]) for SPD problems with Jacobi-PCG on the
  * matrix of the last assembly (CSR).  x holds the data values on entry
  * (x[nfree:]) and the solution on return.  precision = ELFEL_FP64: FP64 PCG;
- * ELFEL_MIXED: FP32 PCG iterations with reliable updates (the FP32 iterate is
- * added into the FP64 solution and the FP64 residual replaces the FP32 one
+ * ELFEL_MIXED: PCG with an FP32 matrix copy and FP32 r, z, p, q, the iterate
+ * kept in FP64, and reliable updates (the FP64 residual replaces the FP32 one
  * at every 100x drop).  Convergence: ||b - A x|| / ||b|| <= tol in FP64 on
  * the free block.  *iters = PCG iterations, *outer = FP64 residual
  * replacements, *relres = the final FP64 relative residual. */
