@@ -26,7 +26,9 @@ Other workloads, each printing its own line:
   c4  256 heat-T3 / elasticity-T6 problems of uneven size in one persistent
       launch replayed from a hipGraph (mixed), against the FP64 plain method
       problem by problem.
-Element families (the plain FP64 method, the only one they run on):
+Element families (the REUSE method with a cached sort permutation, mixed:
+FP32 element values, FP64 sums; beside it the FP64 REUSE method, bit for bit
+the plain method, and the plain FP64 method):
   t4  heat on a 64^3 block of T4 tetrahedra (examples/heat/poisson/t4.jl);
   p1b Reddy-form Stokes, T3 + bubble velocity / T3 pressure, 512 x 512
       (examples/stokes/colliding_flow/p1b_p1.jl).
@@ -699,7 +701,7 @@ def cpu_baseline_batch(probs, sample):
 
 
 # ---------------------------------------------------------------------------
-# element families (plain FP64 method)
+# element families (REUSE method with a cached sort permutation; plain FP64)
 # ---------------------------------------------------------------------------
 def family_case(cfgname, N):
     """Mesh and numbering of a family workload (host arrays)."""
@@ -713,7 +715,8 @@ def family_case(cfgname, N):
                              f"load vector",
                     kernel="k_fam_scalar<HEAT, 3D, T4, 1 point>",
                     kernel_in=E * 16 + Nn * 24 + Nn * 4, per_el_out=16 * 16 + 4 * 16,
-                    mesh_bytes=E * 16 + Nn * 24 + Nn * 4, heat=True)
+                    mesh_bytes=E * 16 + Nn * 24 + Nn * 4, heat=True,
+                    geom_in=E * 16 + Nn * 24, entries=16, fentries=4)
     N = N or 512
     m = M.transform(M.T3block(2.0, 2.0, N, N), lambda x: x - 1.0)
     E, Nn = m.nelements, m.nnodes
@@ -724,20 +727,24 @@ def family_case(cfgname, N):
                 workload=f"Reddy Stokes P1b/P1 {N}x{N} (T3 + bubble velocity, T3 pressure), 3-pt rule, CSR",
                 kernel="k_fam_reddy<T3B, T3, 3 points>",
                 kernel_in=E * 24 + Nn * 16 + Nn * 12 + E * 8, per_el_out=(4 * 16 + 4 * 12) * 16,
-                mesh_bytes=E * 24 + Nn * 16 + Nn * 12 + E * 8, heat=False)
+                mesh_bytes=E * 24 + Nn * 16 + Nn * 12 + E * 8, heat=False,
+                geom_in=E * 12 + Nn * 16, entries=4 * 16 + 4 * 12, fentries=0)
 
 
 def family_upload(ctx, L, cs):
+    """Uploads the workload; returns assemble(method, precision, wait)."""
     if "dn" in cs:
         ctx.set_mesh(0, cs["conn"], cs["x"])
         ctx.set_dofs(0, cs["dn"], cs["n"])
-        return lambda wait=True: ctx.assemble(L.HEAT, L.T4, 1, layout=L.CSR, kappa=1.0, Q=-6.0, wait=wait)
+        return lambda m=L.SORT, p=L.FP64, wait=True: ctx.assemble(L.HEAT, L.T4, 1, precision=p, method=m,
+                                                                  layout=L.CSR, kappa=1.0, Q=-6.0, wait=wait)
     ctx.set_mesh(0, cs["conn"], cs["x"])
     ctx.set_dofs(0, np.stack([cs["dux"], cs["duy"]], 1), cs["n"])
     ctx.set_element_dofs(0, np.stack([cs["eux"], cs["euy"]], 1), cs["n"])
     ctx.set_mesh(1, cs["conn"], cs["x"])
     ctx.set_dofs(1, cs["dp"], cs["n"])
-    return lambda wait=True: ctx.assemble(L.STOKES_REDDY, L.T3B, 3, layout=L.CSR, mu=1.0, fe_p=L.T3, wait=wait)
+    return lambda m=L.SORT, p=L.FP64, wait=True: ctx.assemble(L.STOKES_REDDY, L.T3B, 3, precision=p, method=m,
+                                                              layout=L.CSR, mu=1.0, fe_p=L.T3, wait=wait)
 
 
 def family_oracle(O, cs):
@@ -760,31 +767,58 @@ def bench_family(args, L):
     t_setup = time.time() - t0
     progress(f"{args.config}: mesh, numbering and upload ready ({t_setup:.1f} s)")
     E, Nn, n = cs["E"], cs["Nn"], cs["n"]
+
+    def timed(m, p, steps):
+        for _ in range(args.warmup):
+            assemble(m, p, False)
+        ctx.synchronize()
+        w0 = time.perf_counter()
+        for _ in range(steps):
+            assemble(m, p, False)
+        ctx.synchronize()
+        ms = (time.perf_counter() - w0) * 1e3 / steps
+        ctx.timing(True)
+        for _ in range(steps):
+            assemble(m, p, False)
+        ctx.synchronize()
+        ph = ctx.timing_read()
+        ctx.timing(False)
+        return ms, {k: (ph[k][0] / ph[k][1]) if ph[k][1] else None for k in ("element", "sort", "reduce", "total")}
+
+    # the plain FP64 method, every step
     nnz = assemble()
     ctx.synchronize()
-    ramp = clock_ramp(lambda: assemble(False), ctx.synchronize)
-    for _ in range(args.warmup):
-        assemble(False)
+    ptr0, ind0, val0 = ctx.copy_sparse(base=0)[:3]
+    ramp = clock_ramp(lambda: assemble(L.SORT, L.FP64, False), ctx.synchronize)
+    ms_plain, ph_plain = timed(L.SORT, L.FP64, min(args.steps, 50))
+    progress("fp64 plain timed")
+    # first REUSE call on this mesh: the symbolic phase (one plain-method pass
+    # that sorts stream positions) and the numeric phase
     ctx.synchronize()
     w0 = time.perf_counter()
-    for _ in range(args.steps):
-        assemble(False)
-    ctx.synchronize()
-    ms = (time.perf_counter() - w0) * 1e3 / args.steps
-    ctx.timing(True)
-    for _ in range(args.steps):
-        assemble(False)
-    ctx.synchronize()
-    ph = ctx.timing_read()
-    ctx.timing(False)
-    progress("fp64 plain timed")
-
-    def per(name):
-        return (ph[name][0] / ph[name][1]) if ph[name][1] else None
-    elem_ms = per("element")
-    kernel_bytes = cs["kernel_in"] + E * cs["per_el_out"]
-    achieved = kernel_bytes / (elem_ms * 1e-3) / 1e9
-    full_bytes = cs["mesh_bytes"] + nnz * 12 + (n + 1) * 4 + (n * 8 if cs["heat"] else 0)
+    assemble(L.REUSE, L.MIXED)
+    first_ms = (time.perf_counter() - w0) * 1e3
+    ms_mixed, ph_mixed = timed(L.REUSE, L.MIXED, args.steps)
+    ptr, ind, valm = ctx.copy_sparse(base=0)[:3]
+    same_pattern = bool(np.array_equal(ptr, ptr0) and np.array_equal(ind, ind0))
+    err = float(np.linalg.norm(valm - val0) / max(np.linalg.norm(val0), 1e-300))
+    ms_r64, ph_r64 = timed(L.REUSE, L.FP64, args.steps)
+    val64 = ctx.copy_sparse(base=0)[2]
+    reuse_bitwise = bool(np.array_equal(val64, val0))
+    progress("reuse timed")
+    # the mixed step's two kernels: element values (geometry and the sorted
+    # position of each value in, FP32 values out at those positions) and the
+    # runs (sorted values and run starts in, FP64 matrix and load vector out)
+    nent, nf = E * cs["entries"], E * cs["fentries"]
+    el_bytes = cs["geom_in"] + 4 * (nent + nf) + 4 * (nent + nf)
+    runs_bytes = 4 * nent + 4 * (nnz + 1) + 8 * nnz
+    if cs["heat"]:
+        runs_bytes += 4 * nf + 4 * (n + 1) + 8 * n
+    el_ms, runs_ms = ph_mixed["element"], ph_mixed["reduce"]
+    dom = "element" if el_ms >= runs_ms else "runs"
+    kernel_bytes, kernel_ms = (el_bytes, el_ms) if dom == "element" else (runs_bytes, runs_ms)
+    kname = (cs["kernel"] + " (values only, FP32 out)") if dom == "element" else "k_fam_runs<float>"
+    achieved = kernel_bytes / (kernel_ms * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -805,27 +839,37 @@ def bench_family(args, L):
             cpu = {"error": f"oracle unavailable: {e}"}
     line = {
         "metric": METRIC,
-        "value": E / (ms * 1e-3),
+        "value": E / (ms_mixed * 1e-3),
         "unit": "elements/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": ms_mixed,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64 element arithmetic, f32 stored element values, f64 sums",
         "data": "synthetic structured mesh, inputs resident in HBM",
         "config": {"workload": cs["workload"], "elements": E, "nodes": Nn, "dofs": n, "nnz": nnz, "layout": "CSR",
-                   "method": "plain FP64 (element kernel -> COO in stream order -> radix sort -> segmented reduce)"},
+                   "method": "REUSE: element kernel (values only) -> runs summed through the cached sort permutation"},
         "roofline": dict({"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": cs["kernel"],
-                          "kernel_ms": elem_ms, "algorithmic_bytes": kernel_bytes},
+                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kname,
+                          "kernel_ms": kernel_ms, "algorithmic_bytes": kernel_bytes},
                          **measured_traffic(args.config, args.N, cs["N"])),
         "cpu_baseline": cpu,
-        "phases_ms": {k: per(k) for k in ("element", "sort", "reduce", "total")},
-        "full_assembly_bytes": full_bytes,
-        "full_assembly_gbs": full_bytes / (ms * 1e-3) / 1e9,
+        "phases_ms": ph_mixed,
+        "kernels": {"element_values": {"ms": el_ms, "bytes": el_bytes, "gbs": el_bytes / (el_ms * 1e-3) / 1e9},
+                    "runs": {"ms": runs_ms, "bytes": runs_bytes, "gbs": runs_bytes / (runs_ms * 1e-3) / 1e9}},
+        "mixed_vs_fp64_frobenius": err,
+        "same_pattern_as_plain": same_pattern,
+        "fp64_reuse": {"ms_per_step": ms_r64, "elements_per_s": E / (ms_r64 * 1e-3), "phases_ms": ph_r64,
+                       "equals_fp64_plain_bitwise": reuse_bitwise},
+        "fp64_plain": {"ms_per_step": ms_plain, "elements_per_s": E / (ms_plain * 1e-3), "phases_ms": ph_plain,
+                       "kernel": cs["kernel"]},
+        "speedup_mixed_vs_fp64_plain": ms_plain / ms_mixed,
+        "speedup_mixed_vs_fp64_reuse": ms_r64 / ms_mixed,
+        "first_call": {"ms": first_ms, "elements_per_s": E / (first_ms * 1e-3),
+                       "fp64_plain_ms": ms_plain},
         "clock_ramp_steps": ramp,
         "setup": {"host_mesh_numbering_upload_s": t_setup},
         "device": {"name": devname, "cus": ncu},

@@ -208,7 +208,9 @@ extern "C" int elfel_hip_free(elfel_hip_ctx *c)
                    &c->fvals[1], &c->stage64, &c->segpos, &c->ptr, &c->ind, &c->val, &c->F, &c->dscal, &c->pcg, &c->pcg_sell,
                    &c->pat.ptr, &c->pat.ind, &c->pat.tile, &c->pat.tile_rows, &c->pat.task_elem,
                    &c->pat.task_info, &c->pat.task_pos, &c->pat.tile_elem, &c->pat.conn_pad, &c->pat.xy_pad,
-                   &c->pat.truns, &c->pat.runs, &c->pat.rrec, &c->pat.tconn, &c->pat.txy, &c->pat.sym, &c->pat.sym2};
+                   &c->pat.truns, &c->pat.runs, &c->pat.rrec, &c->pat.tconn, &c->pat.txy, &c->pat.sym, &c->pat.sym2,
+                   &c->fpat.ptr, &c->fpat.ind, &c->fpat.inv, &c->fpat.seg, &c->fpat.finv, &c->fpat.fbeg,
+                   &c->fpat.coo, &c->fpat.fcoo};
     for (Buf *b : bufs) release(*b);
     batch_free(c);
     for (auto &p : c->tm.pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
@@ -288,7 +290,7 @@ extern "C" int elfel_hip_sparse(elfel_hip_ctx *c, int64_t nrow, int64_t ncol, in
     c->nnz = h;
     c->nrow = nrow; c->ncol = ncol; c->nmaj = nmaj; c->nmin = nmin; c->layout = layout;
     c->has_result = true;
-    c->result_is_pattern = false;
+    c->result_src = RES_PLAIN;
     c->used_method = ELFEL_SORT;
     c->used_precision = ELFEL_FP64;
     c->used_note.clear();
@@ -333,8 +335,8 @@ extern "C" int elfel_hip_copy_sparse(elfel_hip_ctx *c, int base, int64_t *ptr, i
     int rc = fetch_nnz(c);
     if (rc) return rc;
     const int64_t nnz = c->nnz, np = c->nmaj + 1;
-    const int32_t *dptr = (const int32_t *)(c->result_is_pattern ? c->pat.ptr.p : c->ptr.p);
-    const int32_t *dind = (const int32_t *)(c->result_is_pattern ? c->pat.ind.p : c->ind.p);
+    const int32_t *dptr = result_ptr(c);
+    const int32_t *dind = result_ind(c);
     if ((rc = ensure(c, c->stage64, 8 * std::max(np, nnz)))) return rc;
     int64_t *st = (int64_t *)c->stage64.p;
     if (ptr) {
@@ -442,6 +444,7 @@ extern "C" int elfel_hip_set_element_dofs(elfel_hip_ctx *c, int field, int32_t n
     if (flags & DERR_RANGE) return set_err(c, ELFEL_ERANGE, "cell dof number outside 1..ndofs (unnumbered dof 0?)");
     f.edof_ncomp = ncomp; f.edof_E = E; f.edof_ndofs = ndofs;
     f.has_edofs = true;
+    f.edof_version++;
     return ELFEL_OK;
 }
 

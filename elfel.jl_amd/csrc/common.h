@@ -67,6 +67,7 @@ struct Field {
     int edof_ncomp = 0;
     int64_t edof_E = 0, edof_ndofs = 0;
     bool has_edofs = false;
+    uint64_t edof_version = 0;
     Buf edof;
     std::vector<int32_t> h_conn, h_dof;  // host copies (0-based) for the symbolic phase
     uint64_t mesh_version = 0, dof_version = 0;
@@ -119,6 +120,23 @@ struct Pattern {
     int rw = 0, tstride = 0;
 };
 
+// Pattern cached for the element families on the REUSE method (families.hip):
+// the stable sort of the COO stream done once, kept as a permutation -- for
+// each stream entry, its position in the sorted stream -- and the start of
+// each matrix entry's run in sorted order (the load vector likewise, one run
+// per dof).  Built once per mesh + dof map + weak form.
+struct FamPattern {
+    bool valid = false;
+    int kind = -1, fe = -1, fe_p = -1, layout = -1;
+    uint64_t key[6] = {0, 0, 0, 0, 0, 0};  // mesh, dof and cell-dof versions of fields 0 and 1
+    int64_t n = 0, nv = 0, nnz = 0, ndofs = 0;
+    double symbolic_ms = 0;
+    Buf ptr, ind;     // CSR / CSC pattern (int32)
+    Buf inv, seg;     // int32 [n] sorted position per stream entry; int32 [nnz + 1] run starts
+    Buf finv, fbeg;   // load vector: int32 [nv]; int32 [ndofs + 1]
+    Buf coo, fcoo;    // the element values of one assembly, in sorted order (FP64, or FP32 for MIXED)
+};
+
 struct Timing {
     bool on = false;
     // region mode: one event pair around everything submitted between
@@ -154,10 +172,11 @@ struct elfel_hip_ctx {
     int64_t nmaj = 0, nmin = 0, nnz = -1, nrow = 0, ncol = 0, nF = 0;
     int layout = 0;
     bool has_result = false, has_F = false;
-    bool result_is_pattern = false;   // ptr/ind live in the cached pattern
+    int result_src = 0;   // where ptr/ind of the result live: RES_PLAIN, RES_PATTERN, RES_FAMILY
     int used_method = -1, used_precision = -1;  // what the last assembly ran (elfel_hip_assembly_info)
     std::string used_note;
     Pattern pat;
+    FamPattern fpat;
     // device scalars: [0] nnz (int32), [1] error flags; int64 counts of badly shaped elements (byte 24) and
     // of elements with det J <= 0 (byte 32)
     Buf dscal;
@@ -167,6 +186,19 @@ struct elfel_hip_ctx {
     // batched problems (batch.hip)
     void *batch = nullptr;
 };
+
+enum { RES_PLAIN = 0, RES_PATTERN = 1, RES_FAMILY = 2 };
+// The device ptr / ind of the last assembly's matrix.
+inline const int32_t *result_ptr(const elfel_hip_ctx *c)
+{
+    return (const int32_t *)(c->result_src == RES_PATTERN ? c->pat.ptr.p
+                             : c->result_src == RES_FAMILY ? c->fpat.ptr.p : c->ptr.p);
+}
+inline const int32_t *result_ind(const elfel_hip_ctx *c)
+{
+    return (const int32_t *)(c->result_src == RES_PATTERN ? c->pat.ind.p
+                             : c->result_src == RES_FAMILY ? c->fpat.ind.p : c->ind.p);
+}
 
 // ---- error handling -------------------------------------------------------
 int set_err(elfel_hip_ctx *c, int code, const std::string &msg);

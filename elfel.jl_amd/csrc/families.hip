@@ -1,6 +1,8 @@
 // families.hip -- the element families beyond T3/T6/Q4 and the weak forms
 // that use them, on the plain FP64 method (COO in Elfel's stream order, then
-// the stable radix sort and in-order segmented reduce of plain.hip).
+// the stable radix sort and in-order segmented reduce of plain.hip) and on the
+// REUSE method (that sort done once and kept as a permutation; each assembly
+// sums the runs through it in the same order).
 // Compiled with -ffp-contract=off: every product and sum rounds as the
 // reference's unfused Julia arithmetic does.
 //
@@ -16,7 +18,10 @@
 // expansion in 3D.  Spatial gradients (src/QPIterators.jl:132-140, g / J):
 // g / J11 in 1D, else the row vector g times the adjugate of J over det J,
 // each component summed in index order.
+#include <string.h>
+
 #include <algorithm>
+#include <chrono>
 
 #include "common.h"
 
@@ -221,23 +226,31 @@ __device__ __forceinline__ uint64_t coo_key(int32_t r, int32_t col, int minor_bi
 // heat: ke[i,j] += dot(gradN_i, gradN_j) * (kappa JxW), fe[j] += N_j Q JxW
 //       (examples/heat/poisson/t4.jl:33-55, the loop of test/test_heat.jl:36-60);
 // mass: me[i,j] += (N_i N_j) * (kappa JxW) (examples/klein_gordon/klein_gordon.jl:34-51).
-template <int KIND, int M, int NB, int NQ>
+//
+// VT: the stored value type (double; float for the REUSE method's MIXED
+// precision, the FP64 result rounded once).  keys / fkeys null: values only
+// (the REUSE method's numeric phase, whose sort order is cached), each stored
+// at vidx[o] / fvidx[o] -- its position in the sorted stream -- instead of at
+// its stream position o.
+template <int KIND, int M, int NB, int NQ, typename VT>
 __global__ __launch_bounds__(256) void k_fam_scalar(int64_t E, const int32_t *__restrict__ conn,
                                                     const double *__restrict__ x, const int32_t *__restrict__ dof,
                                                     FamTab tab, double kappa, double Q, int minor_bits, int csc,
-                                                    uint64_t *__restrict__ keys, double *__restrict__ vals,
-                                                    uint64_t *__restrict__ fkeys, double *__restrict__ fvals)
+                                                    uint64_t *__restrict__ keys, VT *__restrict__ vals,
+                                                    uint64_t *__restrict__ fkeys, VT *__restrict__ fvals,
+                                                    const int32_t *__restrict__ vidx, const int32_t *__restrict__ fvidx)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     double X[NB * M];
     int32_t d[NB];
+    const bool want_keys = keys || fkeys;
 #pragma unroll
     for (int k = 0; k < NB; k++) {
         const int32_t n = conn[e * NB + k];
 #pragma unroll
         for (int s = 0; s < M; s++) X[k * M + s] = x[(int64_t)n * M + s];
-        d[k] = dof[n];
+        d[k] = want_keys ? dof[n] : 0;
     }
     double ke[NB * NB], fe[NB];
 #pragma unroll
@@ -275,15 +288,16 @@ __global__ __launch_bounds__(256) void k_fam_scalar(int64_t E, const int32_t *__
     for (int j = 0; j < NB; j++)
 #pragma unroll
         for (int i = 0; i < NB; i++) {
-            keys[o] = coo_key(d[i], d[j], minor_bits, csc);
-            vals[o] = ke[i + NB * j];
+            if (keys) keys[o] = coo_key(d[i], d[j], minor_bits, csc);
+            vals[vidx ? (int64_t)vidx[o] : o] = (VT)ke[i + NB * j];
             o++;
         }
     if constexpr (KIND == ELFEL_HEAT) {
 #pragma unroll
         for (int j = 0; j < NB; j++) {
-            fkeys[e * NB + j] = (uint64_t)d[j];
-            fvals[e * NB + j] = fe[j];
+            if (fkeys) fkeys[e * NB + j] = (uint64_t)d[j];
+            const int64_t fo = e * NB + j;
+            fvals[fvidx ? (int64_t)fvidx[fo] : fo] = (VT)fe[j];
         }
     }
 }
@@ -299,7 +313,7 @@ __global__ __launch_bounds__(256) void k_fam_scalar(int64_t E, const int32_t *__
 // transpose row-major of its parent.  The spatial gradients and JxW of every
 // point are formed once; each block is then summed over the points in order
 // and written before the next, so that one block at a time is live.
-template <int VFE, int PFE, int NQ>
+template <int VFE, int PFE, int NQ, typename VT>
 __global__ __launch_bounds__(256) void k_fam_reddy(int64_t E, const int32_t *__restrict__ conn_v,
                                                    const double2 *__restrict__ xy_v,
                                                    const int32_t *__restrict__ dof_v,
@@ -309,7 +323,7 @@ __global__ __launch_bounds__(256) void k_fam_reddy(int64_t E, const int32_t *__r
                                                    const int32_t *__restrict__ dof_p,
                                                    const int32_t *__restrict__ edof_p, FamTab tab, double mu,
                                                    int minor_bits, int csc, uint64_t *__restrict__ keys,
-                                                   double *__restrict__ vals)
+                                                   VT *__restrict__ vals, const int32_t *__restrict__ vidx)
 {
     constexpr int NU = fam_nbf(VFE), NV = fam_ngeo(VFE), NP = fam_nbf(PFE);
     constexpr bool PNODAL = PFE == ELFEL_T3;
@@ -335,27 +349,34 @@ __global__ __launch_bounds__(256) void k_fam_reddy(int64_t E, const int32_t *__r
             gx[q][k] = g[0]; gy[q][k] = g[1];
         }
     }
+    // dof numbers (for the keys only)
     int32_t dx[NU], dy[NU], dp[NP];
 #pragma unroll
-    for (int k = 0; k < NV; k++) {
-        const int32_t n = conn_v[e * NV + k];
-        dx[k] = dof_v[2 * (int64_t)n];
-        dy[k] = dof_v[2 * (int64_t)n + 1];
-    }
-    if constexpr (NU > NV) {  // the bubble's cell dofs
-        dx[NV] = edof_v[2 * e];
-        dy[NV] = edof_v[2 * e + 1];
-    }
-    if constexpr (PNODAL) {
+    for (int k = 0; k < NU; k++) dx[k] = dy[k] = 0;
 #pragma unroll
-        for (int k = 0; k < 3; k++) dp[k] = dof_p[conn_p[e * 3 + k]];
-    } else {
-        dp[0] = edof_p[e];
+    for (int k = 0; k < NP; k++) dp[k] = 0;
+    if (keys) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            const int32_t n = conn_v[e * NV + k];
+            dx[k] = dof_v[2 * (int64_t)n];
+            dy[k] = dof_v[2 * (int64_t)n + 1];
+        }
+        if constexpr (NU > NV) {  // the bubble's cell dofs
+            dx[NV] = edof_v[2 * e];
+            dy[NV] = edof_v[2 * e + 1];
+        }
+        if constexpr (PNODAL) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) dp[k] = dof_p[conn_p[e * 3 + k]];
+        } else {
+            dp[0] = edof_p[e];
+        }
     }
     int64_t o = e * (4 * NU * NU + 4 * NU * NP);
     auto put = [&](int32_t r, int32_t col, double v) {
-        keys[o] = coo_key(r, col, minor_bits, csc);
-        vals[o] = v;
+        if (keys) keys[o] = coo_key(r, col, minor_bits, csc);
+        vals[vidx ? (int64_t)vidx[o] : o] = (VT)v;
         o++;
     };
     double k[NU * NU];
@@ -433,42 +454,283 @@ bool reddy_pair(int vfe, int pfe)
            (vfe == ELFEL_Q4 && pfe == ELFEL_Q4L2);
 }
 
-template <int KIND, int M, int NB, int NQ>
-int launch_scalar(elfel_hip_ctx *c, const FamTab &tab, double kappa, double Q, int minor_bits, int csc)
+// The element kernel's outputs: keys (or null) and values of the COO stream
+// and, for heat, of the load-vector stream.
+struct FamOut {
+    uint64_t *keys, *fkeys;
+    void *vals, *fvals;
+    const int32_t *vidx, *fvidx;  // sorted position of each stream entry (null: stream order)
+};
+
+template <int KIND, int M, int NB, int NQ, typename VT>
+int launch_scalar(elfel_hip_ctx *c, const FamTab &tab, double kappa, double Q, int minor_bits, int csc,
+                  const FamOut &o)
 {
     const Field &F = c->f[0];
     const bool heat = KIND == ELFEL_HEAT;
     hipEvent_t ev = nullptr;
     tm_begin(c, ELFEL_PH_ELEMENT, &ev);
     if (F.E > 0)
-        hipLaunchKernelGGL((k_fam_scalar<KIND, M, NB, NQ>), dim3(ceil_div(F.E, 256)), dim3(256), 0, c->stream, F.E,
+        hipLaunchKernelGGL((k_fam_scalar<KIND, M, NB, NQ, VT>), dim3(ceil_div(F.E, 256)), dim3(256), 0, c->stream, F.E,
                            (const int32_t *)F.conn.p, (const double *)F.xy.p, (const int32_t *)F.dof.p, tab, kappa, Q,
-                           minor_bits, csc, (uint64_t *)c->keys[0].p, (double *)c->vals[0].p,
-                           heat ? (uint64_t *)c->fkeys[0].p : nullptr, heat ? (double *)c->fvals[0].p : nullptr);
+                           minor_bits, csc, o.keys, (VT *)o.vals, heat ? o.fkeys : nullptr,
+                           heat ? (VT *)o.fvals : nullptr, o.vidx, heat ? o.fvidx : nullptr);
     HIPCHK(c, hipGetLastError());
     tm_end(c, ELFEL_PH_ELEMENT, ev);
     return 0;
 }
 
-template <int VFE, int PFE, int NQ>
-int launch_reddy(elfel_hip_ctx *c, const FamTab &tab, double mu, int minor_bits, int csc)
+template <int VFE, int PFE, int NQ, typename VT>
+int launch_reddy(elfel_hip_ctx *c, const FamTab &tab, double mu, int minor_bits, int csc, const FamOut &o)
 {
     const Field &V = c->f[0], &P = c->f[1];
     hipEvent_t ev = nullptr;
     tm_begin(c, ELFEL_PH_ELEMENT, &ev);
     if (V.E > 0)
-        hipLaunchKernelGGL((k_fam_reddy<VFE, PFE, NQ>), dim3(ceil_div(V.E, 256)), dim3(256), 0, c->stream, V.E,
+        hipLaunchKernelGGL((k_fam_reddy<VFE, PFE, NQ, VT>), dim3(ceil_div(V.E, 256)), dim3(256), 0, c->stream, V.E,
                            (const int32_t *)V.conn.p, (const double2 *)V.xy.p, (const int32_t *)V.dof.p,
                            VFE == ELFEL_T3B ? (const int32_t *)V.edof.p : nullptr,
                            PFE == ELFEL_T3 ? (const int32_t *)P.conn.p : nullptr,
                            PFE == ELFEL_T3 ? (const double2 *)P.xy.p : nullptr,
                            PFE == ELFEL_T3 ? (const int32_t *)P.dof.p : nullptr,
                            PFE == ELFEL_Q4L2 ? (const int32_t *)P.edof.p : nullptr, tab, mu, minor_bits, csc,
-                           (uint64_t *)c->keys[0].p, (double *)c->vals[0].p);
+                           o.keys, (VT *)o.vals, o.vidx);
     HIPCHK(c, hipGetLastError());
     tm_end(c, ELFEL_PH_ELEMENT, ev);
     return 0;
 }
+
+// ---- the REUSE method on the families ------------------------------------------
+// The plain method's stable sort orders the COO stream by (major, minor) and
+// keeps stream order among equal keys; each matrix entry is then the sum of its
+// run, left to right.  The order depends on the mesh and the dof map only, so
+// the symbolic phase sorts the stream once with each entry's stream position
+// as payload and inverts the result; every assembly after it runs the element
+// kernel, which stores each value at its sorted position (no keys), and sums
+// each run of the sorted values in the same order: FP64 bit for bit the plain
+// method's result.  (Storing in stream order and gathering through the
+// permutation in the sum measured slower: heat T4 64^3 mixed 0.44 ms.)
+__global__ void k_fam_iota(uint64_t *__restrict__ v, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = (uint64_t)i;
+}
+
+// inv[perm[i]] = i: the sorted position of every stream entry
+__global__ void k_fam_invert(const uint64_t *__restrict__ perm, int32_t *__restrict__ inv, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) inv[perm[i]] = (int32_t)i;
+}
+
+// run starts of the matrix entries: the segment heads, then n
+__global__ void k_fam_seg(const uint32_t *__restrict__ hpos, int64_t nnz, int64_t n, int32_t *__restrict__ seg)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= nnz) seg[i] = i < nnz ? (int32_t)hpos[i] : (int32_t)n;
+}
+
+// run start of every dof in the sorted load-vector keys (empty runs for dofs
+// no element touches): beg[d] = first sorted position with key >= d
+__global__ void k_fam_dense_begin(const uint64_t *__restrict__ keys, int64_t nv, int64_t ndofs,
+                                  int32_t *__restrict__ beg)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    const int64_t k = (int64_t)keys[i];
+    const int64_t kp = i == 0 ? -1 : (int64_t)keys[i - 1];
+    for (int64_t d = kp + 1; d <= k; d++) beg[d] = (int32_t)i;
+    if (i == nv - 1)
+        for (int64_t d = k + 1; d <= ndofs; d++) beg[d] = (int32_t)nv;
+}
+
+__global__ void k_fam_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Work-item s < nnz: matrix entry s, its run summed left to right from the
+// first entry (sparse()'s combination of duplicates), in FP64.  Work-item
+// nnz + d (d < ndofs, heat): load-vector entry d the same way (start! zeros
+// it, assemble! adds: src/Assemblers.jl:194-221).
+template <typename VT>
+__global__ __launch_bounds__(256) void k_fam_runs(const int32_t *__restrict__ seg, const VT *__restrict__ sv,
+                                                  int64_t nnz, double *__restrict__ val,
+                                                  const int32_t *__restrict__ fbeg, const VT *__restrict__ fsv,
+                                                  int64_t ndofs, double *__restrict__ F)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nnz) {
+        const int32_t i0 = seg[s], i1 = seg[s + 1];
+        double acc = (double)sv[i0];
+        for (int32_t i = i0 + 1; i < i1; i++) acc += (double)sv[i];
+        val[s] = acc;
+        return;
+    }
+    const int64_t d = s - nnz;
+    if (d >= ndofs) return;
+    const int32_t i0 = fbeg[d], i1 = fbeg[d + 1];
+    if (i0 == i1) {
+        F[d] = 0.0;
+        return;
+    }
+    double acc = (double)fsv[i0];
+    for (int32_t i = i0 + 1; i < i1; i++) acc += (double)fsv[i];
+    F[d] = 0.0 + acc;
+}
+
+// The element kernel of the problem (its weak form, element and rule) with
+// FP64 or FP32 stored values.
+int fam_elements(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &tab, int minor_bits, int csc,
+                 const FamOut &o, bool f32)
+{
+    const int fe = pb->fe;
+    const bool heat = pb->kind == ELFEL_HEAT;
+    const bool reddy = pb->kind == ELFEL_STOKES_REDDY;
+    const int nq = tab.nq;
+    int rc = 1;  // stays 1 when no kernel instance fits
+#define S(K_, M_, NB_, NQ_)                                                                          \
+    rc = f32 ? launch_scalar<K_, M_, NB_, NQ_, float>(c, tab, pb->kappa, pb->Q, minor_bits, csc, o) \
+             : launch_scalar<K_, M_, NB_, NQ_, double>(c, tab, pb->kappa, pb->Q, minor_bits, csc, o)
+#define R(V_, P_, NQ_)                                                                 \
+    rc = f32 ? launch_reddy<V_, P_, NQ_, float>(c, tab, pb->mu, minor_bits, csc, o) \
+             : launch_reddy<V_, P_, NQ_, double>(c, tab, pb->mu, minor_bits, csc, o)
+    if (heat || pb->kind == ELFEL_MASS) {
+        const bool h = heat;
+        if (fe == ELFEL_L2) {
+            if (nq == 1) { if (h) S(ELFEL_HEAT, 1, 2, 1); else S(ELFEL_MASS, 1, 2, 1); }
+            else if (nq == 2) { if (h) S(ELFEL_HEAT, 1, 2, 2); else S(ELFEL_MASS, 1, 2, 2); }
+            else if (nq == 3) { if (h) S(ELFEL_HEAT, 1, 2, 3); else S(ELFEL_MASS, 1, 2, 3); }
+        } else if (fe == ELFEL_T4) {
+            if (nq == 1) { if (h) S(ELFEL_HEAT, 3, 4, 1); else S(ELFEL_MASS, 3, 4, 1); }
+            else if (nq == 4) { if (h) S(ELFEL_HEAT, 3, 4, 4); else S(ELFEL_MASS, 3, 4, 4); }
+        } else if (!h && fe == ELFEL_T3) {
+            if (nq == 1) S(ELFEL_MASS, 2, 3, 1); else if (nq == 3) S(ELFEL_MASS, 2, 3, 3);
+        } else if (!h && fe == ELFEL_T6) {
+            if (nq == 1) S(ELFEL_MASS, 2, 6, 1); else if (nq == 3) S(ELFEL_MASS, 2, 6, 3);
+        } else if (!h && fe == ELFEL_Q4) {
+            if (nq == 1) S(ELFEL_MASS, 2, 4, 1); else if (nq == 4) S(ELFEL_MASS, 2, 4, 4);
+        }
+    } else if (reddy) {
+        if (fe == ELFEL_T6 && nq == 3) R(ELFEL_T6, ELFEL_T3, 3);
+        else if (fe == ELFEL_T3B && nq == 3) R(ELFEL_T3B, ELFEL_T3, 3);
+        else if (fe == ELFEL_T3B && nq == 1) R(ELFEL_T3B, ELFEL_T3, 1);
+        else if (fe == ELFEL_Q4 && nq == 4) R(ELFEL_Q4, ELFEL_Q4L2, 4);
+        else if (fe == ELFEL_Q4 && nq == 1) R(ELFEL_Q4, ELFEL_Q4L2, 1);
+    }
+#undef S
+#undef R
+    if (rc == 1) return set_err(c, ELFEL_ENOTSUP, "unsupported (weak form, element, rule)");
+    return rc;
+}
+
+#define FAM_LAUNCH(kern, n, ...)                                                                           \
+    do {                                                                                                   \
+        if ((n) > 0) {                                                                                     \
+            hipLaunchKernelGGL(kern, dim3(ceil_div((n), 256)), dim3(256), 0, c->stream, __VA_ARGS__);      \
+            HIPCHK(c, hipGetLastError());                                                                  \
+        }                                                                                                  \
+    } while (0)
+
+// Symbolic phase of the REUSE method on the families (c->fpat), when the
+// cached one does not describe this mesh, dof map and weak form.
+int fam_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &tab, int64_t n, int64_t nv,
+                 int64_t ndofs, int minor_bits, int csc, bool heat)
+{
+    FamPattern &P = c->fpat;
+    const Field &F0 = c->f[0], &F1 = c->f[1];
+    const bool reddy = pb->kind == ELFEL_STOKES_REDDY;
+    const int pfe = reddy ? pb->fe_p : -1;
+    const uint64_t key[6] = {F0.mesh_version, F0.dof_version, F0.edof_version, reddy ? F1.mesh_version : 0,
+                             reddy ? F1.dof_version : 0, reddy ? F1.edof_version : 0};
+    if (P.valid && P.kind == pb->kind && P.fe == pb->fe && P.fe_p == pfe && P.layout == pb->layout && P.n == n &&
+        P.nv == nv && P.ndofs == ndofs && memcmp(P.key, key, sizeof key) == 0)
+        return 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    P.valid = false;
+    int rc;
+    if ((rc = plain_buffers(c, n, nv, ndofs, heat)) || (rc = ensure(c, P.coo, 8 * std::max<int64_t>(n, 1))) ||
+        (rc = ensure(c, P.fcoo, 8 * std::max<int64_t>(nv, 1))) || (rc = ensure(c, P.inv, 4 * std::max<int64_t>(n, 1))) ||
+        (rc = ensure(c, P.finv, 4 * std::max<int64_t>(nv, 1))) || (rc = ensure(c, P.fbeg, 4 * (ndofs + 1))) ||
+        (rc = ensure(c, P.ptr, 4 * (ndofs + 1))) || (rc = ensure(c, P.ind, 4 * std::max<int64_t>(n, 1))))
+        return rc;
+    uint64_t *k0 = (uint64_t *)c->keys[0].p, *k1 = (uint64_t *)c->keys[1].p;
+    uint64_t *v0 = (uint64_t *)c->vals[0].p, *v1 = (uint64_t *)c->vals[1].p;
+    uint64_t *fk0 = (uint64_t *)c->fkeys[0].p, *fk1 = (uint64_t *)c->fkeys[1].p;
+    uint64_t *fv0 = (uint64_t *)c->fvals[0].p, *fv1 = (uint64_t *)c->fvals[1].p;
+    // the stream's keys (its values land in coo, unused here)
+    const FamOut o{k0, heat ? fk0 : nullptr, P.coo.p, heat ? P.fcoo.p : nullptr, nullptr, nullptr};
+    if ((rc = fam_elements(c, pb, tab, minor_bits, csc, o, false))) return rc;
+    hipEvent_t ev = nullptr;
+    tm_begin(c, ELFEL_PH_SORT, &ev);
+    FAM_LAUNCH(k_fam_iota, n, v0, n);
+    if ((rc = radix_sort_pairs(c, k0, v0, k1, v1, n, bits_for(ndofs) + minor_bits))) return rc;
+    if (heat) {
+        FAM_LAUNCH(k_fam_iota, nv, fv0, nv);
+        if ((rc = radix_sort_pairs(c, fk0, fv0, fk1, fv1, nv, bits_for(ndofs)))) return rc;
+    }
+    tm_end(c, ELFEL_PH_SORT, ev);
+    if ((rc = build_csr_from_sorted(c, k0, nullptr, n, minor_bits, ndofs, (int32_t *)P.ptr.p, (int32_t *)P.ind.p,
+                                    nullptr, (int32_t *)c->dscal.p, false)))
+        return rc;
+    int32_t hnnz = 0;
+    HIPCHK(c, hipMemcpyAsync(&hnnz, c->dscal.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int64_t nnz = hnnz;
+    if ((rc = ensure(c, P.seg, 4 * (nnz + 1)))) return rc;
+    FAM_LAUNCH(k_fam_invert, n, v0, (int32_t *)P.inv.p, n);
+    if (n > 0) FAM_LAUNCH(k_fam_seg, nnz + 1, (const uint32_t *)c->segpos.p, nnz, n, (int32_t *)P.seg.p);
+    else FAM_LAUNCH(k_fam_fill_i32, (int64_t)1, (int32_t *)P.seg.p, (int64_t)1, 0);
+    if (heat) {
+        FAM_LAUNCH(k_fam_invert, nv, fv0, (int32_t *)P.finv.p, nv);
+        if (nv > 0) FAM_LAUNCH(k_fam_dense_begin, nv, fk0, nv, ndofs, (int32_t *)P.fbeg.p);
+        else FAM_LAUNCH(k_fam_fill_i32, ndofs + 1, (int32_t *)P.fbeg.p, ndofs + 1, 0);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    P.kind = pb->kind; P.fe = pb->fe; P.fe_p = pfe; P.layout = pb->layout;
+    memcpy(P.key, key, sizeof key);
+    P.n = n; P.nv = nv; P.nnz = nnz; P.ndofs = ndofs;
+    P.valid = true;
+    P.symbolic_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+// Numeric phase: element values only, each at its sorted position, then every
+// run summed.
+int fam_reuse(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &tab, int64_t n, int64_t nv,
+              int64_t ndofs, int minor_bits, int csc, bool heat)
+{
+    int rc = fam_symbolic(c, pb, tab, n, nv, ndofs, minor_bits, csc, heat);
+    if (rc) return rc;
+    const FamPattern &P = c->fpat;
+    const bool f32 = pb->precision == ELFEL_MIXED;
+    if ((rc = ensure(c, c->val, 8 * std::max<int64_t>(P.nnz, 1)))) return rc;
+    if (heat && (rc = ensure(c, c->F, 8 * std::max<int64_t>(ndofs, 1)))) return rc;
+    const FamOut o{nullptr, nullptr, P.coo.p, heat ? P.fcoo.p : nullptr, (const int32_t *)P.inv.p,
+                   heat ? (const int32_t *)P.finv.p : nullptr};
+    if ((rc = fam_elements(c, pb, tab, minor_bits, csc, o, f32))) return rc;
+    hipEvent_t ev = nullptr;
+    tm_begin(c, ELFEL_PH_REDUCE, &ev);
+    const int64_t nf = heat ? ndofs : 0, nw = P.nnz + nf;
+    if (f32)
+        FAM_LAUNCH(k_fam_runs<float>, nw, (const int32_t *)P.seg.p, (const float *)P.coo.p, P.nnz, (double *)c->val.p,
+                   (const int32_t *)P.fbeg.p, (const float *)P.fcoo.p, nf, heat ? (double *)c->F.p : nullptr);
+    else
+        FAM_LAUNCH(k_fam_runs<double>, nw, (const int32_t *)P.seg.p, (const double *)P.coo.p, P.nnz,
+                   (double *)c->val.p, (const int32_t *)P.fbeg.p, (const double *)P.fcoo.p, nf,
+                   heat ? (double *)c->F.p : nullptr);
+    tm_end(c, ELFEL_PH_REDUCE, ev);
+    c->nnz = P.nnz;
+    c->nmaj = c->nmin = ndofs;
+    c->layout = pb->layout;
+    c->result_src = RES_FAMILY;
+    c->has_F = heat;
+    c->nF = heat ? ndofs : 0;
+    return 0;
+}
+
+#undef FAM_LAUNCH
 
 }  // namespace
 
@@ -480,8 +742,10 @@ bool family_problem(const elfel_hip_problem *pb)
 
 int family_validate(elfel_hip_ctx *c, const elfel_hip_problem *pb)
 {
-    if (pb->method != ELFEL_SORT || pb->precision != ELFEL_FP64)
-        return set_err(c, ELFEL_ENOTSUP, "element families and the Reddy / mass forms run on the plain FP64 method");
+    if (pb->method != ELFEL_SORT && pb->method != ELFEL_REUSE) return set_err(c, ELFEL_EINVAL, "bad method");
+    if (pb->precision != ELFEL_FP64 && pb->precision != ELFEL_MIXED) return set_err(c, ELFEL_EINVAL, "bad precision");
+    if (pb->method == ELFEL_SORT && pb->precision != ELFEL_FP64)
+        return set_err(c, ELFEL_ENOTSUP, "the sort method is the plain FP64 method");
     if (pb->layout != ELFEL_CSR && pb->layout != ELFEL_CSC) return set_err(c, ELFEL_EINVAL, "bad layout");
     if (c->has_kappa_e || c->has_Q_e || c->has_scale_e)
         return set_err(c, ELFEL_ENOTSUP, "element families take uniform coefficients (clear the per-element ones)");
@@ -543,38 +807,11 @@ int family_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb)
     const int64_t ndofs = c->nrow;
     const int minor_bits = bits_for(ndofs);
     const int csc = pb->layout == ELFEL_CSC ? 1 : 0;
+    if (pb->method == ELFEL_REUSE) return fam_reuse(c, pb, tab, n, nv, ndofs, minor_bits, csc, heat);
     int rc = plain_buffers(c, n, nv, ndofs, heat);
     if (rc) return rc;
-    const int nq = tab.nq;
-    rc = 1;  // stays 1 when no kernel instance fits
-#define S(K_, M_, NB_, NQ_) rc = launch_scalar<K_, M_, NB_, NQ_>(c, tab, pb->kappa, pb->Q, minor_bits, csc)
-#define R(V_, P_, NQ_) rc = launch_reddy<V_, P_, NQ_>(c, tab, pb->mu, minor_bits, csc)
-    if (heat || pb->kind == ELFEL_MASS) {
-        const bool h = heat;
-        if (fe == ELFEL_L2) {
-            if (nq == 1) { if (h) S(ELFEL_HEAT, 1, 2, 1); else S(ELFEL_MASS, 1, 2, 1); }
-            else if (nq == 2) { if (h) S(ELFEL_HEAT, 1, 2, 2); else S(ELFEL_MASS, 1, 2, 2); }
-            else if (nq == 3) { if (h) S(ELFEL_HEAT, 1, 2, 3); else S(ELFEL_MASS, 1, 2, 3); }
-        } else if (fe == ELFEL_T4) {
-            if (nq == 1) { if (h) S(ELFEL_HEAT, 3, 4, 1); else S(ELFEL_MASS, 3, 4, 1); }
-            else if (nq == 4) { if (h) S(ELFEL_HEAT, 3, 4, 4); else S(ELFEL_MASS, 3, 4, 4); }
-        } else if (!h && fe == ELFEL_T3) {
-            if (nq == 1) S(ELFEL_MASS, 2, 3, 1); else if (nq == 3) S(ELFEL_MASS, 2, 3, 3);
-        } else if (!h && fe == ELFEL_T6) {
-            if (nq == 1) S(ELFEL_MASS, 2, 6, 1); else if (nq == 3) S(ELFEL_MASS, 2, 6, 3);
-        } else if (!h && fe == ELFEL_Q4) {
-            if (nq == 1) S(ELFEL_MASS, 2, 4, 1); else if (nq == 4) S(ELFEL_MASS, 2, 4, 4);
-        }
-    } else if (reddy) {
-        if (fe == ELFEL_T6 && nq == 3) R(ELFEL_T6, ELFEL_T3, 3);
-        else if (fe == ELFEL_T3B && nq == 3) R(ELFEL_T3B, ELFEL_T3, 3);
-        else if (fe == ELFEL_T3B && nq == 1) R(ELFEL_T3B, ELFEL_T3, 1);
-        else if (fe == ELFEL_Q4 && nq == 4) R(ELFEL_Q4, ELFEL_Q4L2, 4);
-        else if (fe == ELFEL_Q4 && nq == 1) R(ELFEL_Q4, ELFEL_Q4L2, 1);
-    }
-#undef S
-#undef R
-    if (rc == 1) return set_err(c, ELFEL_ENOTSUP, "unsupported (weak form, element, rule)");
-    if (rc) return rc;
+    const FamOut o{(uint64_t *)c->keys[0].p, heat ? (uint64_t *)c->fkeys[0].p : nullptr, c->vals[0].p,
+                   heat ? c->fvals[0].p : nullptr, nullptr, nullptr};
+    if ((rc = fam_elements(c, pb, tab, minor_bits, csc, o, false))) return rc;
     return plain_finish(c, pb->layout, n, nv, ndofs, minor_bits, heat);
 }

@@ -289,7 +289,7 @@ int plain_finish(elfel_hip_ctx *c, int layout, int64_t n, int64_t nv, int64_t nd
     c->nmaj = ndofs;
     c->nmin = ndofs;
     c->layout = layout;
-    c->result_is_pattern = false;
+    c->result_src = RES_PLAIN;
     return 0;
 }
 

@@ -385,9 +385,8 @@ int tile_range(const TileInput &I, int64_t r0, int64_t r1, ReuseTables &T, std::
 // order as well no change; three tasks per thread: 3 / 4 / 5 / 6 rows 0.64 /
 // 0.62 / 0.62 / 0.64 ms).  Small meshes keep strips: on heat T3 100^2 (10,201 rows, a
 // launch-latency-bound case) bands measured 6.1 against 4.8 us.
-void band_order(const std::vector<double2> &xy, std::vector<int32_t> &order)
+void band_order(const double2 *xy, size_t nn, std::vector<int32_t> &order)
 {
-    const size_t nn = xy.size();
     double x0 = 0, x1 = 0, y0 = 0, y1 = 0;
     for (size_t i = 0; i < nn; i++) {
         if (i == 0 || xy[i].x < x0) x0 = xy[i].x;
@@ -432,10 +431,18 @@ void band_order(const std::vector<double2> &xy, std::vector<int32_t> &order)
 // boundaries -- a fixed cut, whatever the number of host threads, so the
 // tiles are the same on every machine -- and the ranges are packed in
 // parallel; a tile never spans two ranges.
+//
+// With errp set the builder touches neither the device nor the context (the
+// batch runs one per problem on host threads): the coordinates come from
+// host_xy and an error goes to *errp.
 int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1, int64_t n,
                 const std::vector<int32_t> &hptr, const std::vector<int32_t> &hind, int cap, int ucap,
-                ReuseTables &T, bool patch, bool rows)
+                ReuseTables &T, bool patch, bool rows, const double2 *host_xy, std::string *errp)
 {
+    auto fail = [&](int code, const std::string &m) -> int {
+        if (errp) { *errp = m; return code; }
+        return set_err(c, code, m);
+    };
     const int nen = F0.nen, nc = F0.ncomp, nd = nen * nc;
     const int ncol = nd + (stokes ? 3 : 0);
     TileInput I;
@@ -473,14 +480,14 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     for (int64_t nd0 = 0; nd0 < F0.Nn; nd0++)
         for (int cc = 0; cc < nc; cc++) {
             const int32_t d = F0.h_dof[nd0 * nc + cc];
-            if (inv_node[d] >= 0) return set_err(c, ELFEL_ENOTSUP, "a dof shared by two nodes or components");
+            if (inv_node[d] >= 0) return fail(ELFEL_ENOTSUP, "a dof shared by two nodes or components");
             inv_node[d] = (int32_t)nd0;
             inv_fc[d] = (uint8_t)cc;
         }
     if (stokes)
         for (int64_t m = 0; m < F1->Nn; m++) {
             const int32_t d = F1->h_dof[m];
-            if (inv_node[d] >= 0) return set_err(c, ELFEL_ENOTSUP, "a dof shared by two nodes or components");
+            if (inv_node[d] >= 0) return fail(ELFEL_ENOTSUP, "a dof shared by two nodes or components");
             inv_node[d] = (int32_t)m;
             inv_fc[d] = (uint8_t)(1 << 4);
         }
@@ -491,11 +498,16 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     // HEAT_ROWS_BLOCK rows and as many distinct elements
     I.rw = I.kmax = 0;
     I.hxy = nullptr;
-    std::vector<double2> hxy;  // host copy of the coordinates (tile-local geometry, locality order)
+    // host coordinates (tile-local geometry, locality order): the caller's, or
+    // a copy of the device array made on first use
+    std::vector<double2> hcopy;
+    const double2 *hxy = errp ? host_xy : nullptr;
     auto fetch_xy = [&]() -> int {
-        if (hxy.size() == (size_t)F0.Nn) return 0;
-        hxy.resize((size_t)F0.Nn);
-        if (F0.Nn > 0) HIPCHK(c, hipMemcpy(hxy.data(), F0.xy.p, 16 * (size_t)F0.Nn, hipMemcpyDeviceToHost));
+        if (hxy || F0.Nn == 0) return 0;
+        if (errp) return fail(ELFEL_EINVAL, "internal: host coordinates missing");
+        hcopy.resize((size_t)F0.Nn);
+        HIPCHK(c, hipMemcpy(hcopy.data(), F0.xy.p, 16 * (size_t)F0.Nn, hipMemcpyDeviceToHost));
+        hxy = hcopy.data();
         return 0;
     };
     if (rows && I.staged && (nen == 3 || nen == 4)) {
@@ -514,7 +526,7 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
             if (F0.sdim == 2 && env_int("ELFEL_HIP_TILE_GEOM", 1) != 0) {
                 int rc = fetch_xy();
                 if (rc) return rc;
-                I.hxy = hxy.data();
+                I.hxy = hxy;
             }
         }
     }
@@ -526,7 +538,7 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
         int rc = fetch_xy();
         if (rc) return rc;
         std::vector<int32_t> node_order;
-        band_order(hxy, node_order);
+        band_order(hxy, (size_t)F0.Nn, node_order);
         // the rows in the order of their nodes; rows without a node last
         seq.reserve((size_t)n);
         for (int32_t nd : node_order) {
@@ -568,7 +580,7 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     worker();
     for (auto &th : pool) th.join();
     for (int64_t k = 0; k < nranges; k++)
-        if (prc[k]) return set_err(c, prc[k], perr[k]);
+        if (prc[k]) return fail(prc[k], perr[k]);
     // concatenation: tile task offsets become global
     T = ReuseTables{};
     T.ncolp = I.ncolp;
@@ -984,7 +996,7 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
     c->nnz = P.nnz;
     c->nmaj = c->nmin = c->nrow;
     c->layout = pb->layout;
-    c->result_is_pattern = true;
+    c->result_src = RES_PATTERN;
     c->has_F = heat;
     c->nF = heat ? c->nrow : 0;
     return 0;

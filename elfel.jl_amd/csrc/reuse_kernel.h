@@ -963,7 +963,8 @@ int pattern_to_host(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable 
                     std::vector<int32_t> &hptr, std::vector<int32_t> &hind);
 int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1, int64_t n,
                 const std::vector<int32_t> &hptr, const std::vector<int32_t> &hind, int cap, int ucap,
-                ReuseTables &T, bool patch = false, bool rows = false);
+                ReuseTables &T, bool patch = false, bool rows = false, const double2 *host_xy = nullptr,
+                std::string *errp = nullptr);
 int upload_tables(elfel_hip_ctx *c, const ReuseTables &T, Pattern &P);
 // symbolic.hip: the pattern and task tables of an elasticity or Stokes problem
 // built on the device (c->pat; fallback set when the mesh does not fit)

@@ -464,8 +464,8 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
               int *iters, int *outer, double *relres)
 {
     const int64_t n = nfree, N = c->nrow;
-    const int32_t *ptr = (const int32_t *)(c->result_is_pattern ? c->pat.ptr.p : c->ptr.p);
-    const int32_t *ind = (const int32_t *)(c->result_is_pattern ? c->pat.ind.p : c->ind.p);
+    const int32_t *ptr = result_ptr(c);
+    const int32_t *ind = result_ind(c);
     const double *val = (const double *)c->val.p;
     if (n == 0) {
         if (iters) *iters = 0;

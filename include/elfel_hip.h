@@ -168,8 +168,13 @@ enum { ELFEL_CHECK_GEOM = 1 };
  * ht_p2_p1_gen.jl:46-90, test/test_stokes.jl:629-668).  nnz may be NULL:
  * then the call does not wait for the device.
  * Element families (T3B, Q4L2, T4, L2) and the Reddy and mass forms run on
- * the plain FP64 method (ELFEL_SORT, ELFEL_FP64): heat on T4 (rule = tetrahedron
- * points 1 or 4) and L2 (Gauss order 1-3, sdim 1); mass on T3, T6, Q4, T4, L2;
+ * the plain FP64 method (ELFEL_SORT, ELFEL_FP64) or on ELFEL_REUSE: the sort
+ * of their COO stream is done once per mesh + dof map + weak form and kept as
+ * a permutation, and each assembly runs the element kernel and sums every
+ * matrix entry's run through it, in the sort's order (ELFEL_FP64: the plain
+ * method's bits; ELFEL_MIXED: element values stored in FP32, sums in FP64).
+ * Heat on T4 (rule = tetrahedron points 1 or 4) and L2 (Gauss order 1-3, sdim
+ * 1); mass on T3, T6, Q4, T4, L2;
  * Reddy Stokes with field 0 the velocity mesh, its node dofs ncomp = 2
  * (component 1 the ux space's numbers, 2 the uy space's) and for T3B its cell
  * dofs ncomp = 2, field 1 the pressure (T3 mesh + node dofs, or Q4L2 cell

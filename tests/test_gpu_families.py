@@ -1,5 +1,6 @@
 """GPU parity of the element families (FEH1_T4, FEH1_L2, FEH1_T3_BUBBLE,
-FEL2_Q4) and the Reddy / mass forms on the plain FP64 method.
+FEL2_Q4) and the Reddy / mass forms on the plain FP64 method (the REUSE
+method: test_gpu_families_reuse.py).
 
 The device result is compared with the oracle's family kernels
 (oracle/elfel_oracle_fam.c, pinned in test_oracle_families.py) on the same
@@ -187,12 +188,12 @@ def test_family_errors(fctx):
     Nn = xyz.shape[0]
     fctx.set_mesh(0, conn, xyz)
     fctx.set_dofs(0, np.arange(1, Nn + 1), Nn)
-    with pytest.raises(L.ElfelHipError) as e:  # only the plain FP64 method
-        fctx.assemble(L.HEAT, L.T4, 1, method=L.REUSE)
-    assert e.value.code == L.ENOTSUP
-    with pytest.raises(L.ElfelHipError) as e:
+    with pytest.raises(L.ElfelHipError) as e:  # the sort method is the plain FP64 method
         fctx.assemble(L.HEAT, L.T4, 1, precision=L.MIXED)
     assert e.value.code == L.ENOTSUP
+    with pytest.raises(L.ElfelHipError) as e:
+        fctx.assemble(L.HEAT, L.T4, 1, method=7)
+    assert e.value.code == L.EINVAL
     with pytest.raises(L.ElfelHipError) as e:  # no 3-point tetrahedron rule
         fctx.assemble(L.HEAT, L.T4, 3)
     assert e.value.code == L.ENOTSUP
