@@ -806,14 +806,14 @@ def bench_family(args, L):
     val64 = ctx.copy_sparse(base=0)[2]
     reuse_bitwise = bool(np.array_equal(val64, val0))
     progress("reuse timed")
-    # the mixed step's two kernels: element values (geometry and the sorted
-    # position of each value in, FP32 values out at those positions) and the
-    # runs (sorted values and run starts in, FP64 matrix and load vector out)
+    # the mixed step's two kernels: element values (geometry in, FP32 values
+    # out in stream order) and the runs (permutation, run starts and the
+    # values in, FP64 matrix and load vector out)
     nent, nf = E * cs["entries"], E * cs["fentries"]
-    el_bytes = cs["geom_in"] + 4 * (nent + nf) + 4 * (nent + nf)
-    runs_bytes = 4 * nent + 4 * (nnz + 1) + 8 * nnz
+    el_bytes = cs["geom_in"] + 4 * (nent + nf)
+    runs_bytes = 4 * nent + 4 * (nnz + 1) + 4 * nent + 8 * nnz
     if cs["heat"]:
-        runs_bytes += 4 * nf + 4 * (n + 1) + 8 * n
+        runs_bytes += 4 * nf + 4 * (n + 1) + 4 * nf + 8 * n
     el_ms, runs_ms = ph_mixed["element"], ph_mixed["reduce"]
     dom = "element" if el_ms >= runs_ms else "runs"
     kernel_bytes, kernel_ms = (el_bytes, el_ms) if dom == "element" else (runs_bytes, runs_ms)
@@ -848,7 +848,7 @@ def bench_family(args, L):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 element arithmetic, f32 stored element values, f64 sums",
+        "dtype": "f64 geometry, f32 element products, f64 sums",
         "data": "synthetic structured mesh, inputs resident in HBM",
         "config": {"workload": cs["workload"], "elements": E, "nodes": Nn, "dofs": n, "nnz": nnz, "layout": "CSR",
                    "method": "REUSE: element kernel (values only) -> runs summed through the cached sort permutation"},

@@ -209,7 +209,7 @@ extern "C" int elfel_hip_free(elfel_hip_ctx *c)
                    &c->pat.ptr, &c->pat.ind, &c->pat.tile, &c->pat.tile_rows, &c->pat.task_elem,
                    &c->pat.task_info, &c->pat.task_pos, &c->pat.tile_elem, &c->pat.conn_pad, &c->pat.xy_pad,
                    &c->pat.truns, &c->pat.runs, &c->pat.rrec, &c->pat.tconn, &c->pat.txy, &c->pat.sym, &c->pat.sym2,
-                   &c->fpat.ptr, &c->fpat.ind, &c->fpat.inv, &c->fpat.seg, &c->fpat.finv, &c->fpat.fbeg,
+                   &c->fpat.ptr, &c->fpat.ind, &c->fpat.perm, &c->fpat.seg, &c->fpat.fperm, &c->fpat.fbeg,
                    &c->fpat.coo, &c->fpat.fcoo};
     for (Buf *b : bufs) release(*b);
     batch_free(c);

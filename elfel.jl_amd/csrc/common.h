@@ -122,7 +122,7 @@ struct Pattern {
 
 // Pattern cached for the element families on the REUSE method (families.hip):
 // the stable sort of the COO stream done once, kept as a permutation -- for
-// each stream entry, its position in the sorted stream -- and the start of
+// each sorted position, the stream position of its entry -- and the start of
 // each matrix entry's run in sorted order (the load vector likewise, one run
 // per dof).  Built once per mesh + dof map + weak form.
 struct FamPattern {
@@ -132,9 +132,9 @@ struct FamPattern {
     int64_t n = 0, nv = 0, nnz = 0, ndofs = 0;
     double symbolic_ms = 0;
     Buf ptr, ind;     // CSR / CSC pattern (int32)
-    Buf inv, seg;     // int32 [n] sorted position per stream entry; int32 [nnz + 1] run starts
-    Buf finv, fbeg;   // load vector: int32 [nv]; int32 [ndofs + 1]
-    Buf coo, fcoo;    // the element values of one assembly, in sorted order (FP64, or FP32 for MIXED)
+    Buf perm, seg;    // int32 [n] stream position per sorted position; int32 [nnz + 1] run starts
+    Buf fperm, fbeg;  // load vector: int32 [nv]; int32 [ndofs + 1]
+    Buf coo, fcoo;    // the element values of one assembly, in stream order (FP64, or FP32 for MIXED)
 };
 
 struct Timing {

@@ -214,7 +214,56 @@ struct Jac {
             for (int s = 0; s < 3; s++) gx[s] = (g[0] * C[s][0] + g[1] * C[s][1] + g[2] * C[s][2]) / det;
         }
     }
+
+    // 1 / det J (1 / J11 in 1D), for grad_r
+    __device__ __forceinline__ double rdet() const { return 1.0 / (M == 1 ? a[0][0] : det); }
+
+    // grad with the divisions replaced by products with r = rdet() (the
+    // mixed kernels: a few FP64 units in the last place, below the FP32
+    // rounding of what they feed)
+    __device__ __forceinline__ void grad_r(const double *g, double *gx, double r) const
+    {
+        if constexpr (M == 1) {
+            gx[0] = g[0] * r;
+        } else if constexpr (M == 2) {
+            gx[0] = (a[1][1] * g[0] - a[1][0] * g[1]) * r;
+            gx[1] = (a[0][0] * g[1] - a[0][1] * g[0]) * r;
+        } else {
+            double C[3][3];
+            C[0][0] = a[1][1] * a[2][2] - a[1][2] * a[2][1];
+            C[0][1] = a[1][2] * a[2][0] - a[1][0] * a[2][2];
+            C[0][2] = a[1][0] * a[2][1] - a[1][1] * a[2][0];
+            C[1][0] = a[0][2] * a[2][1] - a[0][1] * a[2][2];
+            C[1][1] = a[0][0] * a[2][2] - a[0][2] * a[2][0];
+            C[1][2] = a[0][1] * a[2][0] - a[0][0] * a[2][1];
+            C[2][0] = a[0][1] * a[1][2] - a[0][2] * a[1][1];
+            C[2][1] = a[0][2] * a[1][0] - a[0][0] * a[1][2];
+            C[2][2] = a[0][0] * a[1][1] - a[0][1] * a[1][0];
+#pragma unroll
+            for (int s = 0; s < 3; s++) gx[s] = (g[0] * C[s][0] + g[1] * C[s][1] + g[2] * C[s][2]) * r;
+        }
+    }
 };
+
+// An element's N consecutive values: 16-byte stores when they fill whole
+// 16-byte words from a 16-byte aligned start (every buffer starts 256-byte
+// aligned and the element's offset is a multiple of N), else one per value.
+template <int N, typename VT>
+__device__ __forceinline__ void store_block(VT *dst, const VT (&v)[N])
+{
+    constexpr int W = 16 / (int)sizeof(VT);
+    if constexpr (N % W == 0) {
+#pragma unroll
+        for (int k = 0; k < N / W; k++) {
+            uint4 q;
+            __builtin_memcpy(&q, &v[k * W], 16);
+            *(uint4 *)(dst + k * W) = q;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; k++) dst[k] = v[k];
+    }
+}
 
 __device__ __forceinline__ uint64_t coo_key(int32_t r, int32_t col, int minor_bits, int csc)
 {
@@ -227,19 +276,19 @@ __device__ __forceinline__ uint64_t coo_key(int32_t r, int32_t col, int minor_bi
 //       (examples/heat/poisson/t4.jl:33-55, the loop of test/test_heat.jl:36-60);
 // mass: me[i,j] += (N_i N_j) * (kappa JxW) (examples/klein_gordon/klein_gordon.jl:34-51).
 //
-// VT: the stored value type (double; float for the REUSE method's MIXED
-// precision, the FP64 result rounded once).  keys / fkeys null: values only
-// (the REUSE method's numeric phase, whose sort order is cached), each stored
-// at vidx[o] / fvidx[o] -- its position in the sorted stream -- instead of at
-// its stream position o.
+// VT double: the FP64 kernel (the reference's operation order).  VT float:
+// the REUSE method's MIXED kernel -- FP64 geometry (J, det J, one reciprocal
+// of det J in place of the divisions), FP32 gradients, products and sums over
+// the points, FP32 stored values.  keys / fkeys null: values only (the REUSE
+// method's numeric phase, whose sort order is cached).
 template <int KIND, int M, int NB, int NQ, typename VT>
 __global__ __launch_bounds__(256) void k_fam_scalar(int64_t E, const int32_t *__restrict__ conn,
                                                     const double *__restrict__ x, const int32_t *__restrict__ dof,
                                                     FamTab tab, double kappa, double Q, int minor_bits, int csc,
                                                     uint64_t *__restrict__ keys, VT *__restrict__ vals,
-                                                    uint64_t *__restrict__ fkeys, VT *__restrict__ fvals,
-                                                    const int32_t *__restrict__ vidx, const int32_t *__restrict__ fvidx)
+                                                    uint64_t *__restrict__ fkeys, VT *__restrict__ fvals)
 {
+    constexpr bool F32 = sizeof(VT) == 4;
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     double X[NB * M];
@@ -252,53 +301,66 @@ __global__ __launch_bounds__(256) void k_fam_scalar(int64_t E, const int32_t *__
         for (int s = 0; s < M; s++) X[k * M + s] = x[(int64_t)n * M + s];
         d[k] = want_keys ? dof[n] : 0;
     }
-    double ke[NB * NB], fe[NB];
+    VT ke[NB * NB], fe[NB];
 #pragma unroll
-    for (int i = 0; i < NB * NB; i++) ke[i] = 0.0;
+    for (int i = 0; i < NB * NB; i++) ke[i] = VT(0);
 #pragma unroll
-    for (int i = 0; i < NB; i++) fe[i] = 0.0;
+    for (int i = 0; i < NB; i++) fe[i] = VT(0);
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
         Jac<M, NB> J;
         J.make(X, tab.Gg[q]);
         const double JxW = J.det * tab.w[q];
-        double g[NB][M];
+        const VT kj = (VT)(kappa * JxW);
+        VT g[NB][M];
         if constexpr (KIND == ELFEL_HEAT) {
+            if constexpr (F32) {
+                const double r = J.rdet();
 #pragma unroll
-            for (int k = 0; k < NB; k++) J.grad(tab.Gu[q][k], g[k]);
+                for (int k = 0; k < NB; k++) {
+                    double gd[M];
+                    J.grad_r(tab.Gu[q][k], gd, r);
+#pragma unroll
+                    for (int c = 0; c < M; c++) g[k][c] = (VT)gd[c];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < NB; k++) J.grad(tab.Gu[q][k], g[k]);
+            }
         }
 #pragma unroll
         for (int j = 0; j < NB; j++) {
 #pragma unroll
             for (int i = 0; i < NB; i++) {
                 if constexpr (KIND == ELFEL_HEAT) {
-                    double s = g[i][0] * g[j][0];
+                    VT s = g[i][0] * g[j][0];
 #pragma unroll
                     for (int c = 1; c < M; c++) s += g[i][c] * g[j][c];
-                    ke[i + NB * j] += s * (kappa * JxW);
+                    ke[i + NB * j] += s * kj;
                 } else {
-                    ke[i + NB * j] += (tab.Nu[q][i] * tab.Nu[q][j]) * (kappa * JxW);
+                    ke[i + NB * j] += (VT)(tab.Nu[q][i] * tab.Nu[q][j]) * kj;
                 }
             }
-            if constexpr (KIND == ELFEL_HEAT) fe[j] += tab.Nu[q][j] * Q * JxW;
+            if constexpr (KIND == ELFEL_HEAT) {
+                if constexpr (F32) fe[j] += (VT)(tab.Nu[q][j] * Q) * (VT)JxW;
+                else fe[j] += tab.Nu[q][j] * Q * JxW;
+            }
         }
     }
-    int64_t o = e * NB * NB;
+    if (keys) {
+        int64_t o = e * NB * NB;
 #pragma unroll
-    for (int j = 0; j < NB; j++)
+        for (int j = 0; j < NB; j++)
 #pragma unroll
-        for (int i = 0; i < NB; i++) {
-            if (keys) keys[o] = coo_key(d[i], d[j], minor_bits, csc);
-            vals[vidx ? (int64_t)vidx[o] : o] = (VT)ke[i + NB * j];
-            o++;
-        }
+            for (int i = 0; i < NB; i++) keys[o++] = coo_key(d[i], d[j], minor_bits, csc);
+    }
+    store_block<NB * NB>(vals + e * NB * NB, ke);
     if constexpr (KIND == ELFEL_HEAT) {
+        if (fkeys) {
 #pragma unroll
-        for (int j = 0; j < NB; j++) {
-            if (fkeys) fkeys[e * NB + j] = (uint64_t)d[j];
-            const int64_t fo = e * NB + j;
-            fvals[fvidx ? (int64_t)fvidx[fo] : fo] = (VT)fe[j];
+            for (int j = 0; j < NB; j++) fkeys[e * NB + j] = (uint64_t)d[j];
         }
+        store_block<NB>(fvals + e * NB, fe);
     }
 }
 
@@ -323,8 +385,9 @@ __global__ __launch_bounds__(256) void k_fam_reddy(int64_t E, const int32_t *__r
                                                    const int32_t *__restrict__ dof_p,
                                                    const int32_t *__restrict__ edof_p, FamTab tab, double mu,
                                                    int minor_bits, int csc, uint64_t *__restrict__ keys,
-                                                   VT *__restrict__ vals, const int32_t *__restrict__ vidx)
+                                                   VT *__restrict__ vals)
 {
+    constexpr bool F32 = sizeof(VT) == 4;  // the MIXED kernel (as k_fam_scalar)
     constexpr int NU = fam_nbf(VFE), NV = fam_ngeo(VFE), NP = fam_nbf(PFE);
     constexpr bool PNODAL = PFE == ELFEL_T3;
     constexpr int NG = PNODAL ? 3 : NV;
@@ -336,17 +399,22 @@ __global__ __launch_bounds__(256) void k_fam_reddy(int64_t E, const int32_t *__r
         const double2 p = PNODAL ? xy_p[conn_p[e * 3 + k]] : xy_v[conn_v[e * NV + k]];
         X[2 * k] = p.x; X[2 * k + 1] = p.y;
     }
-    double gx[NQ][NU], gy[NQ][NU], JxW[NQ];
+    // per point: the gradients, mu JxW and -JxW in VT (FP64: as formed)
+    VT gx[NQ][NU], gy[NQ][NU], mj[NQ], nj[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
         Jac<2, NG> J;
         J.make(X, tab.Gg[q]);
-        JxW[q] = J.det * tab.w[q];
+        const double JxW = J.det * tab.w[q];
+        mj[q] = (VT)(mu * JxW);
+        nj[q] = (VT)(-JxW);
+        const double r = F32 ? J.rdet() : 0.0;
 #pragma unroll
         for (int k = 0; k < NU; k++) {
             double g[2];
-            J.grad(tab.Gu[q][k], g);
-            gx[q][k] = g[0]; gy[q][k] = g[1];
+            if constexpr (F32) J.grad_r(tab.Gu[q][k], g, r);
+            else J.grad(tab.Gu[q][k], g);
+            gx[q][k] = (VT)g[0]; gy[q][k] = (VT)g[1];
         }
     }
     // dof numbers (for the keys only)
@@ -373,36 +441,48 @@ __global__ __launch_bounds__(256) void k_fam_reddy(int64_t E, const int32_t *__r
             dp[0] = edof_p[e];
         }
     }
-    int64_t o = e * (4 * NU * NU + 4 * NU * NP);
-    auto put = [&](int32_t r, int32_t col, double v) {
-        if (keys) keys[o] = coo_key(r, col, minor_bits, csc);
-        vals[vidx ? (int64_t)vidx[o] : o] = (VT)v;
-        o++;
+    // values in stream order, W at a time through a register word (cnt is a
+    // constant at every call once the loops are unrolled)
+    constexpr int NOUT = 4 * NU * NU + 4 * NU * NP, W = 16 / (int)sizeof(VT);
+    static_assert(NOUT % W == 0, "a Reddy element's values fill whole 16-byte words");
+    const int64_t o0 = e * NOUT;
+    VT *vout = vals + o0;
+    VT wb[W];
+    int cnt = 0;
+    auto put = [&](int32_t r, int32_t col, VT v) {
+        if (keys) keys[o0 + cnt] = coo_key(r, col, minor_bits, csc);
+        wb[cnt % W] = v;
+        if (cnt % W == W - 1) {
+            uint4 q;
+            __builtin_memcpy(&q, wb, 16);
+            *(uint4 *)(vout + cnt - (W - 1)) = q;
+        }
+        cnt++;
     };
-    double k[NU * NU];
+    VT k[NU * NU];
     // kxx
 #pragma unroll
-    for (int i = 0; i < NU * NU; i++) k[i] = 0.0;
+    for (int i = 0; i < NU * NU; i++) k[i] = VT(0);
 #pragma unroll
     for (int q = 0; q < NQ; q++)
 #pragma unroll
         for (int j = 0; j < NU; j++)
 #pragma unroll
             for (int i = 0; i < NU; i++)
-                k[i + NU * j] += (mu * JxW[q]) * (2 * gx[q][i] * gx[q][j] + gy[q][i] * gy[q][j]);
+                k[i + NU * j] += mj[q] * (2 * gx[q][i] * gx[q][j] + gy[q][i] * gy[q][j]);
 #pragma unroll
     for (int j = 0; j < NU; j++)
 #pragma unroll
         for (int i = 0; i < NU; i++) put(dx[i], dx[j], k[i + NU * j]);
     // kxy and its transpose
 #pragma unroll
-    for (int i = 0; i < NU * NU; i++) k[i] = 0.0;
+    for (int i = 0; i < NU * NU; i++) k[i] = VT(0);
 #pragma unroll
     for (int q = 0; q < NQ; q++)
 #pragma unroll
         for (int j = 0; j < NU; j++)
 #pragma unroll
-            for (int i = 0; i < NU; i++) k[i + NU * j] += (mu * JxW[q]) * (gx[q][i] * gy[q][j]);
+            for (int i = 0; i < NU; i++) k[i + NU * j] += mj[q] * (gx[q][i] * gy[q][j]);
 #pragma unroll
     for (int j = 0; j < NU; j++)
 #pragma unroll
@@ -413,14 +493,14 @@ __global__ __launch_bounds__(256) void k_fam_reddy(int64_t E, const int32_t *__r
         for (int a = 0; a < NU; a++) put(dy[a], dx[b], k[b + NU * a]);
     // kyy
 #pragma unroll
-    for (int i = 0; i < NU * NU; i++) k[i] = 0.0;
+    for (int i = 0; i < NU * NU; i++) k[i] = VT(0);
 #pragma unroll
     for (int q = 0; q < NQ; q++)
 #pragma unroll
         for (int j = 0; j < NU; j++)
 #pragma unroll
             for (int i = 0; i < NU; i++)
-                k[i + NU * j] += (mu * JxW[q]) * (gx[q][i] * gx[q][j] + 2 * gy[q][i] * gy[q][j]);
+                k[i + NU * j] += mj[q] * (gx[q][i] * gx[q][j] + 2 * gy[q][i] * gy[q][j]);
 #pragma unroll
     for (int j = 0; j < NU; j++)
 #pragma unroll
@@ -429,14 +509,14 @@ __global__ __launch_bounds__(256) void k_fam_reddy(int64_t E, const int32_t *__r
 #pragma unroll
     for (int c = 0; c < 2; c++) {
 #pragma unroll
-        for (int i = 0; i < NU * NP; i++) k[i] = 0.0;
+        for (int i = 0; i < NU * NP; i++) k[i] = VT(0);
 #pragma unroll
         for (int q = 0; q < NQ; q++)
 #pragma unroll
             for (int j = 0; j < NP; j++)
 #pragma unroll
                 for (int i = 0; i < NU; i++)
-                    k[i + NU * j] += (-JxW[q]) * ((c == 0 ? gx[q][i] : gy[q][i]) * tab.Np[q][j]);
+                    k[i + NU * j] += nj[q] * ((c == 0 ? gx[q][i] : gy[q][i]) * (VT)tab.Np[q][j]);
 #pragma unroll
         for (int j = 0; j < NP; j++)
 #pragma unroll
@@ -459,7 +539,6 @@ bool reddy_pair(int vfe, int pfe)
 struct FamOut {
     uint64_t *keys, *fkeys;
     void *vals, *fvals;
-    const int32_t *vidx, *fvidx;  // sorted position of each stream entry (null: stream order)
 };
 
 template <int KIND, int M, int NB, int NQ, typename VT>
@@ -474,7 +553,7 @@ int launch_scalar(elfel_hip_ctx *c, const FamTab &tab, double kappa, double Q, i
         hipLaunchKernelGGL((k_fam_scalar<KIND, M, NB, NQ, VT>), dim3(ceil_div(F.E, 256)), dim3(256), 0, c->stream, F.E,
                            (const int32_t *)F.conn.p, (const double *)F.xy.p, (const int32_t *)F.dof.p, tab, kappa, Q,
                            minor_bits, csc, o.keys, (VT *)o.vals, heat ? o.fkeys : nullptr,
-                           heat ? (VT *)o.fvals : nullptr, o.vidx, heat ? o.fvidx : nullptr);
+                           heat ? (VT *)o.fvals : nullptr);
     HIPCHK(c, hipGetLastError());
     tm_end(c, ELFEL_PH_ELEMENT, ev);
     return 0;
@@ -494,7 +573,7 @@ int launch_reddy(elfel_hip_ctx *c, const FamTab &tab, double mu, int minor_bits,
                            PFE == ELFEL_T3 ? (const double2 *)P.xy.p : nullptr,
                            PFE == ELFEL_T3 ? (const int32_t *)P.dof.p : nullptr,
                            PFE == ELFEL_Q4L2 ? (const int32_t *)P.edof.p : nullptr, tab, mu, minor_bits, csc,
-                           o.keys, (VT *)o.vals, o.vidx);
+                           o.keys, (VT *)o.vals);
     HIPCHK(c, hipGetLastError());
     tm_end(c, ELFEL_PH_ELEMENT, ev);
     return 0;
@@ -505,22 +584,22 @@ int launch_reddy(elfel_hip_ctx *c, const FamTab &tab, double mu, int minor_bits,
 // keeps stream order among equal keys; each matrix entry is then the sum of its
 // run, left to right.  The order depends on the mesh and the dof map only, so
 // the symbolic phase sorts the stream once with each entry's stream position
-// as payload and inverts the result; every assembly after it runs the element
-// kernel, which stores each value at its sorted position (no keys), and sums
-// each run of the sorted values in the same order: FP64 bit for bit the plain
-// method's result.  (Storing in stream order and gathering through the
-// permutation in the sum measured slower: heat T4 64^3 mixed 0.44 ms.)
+// as payload; every assembly after it runs the element kernel (values only, in
+// stream order) and sums each run through the cached permutation, in the same
+// order: FP64 bit for bit the plain method's result.  (Storing each value at
+// its sorted position instead, so that the sums read contiguous runs, measured
+// slower: the scattered 4-byte stores took heat T4 64^3's element kernel from
+// 0.14 to 0.38 ms.)
 __global__ void k_fam_iota(uint64_t *__restrict__ v, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) v[i] = (uint64_t)i;
 }
 
-// inv[perm[i]] = i: the sorted position of every stream entry
-__global__ void k_fam_invert(const uint64_t *__restrict__ perm, int32_t *__restrict__ inv, int64_t n)
+__global__ void k_fam_narrow(const uint64_t *__restrict__ src, int32_t *__restrict__ dst, int64_t n)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) inv[perm[i]] = (int32_t)i;
+    if (i < n) dst[i] = (int32_t)src[i];
 }
 
 // run starts of the matrix entries: the segment heads, then n
@@ -554,30 +633,47 @@ __global__ void k_fam_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v)
 // first entry (sparse()'s combination of duplicates), in FP64.  Work-item
 // nnz + d (d < ndofs, heat): load-vector entry d the same way (start! zeros
 // it, assemble! adds: src/Assemblers.jl:194-221).
-template <typename VT>
-__global__ __launch_bounds__(256) void k_fam_runs(const int32_t *__restrict__ seg, const VT *__restrict__ sv,
-                                                  int64_t nnz, double *__restrict__ val,
-                                                  const int32_t *__restrict__ fbeg, const VT *__restrict__ fsv,
-                                                  int64_t ndofs, double *__restrict__ F)
+//
+// The run sorted positions i0 .. i1 - 1 (i0 < i1), summed left to right from
+// the first: its stream positions, then its values, RUNB at a time, so that a
+// run's loads are in flight together instead of one dependent pair per entry.
+template <int RUNB, typename VT>
+__device__ __forceinline__ double run_sum(const int32_t *__restrict__ perm, const VT *__restrict__ v, int32_t i0,
+                                          int32_t i1)
+{
+    double acc = 0.0;
+    for (int32_t b = i0; b < i1; b += RUNB) {
+        int32_t p[RUNB];
+        VT x[RUNB];
+#pragma unroll
+        for (int u = 0; u < RUNB; u++) p[u] = b + u < i1 ? perm[b + u] : 0;
+#pragma unroll
+        for (int u = 0; u < RUNB; u++) x[u] = b + u < i1 ? v[p[u]] : VT(0);
+#pragma unroll
+        for (int u = 0; u < RUNB; u++)
+            if (b + u < i1) acc = (b + u == i0) ? (double)x[u] : acc + (double)x[u];
+    }
+    return acc;
+}
+
+//
+// (Contiguous eighths of the entries per XCD, so that neighbouring rows share
+// one XCD's L2, measured slower: heat T4 64^3 0.48 against 0.20 ms.)
+template <typename VT, int RUNB>
+__global__ __launch_bounds__(256) void k_fam_runs(const int32_t *__restrict__ perm, const int32_t *__restrict__ seg,
+                                                  const VT *__restrict__ coo, int64_t nnz, double *__restrict__ val,
+                                                  const int32_t *__restrict__ fperm, const int32_t *__restrict__ fbeg,
+                                                  const VT *__restrict__ fcoo, int64_t ndofs, double *__restrict__ F)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s < nnz) {
-        const int32_t i0 = seg[s], i1 = seg[s + 1];
-        double acc = (double)sv[i0];
-        for (int32_t i = i0 + 1; i < i1; i++) acc += (double)sv[i];
-        val[s] = acc;
+        val[s] = run_sum<RUNB>(perm, coo, seg[s], seg[s + 1]);
         return;
     }
     const int64_t d = s - nnz;
     if (d >= ndofs) return;
     const int32_t i0 = fbeg[d], i1 = fbeg[d + 1];
-    if (i0 == i1) {
-        F[d] = 0.0;
-        return;
-    }
-    double acc = (double)fsv[i0];
-    for (int32_t i = i0 + 1; i < i1; i++) acc += (double)fsv[i];
-    F[d] = 0.0 + acc;
+    F[d] = i0 == i1 ? 0.0 : 0.0 + run_sum<RUNB>(fperm, fcoo, i0, i1);
 }
 
 // The element kernel of the problem (its weak form, element and rule) with
@@ -651,8 +747,8 @@ int fam_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &ta
     P.valid = false;
     int rc;
     if ((rc = plain_buffers(c, n, nv, ndofs, heat)) || (rc = ensure(c, P.coo, 8 * std::max<int64_t>(n, 1))) ||
-        (rc = ensure(c, P.fcoo, 8 * std::max<int64_t>(nv, 1))) || (rc = ensure(c, P.inv, 4 * std::max<int64_t>(n, 1))) ||
-        (rc = ensure(c, P.finv, 4 * std::max<int64_t>(nv, 1))) || (rc = ensure(c, P.fbeg, 4 * (ndofs + 1))) ||
+        (rc = ensure(c, P.fcoo, 8 * std::max<int64_t>(nv, 1))) || (rc = ensure(c, P.perm, 4 * std::max<int64_t>(n, 1))) ||
+        (rc = ensure(c, P.fperm, 4 * std::max<int64_t>(nv, 1))) || (rc = ensure(c, P.fbeg, 4 * (ndofs + 1))) ||
         (rc = ensure(c, P.ptr, 4 * (ndofs + 1))) || (rc = ensure(c, P.ind, 4 * std::max<int64_t>(n, 1))))
         return rc;
     uint64_t *k0 = (uint64_t *)c->keys[0].p, *k1 = (uint64_t *)c->keys[1].p;
@@ -660,7 +756,7 @@ int fam_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &ta
     uint64_t *fk0 = (uint64_t *)c->fkeys[0].p, *fk1 = (uint64_t *)c->fkeys[1].p;
     uint64_t *fv0 = (uint64_t *)c->fvals[0].p, *fv1 = (uint64_t *)c->fvals[1].p;
     // the stream's keys (its values land in coo, unused here)
-    const FamOut o{k0, heat ? fk0 : nullptr, P.coo.p, heat ? P.fcoo.p : nullptr, nullptr, nullptr};
+    const FamOut o{k0, heat ? fk0 : nullptr, P.coo.p, heat ? P.fcoo.p : nullptr};
     if ((rc = fam_elements(c, pb, tab, minor_bits, csc, o, false))) return rc;
     hipEvent_t ev = nullptr;
     tm_begin(c, ELFEL_PH_SORT, &ev);
@@ -679,11 +775,11 @@ int fam_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &ta
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const int64_t nnz = hnnz;
     if ((rc = ensure(c, P.seg, 4 * (nnz + 1)))) return rc;
-    FAM_LAUNCH(k_fam_invert, n, v0, (int32_t *)P.inv.p, n);
+    FAM_LAUNCH(k_fam_narrow, n, v0, (int32_t *)P.perm.p, n);
     if (n > 0) FAM_LAUNCH(k_fam_seg, nnz + 1, (const uint32_t *)c->segpos.p, nnz, n, (int32_t *)P.seg.p);
     else FAM_LAUNCH(k_fam_fill_i32, (int64_t)1, (int32_t *)P.seg.p, (int64_t)1, 0);
     if (heat) {
-        FAM_LAUNCH(k_fam_invert, nv, fv0, (int32_t *)P.finv.p, nv);
+        FAM_LAUNCH(k_fam_narrow, nv, fv0, (int32_t *)P.fperm.p, nv);
         if (nv > 0) FAM_LAUNCH(k_fam_dense_begin, nv, fk0, nv, ndofs, (int32_t *)P.fbeg.p);
         else FAM_LAUNCH(k_fam_fill_i32, ndofs + 1, (int32_t *)P.fbeg.p, ndofs + 1, 0);
     }
@@ -696,8 +792,8 @@ int fam_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &ta
     return 0;
 }
 
-// Numeric phase: element values only, each at its sorted position, then every
-// run summed.
+// Numeric phase: element values only, then every run summed through the
+// cached permutation.
 int fam_reuse(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &tab, int64_t n, int64_t nv,
               int64_t ndofs, int minor_bits, int csc, bool heat)
 {
@@ -707,19 +803,20 @@ int fam_reuse(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &tab, 
     const bool f32 = pb->precision == ELFEL_MIXED;
     if ((rc = ensure(c, c->val, 8 * std::max<int64_t>(P.nnz, 1)))) return rc;
     if (heat && (rc = ensure(c, c->F, 8 * std::max<int64_t>(ndofs, 1)))) return rc;
-    const FamOut o{nullptr, nullptr, P.coo.p, heat ? P.fcoo.p : nullptr, (const int32_t *)P.inv.p,
-                   heat ? (const int32_t *)P.finv.p : nullptr};
+    const FamOut o{nullptr, nullptr, P.coo.p, heat ? P.fcoo.p : nullptr};
     if ((rc = fam_elements(c, pb, tab, minor_bits, csc, o, f32))) return rc;
     hipEvent_t ev = nullptr;
     tm_begin(c, ELFEL_PH_REDUCE, &ev);
     const int64_t nf = heat ? ndofs : 0, nw = P.nnz + nf;
-    if (f32)
-        FAM_LAUNCH(k_fam_runs<float>, nw, (const int32_t *)P.seg.p, (const float *)P.coo.p, P.nnz, (double *)c->val.p,
-                   (const int32_t *)P.fbeg.p, (const float *)P.fcoo.p, nf, heat ? (double *)c->F.p : nullptr);
-    else
-        FAM_LAUNCH(k_fam_runs<double>, nw, (const int32_t *)P.seg.p, (const double *)P.coo.p, P.nnz,
-                   (double *)c->val.p, (const int32_t *)P.fbeg.p, (const double *)P.fcoo.p, nf,
-                   heat ? (double *)c->F.p : nullptr);
+    // 8 loads of a run in flight together (4 and 16 measured the same on heat
+    // T4 64^3, 16 slower on P1b/P1 512^2)
+#define RUNS(VT_)                                                                                               \
+    FAM_LAUNCH((k_fam_runs<VT_, 8>), nw, (const int32_t *)P.perm.p, (const int32_t *)P.seg.p,                   \
+               (const VT_ *)P.coo.p, P.nnz, (double *)c->val.p, (const int32_t *)P.fperm.p,                     \
+               (const int32_t *)P.fbeg.p, (const VT_ *)P.fcoo.p, nf, heat ? (double *)c->F.p : nullptr)
+    if (f32) RUNS(float);
+    else RUNS(double);
+#undef RUNS
     tm_end(c, ELFEL_PH_REDUCE, ev);
     c->nnz = P.nnz;
     c->nmaj = c->nmin = ndofs;
@@ -811,7 +908,7 @@ int family_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb)
     int rc = plain_buffers(c, n, nv, ndofs, heat);
     if (rc) return rc;
     const FamOut o{(uint64_t *)c->keys[0].p, heat ? (uint64_t *)c->fkeys[0].p : nullptr, c->vals[0].p,
-                   heat ? c->fvals[0].p : nullptr, nullptr, nullptr};
+                   heat ? c->fvals[0].p : nullptr};
     if ((rc = fam_elements(c, pb, tab, minor_bits, csc, o, false))) return rc;
     return plain_finish(c, pb->layout, n, nv, ndofs, minor_bits, heat);
 }

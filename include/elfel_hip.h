@@ -172,7 +172,8 @@ enum { ELFEL_CHECK_GEOM = 1 };
  * of their COO stream is done once per mesh + dof map + weak form and kept as
  * a permutation, and each assembly runs the element kernel and sums every
  * matrix entry's run through it, in the sort's order (ELFEL_FP64: the plain
- * method's bits; ELFEL_MIXED: element values stored in FP32, sums in FP64).
+ * method's bits; ELFEL_MIXED: FP64 geometry, FP32 products and element
+ * values, FP64 sums).
  * Heat on T4 (rule = tetrahedron points 1 or 4) and L2 (Gauss order 1-3, sdim
  * 1); mass on T3, T6, Q4, T4, L2;
  * Reddy Stokes with field 0 the velocity mesh, its node dofs ncomp = 2

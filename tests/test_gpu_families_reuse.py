@@ -4,9 +4,10 @@ runs the element kernel (values only) and sums every entry's run through it.
 
 FP64: the plain method's result bit for bit, hence the oracle's (pattern,
 values, load vector), on the first call (symbolic phase) and on cached calls.
-MIXED (element values stored in FP32, sums in FP64): the same pattern, values
-and load vector within 1e-6 of the oracle's in the Frobenius norm relative to
-the oracle's (each stored value carries one FP32 rounding, 6e-8 relative).
+MIXED (FP64 geometry, FP32 gradients, products and element values, FP64
+sums): the same pattern, values and load vector within 1e-6 of the oracle's in
+the Frobenius norm relative to the oracle's (a few FP32 roundings per value,
+each 6e-8 relative).
 """
 import numpy as np
 import pytest
