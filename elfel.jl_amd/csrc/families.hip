@@ -656,24 +656,54 @@ __device__ __forceinline__ double run_sum(const int32_t *__restrict__ perm, cons
     return acc;
 }
 
-//
-// (Contiguous eighths of the entries per XCD, so that neighbouring rows share
-// one XCD's L2, measured slower: heat T4 64^3 0.48 against 0.20 ms.)
-template <typename VT, int RUNB>
+// Workgroup b < nbm: matrix entries 256 b .. 256 b + 255; workgroup nbm + b:
+// load-vector entries likewise.  The runs of a workgroup's entries are one
+// contiguous range of sorted positions: when it holds at most 16 KiB of
+// values, the workgroup first gathers them into LDS, one sorted position per
+// work-item (the permutation read in whole lines, every lane's gather useful),
+// and each work-item then sums its run from LDS; a longer range gathers run by
+// run (run_sum).  Gathering run by run everywhere left most lanes idle in the
+// waves that hold a long run (a heat T4 diagonal entry sums 24 values, an
+// off-diagonal one 1 to 6): heat T4 64^3 0.20 ms.  (Contiguous eighths of the
+// entries per XCD, so that neighbouring rows share one XCD's L2, measured
+// slower still: 0.48 ms.)
+template <typename VT>
 __global__ __launch_bounds__(256) void k_fam_runs(const int32_t *__restrict__ perm, const int32_t *__restrict__ seg,
                                                   const VT *__restrict__ coo, int64_t nnz, double *__restrict__ val,
                                                   const int32_t *__restrict__ fperm, const int32_t *__restrict__ fbeg,
-                                                  const VT *__restrict__ fcoo, int64_t ndofs, double *__restrict__ F)
+                                                  const VT *__restrict__ fcoo, int64_t ndofs, double *__restrict__ F,
+                                                  int64_t nbm)
 {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < nnz) {
-        val[s] = run_sum<RUNB>(perm, coo, seg[s], seg[s + 1]);
-        return;
+    constexpr int RUNS_CAP = 16384 / (int)sizeof(VT);  // 32 KiB of FP64 values: P1b/P1 FP64 0.72 against 0.59 ms
+    __shared__ VT sv[RUNS_CAP];
+    const bool mat = blockIdx.x < nbm;
+    const int32_t *__restrict__ pm = mat ? perm : fperm;
+    const int32_t *__restrict__ sg = mat ? seg : fbeg;
+    const VT *__restrict__ v = mat ? coo : fcoo;
+    const int64_t cnt = mat ? nnz : ndofs;
+    const int64_t e0 = (mat ? (int64_t)blockIdx.x : (int64_t)blockIdx.x - nbm) * 256;
+    const int64_t e1 = e0 + 256 < cnt ? e0 + 256 : cnt;
+    const int32_t p0 = sg[e0];
+    const int32_t np = sg[e1] - p0;
+    const bool staged = np <= RUNS_CAP;  // the same in the whole workgroup
+    if (staged) {
+        for (int32_t k = threadIdx.x; k < np; k += 256) sv[k] = v[pm[p0 + k]];
+        __syncthreads();
     }
-    const int64_t d = s - nnz;
-    if (d >= ndofs) return;
-    const int32_t i0 = fbeg[d], i1 = fbeg[d + 1];
-    F[d] = i0 == i1 ? 0.0 : 0.0 + run_sum<RUNB>(fperm, fcoo, i0, i1);
+    const int64_t s = e0 + threadIdx.x;
+    if (s >= e1) return;
+    const int32_t i0 = sg[s], i1 = sg[s + 1];
+    double acc = 0.0;
+    if (i0 < i1) {
+        if (staged) {
+            acc = (double)sv[i0 - p0];
+            for (int32_t i = i0 + 1; i < i1; i++) acc += (double)sv[i - p0];
+        } else {
+            acc = run_sum<8>(pm, v, i0, i1);
+        }
+    }
+    if (mat) val[s] = acc;
+    else F[s] = i0 == i1 ? 0.0 : 0.0 + acc;  // start! zeros the vector, assemble! adds
 }
 
 // The element kernel of the problem (its weak form, element and rule) with
@@ -807,16 +837,21 @@ int fam_reuse(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &tab, 
     if ((rc = fam_elements(c, pb, tab, minor_bits, csc, o, f32))) return rc;
     hipEvent_t ev = nullptr;
     tm_begin(c, ELFEL_PH_REDUCE, &ev);
-    const int64_t nf = heat ? ndofs : 0, nw = P.nnz + nf;
-    // 8 loads of a run in flight together (4 and 16 measured the same on heat
-    // T4 64^3, 16 slower on P1b/P1 512^2)
-#define RUNS(VT_)                                                                                               \
-    FAM_LAUNCH((k_fam_runs<VT_, 8>), nw, (const int32_t *)P.perm.p, (const int32_t *)P.seg.p,                   \
-               (const VT_ *)P.coo.p, P.nnz, (double *)c->val.p, (const int32_t *)P.fperm.p,                     \
-               (const int32_t *)P.fbeg.p, (const VT_ *)P.fcoo.p, nf, heat ? (double *)c->F.p : nullptr)
-    if (f32) RUNS(float);
-    else RUNS(double);
-#undef RUNS
+    const int64_t nf = heat ? ndofs : 0;
+    const int64_t nbm = ceil_div(P.nnz, 256), nbw = nbm + ceil_div(nf, 256);
+    if (nbw > 0) {
+        if (f32)
+            hipLaunchKernelGGL((k_fam_runs<float>), dim3(nbw), dim3(256), 0, c->stream, (const int32_t *)P.perm.p,
+                               (const int32_t *)P.seg.p, (const float *)P.coo.p, P.nnz, (double *)c->val.p,
+                               (const int32_t *)P.fperm.p, (const int32_t *)P.fbeg.p, (const float *)P.fcoo.p, nf,
+                               heat ? (double *)c->F.p : nullptr, nbm);
+        else
+            hipLaunchKernelGGL((k_fam_runs<double>), dim3(nbw), dim3(256), 0, c->stream, (const int32_t *)P.perm.p,
+                               (const int32_t *)P.seg.p, (const double *)P.coo.p, P.nnz, (double *)c->val.p,
+                               (const int32_t *)P.fperm.p, (const int32_t *)P.fbeg.p, (const double *)P.fcoo.p, nf,
+                               heat ? (double *)c->F.p : nullptr, nbm);
+        HIPCHK(c, hipGetLastError());
+    }
     tm_end(c, ELFEL_PH_REDUCE, ev);
     c->nnz = P.nnz;
     c->nmaj = c->nmin = ndofs;
