@@ -52,7 +52,8 @@ enum { ELFEL_T3 = 0, ELFEL_T6 = 1, ELFEL_Q4 = 2, ELFEL_T3B = 3, ELFEL_Q4L2 = 4, 
 enum { ELFEL_FP64 = 0, ELFEL_MIXED = 1 };
 /* global build method */
 enum { ELFEL_SORT = 0,    /* COO in stream order -> stable radix sort -> in-order segmented reduce */
-       ELFEL_REUSE = 1 }; /* cached pattern + fused row-owner kernel (no sort, no atomics) */
+       ELFEL_REUSE = 1 }; /* cached pattern + fused row-owner kernel; for the element families the
+                             cached sort permutation + in-order run sums (no sort, no atomics) */
 /* sparse layout of the result */
 enum { ELFEL_CSR = 0, ELFEL_CSC = 1 };
 /* timing phases (elfel_hip_timing_read) */
