@@ -312,7 +312,7 @@ def bench_single(args, L, W):
         "device": {"name": devname, "cus": ncu},
     }
     line.update(extra)
-    print(json.dumps(line))
+    print(json.dumps(with_copy_peak(line, ctx)))
 
 
 RAMP_S = 0.3  # seconds of untimed back-to-back launches before the warm-up steps
@@ -382,6 +382,24 @@ def measured_traffic(cfgname, N_arg, N, block=None):
     return {"traffic": t["traffic_bytes"], "traffic_fetch": t["fetch_bytes"], "traffic_write": t["write_bytes"],
             "traffic_source": t["source"] + " (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, x2 / x1 copy calibration)",
             "source_hash": src}
+
+
+def with_copy_peak(line, ctx):
+    """The roofline beside a measured ceiling: a plain 1 GiB device copy
+    (elfel_hip_copy_bandwidth, read + write bytes, 10 launches)."""
+    try:
+        peak = ctx.copy_bandwidth(1 << 30, 10)
+    except Exception as e:  # the bench line stands without it
+        line["roofline"]["copy_peak_note"] = f"copy probe failed: {e}"
+        return line
+    r = line["roofline"]
+    r["copy_peak_gbs"] = round(peak, 1)
+    r["frac_of_copy_peak"] = round(r["achieved"] / peak, 4)
+    if r.get("traffic"):
+        ms = r.get("kernel_ms") or r.get("kernel_ms_per_launch")
+        if ms:
+            r["traffic_gbs"] = round(r["traffic"] / (ms * 1e-3) / 1e9, 1)
+    return line
 
 
 def device_setup_c5(L, cs):
@@ -591,7 +609,7 @@ def bench_batch(args, L, W):
         line["roofline"].update(measured_traffic("c4", 0, 0, block=info0["block"]))
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_batch(probs, sample)
-    print(json.dumps(line))
+    print(json.dumps(with_copy_peak(line, ctx)))
 
 
 def copy_probe(mib):
@@ -874,7 +892,7 @@ def bench_family(args, L):
         "setup": {"host_mesh_numbering_upload_s": t_setup},
         "device": {"name": devname, "cus": ncu},
     }
-    print(json.dumps(line))
+    print(json.dumps(with_copy_peak(line, ctx)))
 
 
 def main():

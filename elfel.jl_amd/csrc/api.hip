@@ -240,6 +240,47 @@ extern "C" int elfel_hip_device_name(elfel_hip_ctx *c, char *buf, int len, int *
     return ELFEL_OK;
 }
 
+// A plain device copy, 16 bytes per work-item: the measured bandwidth that the
+// roofline fractions are read beside (the published 8 TB/s is the peak).
+__global__ __launch_bounds__(256) void k_copy_u4(const uint4 *__restrict__ a, uint4 *__restrict__ b, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+extern "C" int elfel_hip_copy_bandwidth(elfel_hip_ctx *c, int64_t bytes, int reps, double *gbs)
+{
+    CHECK_CTX(c);
+    if (bytes < 16 || (bytes & 15) || reps < 1 || !gbs) return set_err(c, ELFEL_EINVAL, "elfel_hip_copy_bandwidth: bad arguments");
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int64_t n = bytes / 16;
+    int rc = ELFEL_OK;
+    float ms = 0;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+        rc = set_err(c, ELFEL_ENOMEM, "elfel_hip_copy_bandwidth: hipMalloc failed");
+    } else if (hipMemsetAsync(a, 1, bytes, c->stream) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+               hipEventCreate(&e1) != hipSuccess) {
+        rc = set_err(c, ELFEL_EHIP, "elfel_hip_copy_bandwidth: set-up failed");
+    } else {
+        const dim3 grid((unsigned)ceil_div(n, 256));
+        hipLaunchKernelGGL(k_copy_u4, grid, dim3(256), 0, c->stream, (const uint4 *)a, (uint4 *)b, n);  // warm-up
+        hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL(k_copy_u4, grid, dim3(256), 0, c->stream, (const uint4 *)a, (uint4 *)b, n);
+        hipEventRecord(e1, c->stream);
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0)
+            rc = set_err(c, ELFEL_EHIP, "elfel_hip_copy_bandwidth: timing failed");
+        else
+            *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    return rc;
+}
+
 extern "C" int elfel_hip_synchronize(elfel_hip_ctx *c)
 {
     CHECK_CTX(c);

@@ -25,7 +25,7 @@ NPHASES = 8
 PHASE_NAMES = ["gather", "element", "sort", "reduce", "numeric", "symbolic", "total", "solve"]
 
 EXPORTED = [
-    "elfel_hip_init", "elfel_hip_free", "elfel_hip_last_error", "elfel_hip_device_name",
+    "elfel_hip_init", "elfel_hip_free", "elfel_hip_last_error", "elfel_hip_device_name", "elfel_hip_copy_bandwidth",
     "elfel_hip_sparse", "elfel_hip_copy_sparse", "elfel_hip_result_info", "elfel_hip_vector",
     "elfel_hip_set_mesh", "elfel_hip_set_dofs", "elfel_hip_set_coefficients", "elfel_hip_assemble",
     "elfel_hip_copy_vector", "elfel_hip_synchronize", "elfel_hip_geometry_stats", "elfel_hip_pcg",
@@ -80,6 +80,7 @@ def load():
     lib.elfel_hip_last_error.argtypes = [P]
     lib.elfel_hip_last_error.restype = ctypes.c_char_p
     lib.elfel_hip_device_name.argtypes = [P, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.elfel_hip_copy_bandwidth.argtypes = [P, ctypes.c_int64, ctypes.c_int, f64p]
     lib.elfel_hip_sparse.argtypes = [P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, i64p, i64p, f64p,
                                      ctypes.c_int, i64p]
     lib.elfel_hip_copy_sparse.argtypes = [P, ctypes.c_int, i64p, i64p, f64p]
@@ -180,6 +181,12 @@ class Context:
         cus = ctypes.c_int(0)
         self._chk(self.lib.elfel_hip_device_name(self.h, buf, 256, ctypes.byref(cus)), "device_name")
         return buf.value.decode(), cus.value
+
+    def copy_bandwidth(self, nbytes=1 << 30, reps=10):
+        """GB/s (read + write) of a plain device copy of nbytes."""
+        g = ctypes.c_double()
+        self._chk(self.lib.elfel_hip_copy_bandwidth(self.h, nbytes, reps, ctypes.byref(g)), "copy_bandwidth")
+        return g.value
 
     # ---- generic drop-in --------------------------------------------------
     def sparse(self, I, J, V, nrow, ncol, layout=CSC):

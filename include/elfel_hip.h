@@ -68,6 +68,12 @@ int elfel_hip_init(int device, elfel_hip_ctx **ctx);
 int elfel_hip_free(elfel_hip_ctx *ctx);
 const char *elfel_hip_last_error(const elfel_hip_ctx *ctx);
 int elfel_hip_device_name(elfel_hip_ctx *ctx, char *buf, int len, int *num_cus);
+/* Bandwidth of a plain device-to-device copy of `bytes` (a multiple of 16)
+ * through HBM, timed over `reps` launches after one warm-up: *gbs = bytes read
+ * + bytes written per second / 1e9.  The measured ceiling beside which the
+ * benchmark reads its roofline fractions (the published HBM peak is 8 TB/s).
+ * Allocates and frees two buffers.  No counterpart in the reference. */
+int elfel_hip_copy_bandwidth(elfel_hip_ctx *ctx, int64_t bytes, int reps, double *gbs);
 
 /* ---- generic drop-in for SysmatAssemblerSparse ----------------------------
  * Replaces finish!(self::SysmatAssemblerSparse) = sparse(row, col, val, nrow,
