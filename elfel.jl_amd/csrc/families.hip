@@ -939,7 +939,15 @@ int family_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb)
     const int64_t ndofs = c->nrow;
     const int minor_bits = bits_for(ndofs);
     const int csc = pb->layout == ELFEL_CSC ? 1 : 0;
-    if (pb->method == ELFEL_REUSE) return fam_reuse(c, pb, tab, n, nv, ndofs, minor_bits, csc, heat);
+    // the cached permutation and run starts are int32: a longer stream runs the
+    // plain FP64 method (its sort carries 64-bit positions)
+    if (pb->method == ELFEL_REUSE && n < INT32_MAX && nv < INT32_MAX)
+        return fam_reuse(c, pb, tab, n, nv, ndofs, minor_bits, csc, heat);
+    if (pb->method == ELFEL_REUSE) {
+        c->used_method = ELFEL_SORT;
+        c->used_precision = ELFEL_FP64;
+        c->used_note = "plain FP64 method: a COO stream of 2^31 entries or more";
+    }
     int rc = plain_buffers(c, n, nv, ndofs, heat);
     if (rc) return rc;
     const FamOut o{(uint64_t *)c->keys[0].p, heat ? (uint64_t *)c->fkeys[0].p : nullptr, c->vals[0].p,
