@@ -278,10 +278,14 @@ __global__ __launch_bounds__(SV_BLOCK) void k_cg_spmv(int64_t n, const uint32_t 
 
 // alpha = rz_old / pq; x += alpha p; r -= alpha q; z = M r; partials r.z (into
 // rz_new) and r.r
-// The iterate x is FP64 in both solves (TX = double): an FP32 iterate
-// absorbs the late, small corrections (x + alpha p rounds back to x), and the
-// mixed solve then needed 69 % more iterations than FP64 PCG on heat Q4
-// 1024^2 against 9 % with the FP64 iterate (numpy emulation of this code).
+// The FP64 solve updates its FP64 iterate here (TX = double).  The mixed
+// solve adds alpha p to an FP32 accumulator of the last few iterations' steps
+// (TX = float), which k_cg_flush folds into the FP64 iterate every CHECK
+// iterations: an FP32 iterate itself absorbs the late, small corrections
+// (x + alpha p rounds back to x; the mixed solve then needed 69 % more
+// iterations than FP64 PCG on heat Q4 1024^2 against 9 % with an FP64
+// iterate, numpy emulation of this code), while the accumulator holds only
+// corrections, so each keeps its FP32 relative precision.
 template <typename T, typename TX>
 __global__ __launch_bounds__(SV_BLOCK) void k_cg_update(int64_t n, const T *__restrict__ dinv, TX *__restrict__ x,
                                                         T *__restrict__ r, T *__restrict__ z,
@@ -310,6 +314,16 @@ __global__ __launch_bounds__(SV_BLOCK) void k_cg_update(int64_t n, const T *__re
     a = block_sum(a, sh);
     b = block_sum(b, sh);
     if (threadIdx.x == 0) { rz_new[blockIdx.x] = a; part_rr[blockIdx.x] = b; }
+}
+
+// x += xlo, xlo = 0: the mixed solve's accumulated steps into its FP64 iterate
+__global__ void k_cg_flush(int64_t n, double *__restrict__ x, float *__restrict__ xlo)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        x[i] += (double)xlo[i];
+        xlo[i] = 0.f;
+    }
 }
 
 // beta = rz_new / rz_old; p = z + beta p.  Workgroup 0 also leaves rz and
@@ -438,6 +452,7 @@ template <typename T>
 struct Vecs {
     double *x;  // the iterate: FP64 in both solves
     T *r, *z, *p, *q, *dinv;
+    float *xlo;  // mixed solve: the steps since the last k_cg_flush (null: x updated directly)
 };
 
 struct Sell {
@@ -473,8 +488,12 @@ void cg_step(hipStream_t st, int64_t n, const Sell &S, const Vecs<T> &v, const P
     else
         hipLaunchKernelGGL((k_cg_spmv<T, int32_t>), g, b, 0, st, n, S.off, (const int32_t *)S.col, sell_vals<T>(S), S.U,
                            v.p, v.q, P.pq);
-    hipLaunchKernelGGL((k_cg_update<T, double>), g, b, 0, st, n, v.dinv, v.x, v.r, v.z, v.p, v.q, P.pq, P.rz[par ^ 1],
-                       P.rz[par], P.rr);
+    if (v.xlo)
+        hipLaunchKernelGGL((k_cg_update<T, float>), g, b, 0, st, n, v.dinv, v.xlo, v.r, v.z, v.p, v.q, P.pq,
+                           P.rz[par ^ 1], P.rz[par], P.rr);
+    else
+        hipLaunchKernelGGL((k_cg_update<T, double>), g, b, 0, st, n, v.dinv, v.x, v.r, v.z, v.p, v.q, P.pq,
+                           P.rz[par ^ 1], P.rz[par], P.rr);
     hipLaunchKernelGGL((k_cg_dir<T>), g, b, 0, st, n, P.rz[par], P.rz[par ^ 1], P.rr, v.z, v.p, sc);
 }
 
@@ -507,10 +526,10 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
     // ---- scratch layout (8-byte words first, then 4-byte) -----------------
     // doubles: xfull[N], b[n], r64[n], dinv64[n], parts[5 G], sc[S_NS],
     //          FP64 vectors r z p q [4n] (FP64 solve only)
-    // floats:  r z p q dinv [5n] (mixed solve only; its iterate is the FP64 xfull)
+    // floats:  r z p q dinv xlo [6n] (mixed solve only; its iterate is the FP64 xfull + xlo)
     // uint32:  off[nslices + 1], len[max(n, nslices)], total[4]
     const size_t nd = (size_t)N + 3 * (size_t)n + 5 * (size_t)G + S_NS + (mixed ? 0 : 4 * (size_t)n);
-    const size_t nf = mixed ? 5 * (size_t)n : 0;
+    const size_t nf = mixed ? 6 * (size_t)n : 0;
     const size_t nu32 = (size_t)nslices + 1 + (size_t)std::max<int64_t>(n, nslices) + (size_t)n + 4;
     int rc = ensure(c, c->pcg, 8 * nd + 4 * nf + 4 * nu32 + 256);
     if (rc) return rc;
@@ -535,6 +554,7 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
         v32.x = xfull;  // the free part of the full vector is the iterate (FP64)
         v32.r = fp; fp += n; v32.z = fp; fp += n; v32.p = fp; fp += n; v32.q = fp; fp += n;
         v32.dinv = fp; fp += n;
+        v32.xlo = fp; fp += n;
     }
     uint32_t *up = (uint32_t *)fp;
     uint32_t *off = up; up += nslices + 1;
@@ -659,6 +679,7 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
             }
         }
     } else {
+        hipLaunchKernelGGL(k_fill<float>, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, v32.xlo, 0.f);
         reset(v32, 1);
         double rlast = rn;
         while (rn > target && it < maxit) {
@@ -669,6 +690,7 @@ int pcg_solve(elfel_hip_ctx *c, int64_t nfree, const double *F, double *x, int p
                 rrec = h[S_RR];
                 if (!(rrec > goal * goal) || it >= maxit) break;
                 for (int k = 0; k < CHECK; k++, par ^= 1) cg_step<float>(st, n, S, v32, P, par, sc);
+                hipLaunchKernelGGL(k_cg_flush, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, v32.x, v32.xlo);
                 HIPCHK(c, hipGetLastError());
                 it += CHECK;
             }
