@@ -228,7 +228,8 @@ int elfel_hip_geometry_stats(elfel_hip_ctx *ctx, double *min_det, int64_t *n_bad
  * matrix of the last assembly (CSR).  x holds the data values on entry
  * (x[nfree:]) and the solution on return.  precision = ELFEL_FP64: FP64 PCG;
  * ELFEL_MIXED: PCG with an FP32 matrix copy and FP32 r, z, p, q, the iterate
- * kept in FP64, and reliable updates (the FP64 residual replaces the FP32 one
+ * kept in FP64 (its steps gathered in FP32 over 8 iterations), and reliable
+ * updates (the FP64 residual replaces the FP32 one
  * at every 100x drop).  Convergence: ||b - A x|| / ||b|| <= tol in FP64 on
  * the free block.  *iters = PCG iterations, *outer = FP64 residual
  * replacements, *relres = the final FP64 relative residual. */
