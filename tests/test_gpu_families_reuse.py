@@ -186,3 +186,31 @@ def test_empty_mesh_reuse(fctx):
         ptr, ind, val = fctx.copy_sparse(base=0)[:3]
         assert nnz == 0 and ind.size == 0 and not ptr.any()
         assert np.array_equal(fctx.copy_vector(5), np.zeros(5))
+
+
+def test_reuse_any_coo_stream(fctx):
+    """The families' REUSE method keeps the sort of whatever stream the mesh
+    and dof map give: two nodes sharing a dof (a periodic tie), here also two
+    nodes of one element, so that an element block holds the same (row,
+    column) pair more than once, still equal the oracle's sparse() bit for bit."""
+    conn, xyz = _t4_mesh(4, seed=6)
+    Nn = xyz.shape[0]
+    dn = np.arange(1, Nn + 1)
+    dn[7] = dn[3]  # node 8 takes node 4's dof
+    dn[conn[0, 1] - 1] = dn[conn[0, 0] - 1]  # two nodes of element 1 share a dof
+    used = np.unique(dn)
+    remap = np.zeros(used.max() + 1, np.int64)
+    remap[used] = np.arange(1, used.size + 1)
+    dn = remap[dn]
+    n = used.size
+    fctx.set_mesh(0, conn, xyz)
+    fctx.set_dofs(0, dn, n)
+    for kind in (L.HEAT, L.MASS):
+        ok = O.HEAT if kind == L.HEAT else O.MASS
+        I, J, V, FI, FV = O.fam_scalar_coo(ok, O.T4, 1, conn, xyz, dn, kappa=1.0, Q=2.0)
+        ref = _ref(I, J, V, n, L.CSR)
+        F_ref = O.vec_assemble(FI, FV, n) if kind == L.HEAT else None
+        for prec in (L.FP64, L.MIXED):
+            fctx.assemble(kind, L.T4, 1, precision=prec, method=L.REUSE, kappa=1.0, Q=2.0)
+            assert fctx.assembly_info()[:2] == (L.REUSE, prec)
+            _check(fctx, ref, F_ref, prec, (kind, prec))
