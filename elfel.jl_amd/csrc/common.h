@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -108,7 +109,8 @@ struct Pattern {
     int maxu = 0;                // most distinct elements in one tile
     // the mesh does not fit the tiles (a node in more than 16 elements, a row
     // of more than 128 entries, a dof shared by two nodes, an element that
-    // repeats a dof): assemblies of this pattern run the plain FP64 method
+    // repeats a dof): assemblies of this pattern sum the runs of the plain
+    // method's stream in its cached sort order, in FP64 (stream_reuse)
     bool fallback = false;
     std::string fallback_note;
     // row-owner heat tiles: one record of rw words per row (rw 0: task tables),
@@ -259,6 +261,25 @@ int plain_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
 // fkeys[0]/fvals[0]) into the CSR/CSC result and the load vector.
 // vec: the problem has a load vector (nv pairs, possibly none).
 int plain_buffers(elfel_hip_ctx *c, int64_t n, int64_t nv, int64_t ndofs, bool vec);
+// ---- families.hip: the REUSE method over a cached sort of the COO stream -----
+// What an element kernel writes: keys of the matrix and load-vector streams
+// (null: values only) and their values (FP64, or FP32 when asked for f32).
+struct StreamOut {
+    uint64_t *keys, *fkeys;
+    void *vals, *fvals;
+};
+typedef std::function<int(const StreamOut &, bool f32)> StreamElements;
+// Assembles pb's problem by summing the runs of its stream through the cached
+// sort permutation (c->fpat; built on the first call for this mesh, dof map
+// and weak form): n matrix and nv load-vector stream entries (nv 0: no load
+// vector), ndofs rows, keys with minor_bits of minor index.  The result: the
+// sparse matrix (and load vector) in the context, FP64 values equal to the
+// plain method's bit for bit.
+int stream_reuse(elfel_hip_ctx *c, const elfel_hip_problem *pb, int64_t n, int64_t nv, int64_t ndofs, int minor_bits,
+                 bool f32, const StreamElements &elements);
+// The 2D weak forms (plain.hip) on stream_reuse, FP64: the REUSE method's
+// path for meshes its tiles do not hold.
+int plain_stream_reuse(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf);
 int plain_finish(elfel_hip_ctx *c, int layout, int64_t n, int64_t nv, int64_t ndofs, int minor_bits, bool vec);
 
 // ---- families.hip (element families, plain FP64 method) -------------------------

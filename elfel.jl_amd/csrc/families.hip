@@ -536,10 +536,7 @@ bool reddy_pair(int vfe, int pfe)
 
 // The element kernel's outputs: keys (or null) and values of the COO stream
 // and, for heat, of the load-vector stream.
-struct FamOut {
-    uint64_t *keys, *fkeys;
-    void *vals, *fvals;
-};
+using FamOut = StreamOut;
 
 template <int KIND, int M, int NB, int NQ, typename VT>
 int launch_scalar(elfel_hip_ctx *c, const FamTab &tab, double kappa, double Q, int minor_bits, int csc,
@@ -759,17 +756,19 @@ int fam_elements(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &ta
         }                                                                                                  \
     } while (0)
 
-// Symbolic phase of the REUSE method on the families (c->fpat), when the
-// cached one does not describe this mesh, dof map and weak form.
-int fam_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &tab, int64_t n, int64_t nv,
-                 int64_t ndofs, int minor_bits, int csc, bool heat)
+// Symbolic phase of the stream REUSE method (c->fpat), when the cached one
+// does not describe this mesh, dof map and weak form.
+int stream_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int64_t n, int64_t nv, int64_t ndofs,
+                    int minor_bits, const StreamElements &elements)
 {
     FamPattern &P = c->fpat;
+    const bool heat = pb->kind == ELFEL_HEAT;
     const Field &F0 = c->f[0], &F1 = c->f[1];
     const bool reddy = pb->kind == ELFEL_STOKES_REDDY;
+    const bool two = reddy || pb->kind == ELFEL_STOKES_GEN || pb->kind == ELFEL_STOKES_VECLAP;
     const int pfe = reddy ? pb->fe_p : -1;
-    const uint64_t key[6] = {F0.mesh_version, F0.dof_version, F0.edof_version, reddy ? F1.mesh_version : 0,
-                             reddy ? F1.dof_version : 0, reddy ? F1.edof_version : 0};
+    const uint64_t key[6] = {F0.mesh_version, F0.dof_version, F0.edof_version, two ? F1.mesh_version : 0,
+                             two ? F1.dof_version : 0, two ? F1.edof_version : 0};
     if (P.valid && P.kind == pb->kind && P.fe == pb->fe && P.fe_p == pfe && P.layout == pb->layout && P.n == n &&
         P.nv == nv && P.ndofs == ndofs && memcmp(P.key, key, sizeof key) == 0)
         return 0;
@@ -786,8 +785,8 @@ int fam_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &ta
     uint64_t *fk0 = (uint64_t *)c->fkeys[0].p, *fk1 = (uint64_t *)c->fkeys[1].p;
     uint64_t *fv0 = (uint64_t *)c->fvals[0].p, *fv1 = (uint64_t *)c->fvals[1].p;
     // the stream's keys (its values land in coo, unused here)
-    const FamOut o{k0, heat ? fk0 : nullptr, P.coo.p, heat ? P.fcoo.p : nullptr};
-    if ((rc = fam_elements(c, pb, tab, minor_bits, csc, o, false))) return rc;
+    const StreamOut o{k0, heat ? fk0 : nullptr, P.coo.p, heat ? P.fcoo.p : nullptr};
+    if ((rc = elements(o, false))) return rc;
     hipEvent_t ev = nullptr;
     tm_begin(c, ELFEL_PH_SORT, &ev);
     FAM_LAUNCH(k_fam_iota, n, v0, n);
@@ -822,19 +821,21 @@ int fam_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &ta
     return 0;
 }
 
+}  // namespace
+
 // Numeric phase: element values only, then every run summed through the
 // cached permutation.
-int fam_reuse(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &tab, int64_t n, int64_t nv,
-              int64_t ndofs, int minor_bits, int csc, bool heat)
+int stream_reuse(elfel_hip_ctx *c, const elfel_hip_problem *pb, int64_t n, int64_t nv, int64_t ndofs, int minor_bits,
+                 bool f32, const StreamElements &elements)
 {
-    int rc = fam_symbolic(c, pb, tab, n, nv, ndofs, minor_bits, csc, heat);
+    const bool heat = pb->kind == ELFEL_HEAT;
+    int rc = stream_symbolic(c, pb, n, nv, ndofs, minor_bits, elements);
     if (rc) return rc;
     const FamPattern &P = c->fpat;
-    const bool f32 = pb->precision == ELFEL_MIXED;
     if ((rc = ensure(c, c->val, 8 * std::max<int64_t>(P.nnz, 1)))) return rc;
     if (heat && (rc = ensure(c, c->F, 8 * std::max<int64_t>(ndofs, 1)))) return rc;
-    const FamOut o{nullptr, nullptr, P.coo.p, heat ? P.fcoo.p : nullptr};
-    if ((rc = fam_elements(c, pb, tab, minor_bits, csc, o, f32))) return rc;
+    const StreamOut o{nullptr, nullptr, P.coo.p, heat ? P.fcoo.p : nullptr};
+    if ((rc = elements(o, f32))) return rc;
     hipEvent_t ev = nullptr;
     tm_begin(c, ELFEL_PH_REDUCE, &ev);
     const int64_t nf = heat ? ndofs : 0;
@@ -863,8 +864,6 @@ int fam_reuse(elfel_hip_ctx *c, const elfel_hip_problem *pb, const FamTab &tab, 
 }
 
 #undef FAM_LAUNCH
-
-}  // namespace
 
 bool family_problem(const elfel_hip_problem *pb)
 {
@@ -942,7 +941,8 @@ int family_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb)
     // the cached permutation and run starts are int32: a longer stream runs the
     // plain FP64 method (its sort carries 64-bit positions)
     if (pb->method == ELFEL_REUSE && n < INT32_MAX && nv < INT32_MAX)
-        return fam_reuse(c, pb, tab, n, nv, ndofs, minor_bits, csc, heat);
+        return stream_reuse(c, pb, n, nv, ndofs, minor_bits, pb->precision == ELFEL_MIXED,
+                            [&](const StreamOut &o, bool f32) { return fam_elements(c, pb, tab, minor_bits, csc, o, f32); });
     if (pb->method == ELFEL_REUSE) {
         c->used_method = ELFEL_SORT;
         c->used_precision = ELFEL_FP64;

@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void k_elem_coo(int64_t E, const double2 *__re
         ent.column(g, D, j);
 #pragma unroll
         for (int i = 0; i < ND; i++) {
-            keys[o] = key(d[i], d[j]);
+            if (keys) keys[o] = key(d[i], d[j]);
             vals[o] = ent.value(g, kap, cf.mu, i, j);
             o++;
         }
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) void k_elem_coo(int64_t E, const double2 *__re
         for (int jp = 0; jp < 3; jp++)
 #pragma unroll
             for (int i = 0; i < ND; i++) {
-                keys[o] = key(d[i], d[ND + jp]);
+                if (keys) keys[o] = key(d[i], d[ND + jp]);
                 vals[o] = kup[i][jp];
                 o++;
             }
@@ -121,17 +121,17 @@ __global__ __launch_bounds__(256) void k_elem_coo(int64_t E, const double2 *__re
         for (int b = 0; b < ND; b++)
 #pragma unroll
             for (int a = 0; a < 3; a++) {
-                keys[o] = key(d[ND + a], d[b]);
+                if (keys) keys[o] = key(d[ND + a], d[b]);
                 vals[o] = kup[b][a];
                 o++;
             }
     }
     if constexpr (KIND == ELFEL_HEAT) {
-        if (fkeys) {
+        if (fvals) {
             const double Qe = cf.Q_e ? cf.Q_e[padded(e, cf.cgap)] : cf.Q;
 #pragma unroll
             for (int j = 0; j < ND; j++) {
-                fkeys[e * ND + j] = (uint64_t)d[j];
+                if (fkeys) fkeys[e * ND + j] = (uint64_t)d[j];
                 fvals[e * ND + j] = fe_value<double, NEN, NQ>(tab, g, Qe, j);
             }
         }
@@ -291,6 +291,36 @@ int plain_finish(elfel_hip_ctx *c, int layout, int64_t n, int64_t nv, int64_t nd
     c->layout = layout;
     c->result_src = RES_PLAIN;
     return 0;
+}
+
+// The 2D weak forms on the stream REUSE method (FP64): the element kernel of
+// the plain method writes keys on the first call, values only afterwards, and
+// every entry is the sum of its run in the cached sort order.
+int plain_stream_reuse(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf)
+{
+    const Field &F = c->f[0];
+    const bool stokes = pb->kind == ELFEL_STOKES_GEN || pb->kind == ELFEL_STOKES_VECLAP;
+    plain_fn fn = pick_plain(pb->kind, pb->fe, tab.nq);
+    if (!fn) return set_err(c, ELFEL_ENOTSUP, "unsupported (kind, element, rule) for the plain method");
+    const int64_t nd = (int64_t)F.nen * F.ncomp;
+    const int64_t n = F.E * (nd * nd + (stokes ? 2 * nd * 3 : 0));
+    const bool heat = pb->kind == ELFEL_HEAT;
+    const int64_t nv = heat ? F.E * nd : 0;
+    const int64_t ndofs = c->nrow;
+    const int minor_bits = bits_for(ndofs);
+    const int csc = pb->layout == ELFEL_CSC ? 1 : 0;
+    elfel_hip_problem p = *pb;
+    p.precision = ELFEL_FP64;
+    if (n >= INT32_MAX || nv >= INT32_MAX) {  // the cached permutation is int32
+        p.method = ELFEL_SORT;
+        c->used_method = ELFEL_SORT;
+        c->used_note += " (a COO stream of 2^31 entries or more: the plain FP64 method)";
+        c->nnz = -1;
+        return plain_assemble(c, &p, tab, cf);
+    }
+    return stream_reuse(c, &p, n, nv, ndofs, minor_bits, false, [&](const StreamOut &o, bool) {
+        return fn(c, tab, cf, minor_bits, csc, o.keys, (double *)o.vals, o.fkeys, (double *)o.fvals);
+    });
 }
 
 int plain_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &tab, const Coef &cf)

@@ -876,10 +876,10 @@ static int build_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, const Q
     rc = tile_tables(c, stokes, F0, F1, n, hptr, hind, cap, block, T, env_int("ELFEL_HIP_PATCH_TILES", 1) != 0,
                      pb->kind == ELFEL_HEAT && env_int("ELFEL_HIP_HEAT_ROWS", 1) != 0);
     if (rc == ELFEL_ENOTSUP) {
-        // the mesh does not fit the tiles: this pattern's assemblies run the
-        // plain FP64 method, whose result the pattern build just produced
+        // the mesh does not fit the tiles: this pattern's assemblies sum the
+        // runs of the plain method's stream in its cached sort order (FP64)
         P.fallback = true;
-        P.fallback_note = "plain FP64 method: " + c->err;
+        P.fallback_note = "FP64 runs of the cached sort order: " + c->err;
         c->err.clear();
     } else if (rc) {
         return rc;
@@ -932,14 +932,12 @@ int reuse_assemble(elfel_hip_ctx *c, const elfel_hip_problem *pb, const QTable &
         if (rc) return rc;
     }
     if (P.fallback) {
-        elfel_hip_problem p2 = *pb;
-        p2.method = ELFEL_SORT;
-        p2.precision = ELFEL_FP64;
-        c->used_method = ELFEL_SORT;
+        // the mesh does not fit the tiles: its stream is summed run by run in
+        // the cached sort order (FP64, the plain method's bits)
+        c->used_method = ELFEL_REUSE;
         c->used_precision = ELFEL_FP64;
         c->used_note = P.fallback_note;
-        c->nnz = -1;
-        return plain_assemble(c, &p2, tab, cf);
+        return plain_stream_reuse(c, pb, tab, cf);
     }
     reuse_launch_fn fn = P.rw ? (mixed ? pick_heat_rows32(pb->fe, tab.nq, csc) : pick_heat_rows64(pb->fe, tab.nq, csc))
                               : (mixed ? pick_reuse32(pb->kind, pb->fe, tab.nq, csc, block)

@@ -1186,7 +1186,7 @@ int device_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int block)
     if (hv[1]) {
         P.fallback = true;
         const uint32_t f = hv[1];
-        P.fallback_note = std::string("plain FP64 method: ") +
+        P.fallback_note = std::string("FP64 runs of the cached sort order: ") +
                           ((f & SYF_SHARED) ? "a dof shared by two nodes or components"
                            : (f & SYF_REPEAT) ? "an element with a repeated dof"
                                               : "node too highly connected for the reuse tiles");
@@ -1210,7 +1210,7 @@ int device_symbolic(elfel_hip_ctx *c, const elfel_hip_problem *pb, int block)
     HIPCHK(c, hipStreamSynchronize(st));
     if (hn[1]) {
         P.fallback = true;
-        P.fallback_note = "plain FP64 method: node too highly connected for the reuse tiles";
+        P.fallback_note = "FP64 runs of the cached sort order: node too highly connected for the reuse tiles";
         P.nnz = nnz;
         return 0;
     }
@@ -1380,7 +1380,7 @@ int device_symbolic_heat(elfel_hip_ctx *c, const elfel_hip_problem *pb, bool pat
     P.fallback = false;
     if (flags & (SYF_SHARED | SYF_REPEAT)) {
         P.fallback = true;
-        P.fallback_note = std::string("plain FP64 method: ") +
+        P.fallback_note = std::string("FP64 runs of the cached sort order: ") +
                           ((flags & SYF_SHARED) ? "a dof shared by two nodes or components" : "an element with a repeated dof");
         P.nnz = nnz;
         return 0;

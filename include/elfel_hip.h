@@ -193,8 +193,9 @@ int elfel_hip_assemble(elfel_hip_ctx *ctx, const elfel_hip_problem *pb, int64_t 
  * at most 16 elements, rows of at most 128 entries, each dof owned by one
  * (node, component) and elements whose dofs are distinct; a mesh or dof map
  * that breaks one of these (valid input: the reference's sparse() assembles any
- * COO stream, src/Assemblers.jl:97-123) is assembled by the plain FP64 method
- * instead, with the same pattern and the reference's values.  method and
+ * COO stream, src/Assemblers.jl:97-123) is assembled by summing the plain
+ * method's stream in its sort order, cached on the first call, in FP64: the
+ * same pattern and the reference's values.  method and
  * precision receive the method and arithmetic used; note (len bytes, may be
  * null) the reason when they differ from the request.  No counterpart in the
  * reference, which has one method. */

@@ -90,8 +90,8 @@ def test_device_symbolic_stokes(form, perm):
 
 
 def test_device_symbolic_shared_dof_falls_back():
-    """Two nodes with one dof (a periodic tie): the plain FP64 method runs,
-    with the oracle's pattern and values."""
+    """Two nodes with one dof (a periodic tie): the runs of the cached sort
+    order are summed in FP64, with the oracle's pattern and values."""
     mesh, dn, n = _elast_mesh(6, "plain")
     dn = dn.copy()
     dn[5] = dn[4]  # node 6 takes node 5's dofs
@@ -107,7 +107,7 @@ def test_device_symbolic_shared_dof_falls_back():
     ctx.set_dofs(0, dn, n)
     ctx.assemble(L.ELAST, L.T6, 3, precision=L.MIXED, method=L.REUSE, layout=L.CSC, D=D)
     m, p, note = ctx.assembly_info()
-    assert (m, p) == (L.SORT, L.FP64) and "shared" in note, note
+    assert (m, p) == (L.REUSE, L.FP64) and "shared" in note, note
     ptr, ind, val = ctx.copy_sparse(base=0)[:3]
     assert np.array_equal(ptr, ref[0]) and np.array_equal(ind, ref[1]) and np.array_equal(val, ref[2])
     ctx.close()
