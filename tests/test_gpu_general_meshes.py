@@ -4,9 +4,10 @@ state checks.
 The reference's sparse() assembles any COO stream (src/Assemblers.jl:97-123),
 so the library must too.  The REUSE method's tiles need every node in at most
 16 elements, rows of at most 128 entries and each dof owned by one (node,
-component); a mesh that breaks one of these is assembled by the plain FP64
-method instead (elfel_hip_assembly_info says so), and the result equals the
-oracle's bit for bit (tolerance 0).
+component); on a mesh that breaks one of these, REUSE caches the plain
+method's sort order instead and sums the runs of the element stream in FP64
+(elfel_hip_assembly_info says so), and the result equals the oracle's bit for
+bit (tolerance 0).
 """
 import numpy as np
 import pytest
@@ -120,6 +121,64 @@ def test_shared_dof_two_nodes(gctx):
     gctx.set_dofs(0, np.arange(1, Nn + 1), Nn)
     gctx.assemble(L.HEAT, L.T3, 1, precision=L.MIXED, method=L.REUSE, kappa=1.3, Q=-2.0)
     assert gctx.assembly_info()[:2] == (L.REUSE, L.MIXED)
+
+
+@pytest.mark.parametrize("N", [8, 600])
+def test_q4_shared_dof_two_nodes(gctx, N):
+    """Heat Q4 (2x2 Gauss, per-element coefficients) with the right edge tied
+    to the left edge: the row-owner tiles do not hold a dof of two nodes; the
+    cached sort order's FP64 runs equal the oracle's matrix and load vector
+    bit for bit (600^2: past the device builder's 2^18-row threshold)."""
+    conn, xy = O.q4block(1.0, 1.0, N, N)
+    Nn, E = xy.shape[0], conn.shape[0]
+    node = np.arange(Nn).reshape(N + 1, N + 1)
+    owner = np.arange(Nn)
+    owner[node[:, N]] = node[:, 0]
+    uniq, inv = np.unique(owner, return_inverse=True)
+    dn = inv + 1
+    n = uniq.size
+    rng = np.random.default_rng(8)
+    kap, Qe = np.exp(rng.normal(size=E)), rng.uniform(-1.0, 1.0, E)
+    gctx.set_mesh(0, conn, xy)
+    gctx.set_dofs(0, dn, n)
+    gctx.set_coefficients(E, kap, Qe, None)
+    ref = O.assemble(O.make_problem(O.HEAT, O.Q4, 2, conn, xy, dn.reshape(-1, 1), kappa_e=kap, Q_e=Qe), n,
+                     csc=False)
+    gctx.assemble(L.HEAT, L.Q4, 2, precision=L.MIXED, method=L.REUSE, layout=L.CSR)
+    _same(gctx.copy_sparse(base=0), ref)
+    assert np.array_equal(gctx.copy_vector(n), ref[3])
+    m, p, note = gctx.assembly_info()
+    assert (m, p) == (L.REUSE, L.FP64) and "shared" in note, note
+
+
+@pytest.mark.parametrize("form", [L.STOKES_GEN, L.STOKES_VECLAP])
+def test_stokes_shared_velocity_dof(gctx, form):
+    """Taylor-Hood T6/T3 with two velocity nodes tied to one pair of dofs:
+    the two-field problem's runs (kuu, kup and kup^T in one stream) summed in
+    the cached sort order equal the oracle's sparse() bit for bit; then the
+    pressure map changes and the cached order is rebuilt."""
+    from elfel_hip import workloads as W
+    cfg = W.stokes_t6t3(7)
+    u, p = cfg["ufesp"], cfg["pfesp"]
+    du, dp = u.dofnums.copy(), p.dofnums.copy()
+    du[9] = du[8]  # node 10 takes node 9's two dofs
+    used = np.unique(np.concatenate([du.ravel(), dp.ravel()]))
+    remap = np.zeros(used.max() + 1, np.int64)
+    remap[used] = np.arange(1, used.size + 1)
+    du, dp, n = remap[du], remap[dp], used.size
+    okind = O.STOKES_GEN if form == L.STOKES_GEN else O.STOKES_VECLAP
+    gctx.set_mesh(0, u.mesh.conn, u.mesh.xy)
+    gctx.set_dofs(0, du, n)
+    gctx.set_mesh(1, p.mesh.conn, p.mesh.xy)
+    for perm in (False, True):
+        dpp = dp if not perm else dp[np.random.default_rng(4).permutation(dp.shape[0])]
+        gctx.set_dofs(1, dpp, n)
+        ref = O.assemble(O.make_problem(okind, O.T6, 3, u.mesh.conn, u.mesh.xy, du, conn_p=p.mesh.conn,
+                                        dofnums_p=dpp, D=cfg["D"], mu=1.0), n, csc=True)
+        gctx.assemble(form, L.T6, 3, precision=L.MIXED, method=L.REUSE, layout=L.CSC, D=cfg["D"], mu=1.0)
+        _same(gctx.copy_sparse(base=0), ref)
+        m, prec, note = gctx.assembly_info()
+        assert (m, prec) == (L.REUSE, L.FP64) and "shared" in note, note
 
 
 def test_new_mesh_invalidates_dof_map(gctx):
