@@ -13,8 +13,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <exception>
 #include <functional>
+#include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/elfel_hip.h"
@@ -210,6 +213,52 @@ int set_err(elfel_hip_ctx *c, int code, const std::string &msg);
         if (e_ != hipSuccess)                                                  \
             return set_err((c), ELFEL_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// No C++ exception crosses the C ABI or ends a host worker thread: the host
+// tables are std containers, and a failed host allocation (std::bad_alloc) or
+// any other exception becomes an error code and message.
+inline int exception_code(std::string *msg) noexcept
+{
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        if (msg) *msg = "host allocation failed";
+        return ELFEL_ENOMEM;
+    } catch (const std::exception &e) {
+        if (msg) *msg = std::string("host exception: ") + e.what();
+        return ELFEL_EINVAL;
+    } catch (...) {
+        if (msg) *msg = "host exception";
+        return ELFEL_EINVAL;
+    }
+}
+template <typename F>
+int host_guard(elfel_hip_ctx *c, F &&f) noexcept
+{
+    try {
+        return f();
+    } catch (...) {
+        std::string m;
+        const int code = exception_code(&m);
+        return set_err(c, code, m);
+    }
+}
+// Runs `before` (wakes waiting threads) and joins a set of host threads on
+// every way out of a scope, an exception included: no joinable std::thread is
+// ever destroyed.
+template <typename F>
+struct ThreadJoiner {
+    std::vector<std::thread> &ts;
+    F before;
+    ~ThreadJoiner()
+    {
+        before();
+        for (auto &t : ts)
+            if (t.joinable()) t.join();
+    }
+};
+template <typename F>
+ThreadJoiner<F> join_on_exit(std::vector<std::thread> &ts, F before) { return ThreadJoiner<F>{ts, before}; }
 
 // Grows a buffer to at least `bytes` (contents are not preserved).
 int ensure(elfel_hip_ctx *c, Buf &b, size_t bytes);

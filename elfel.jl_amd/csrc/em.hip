@@ -160,50 +160,52 @@ __global__ __launch_bounds__(256) void k_ke_mfma(int64_t E, const int32_t *__res
 extern "C" int elfel_hip_element_matrices(elfel_hip_ctx *c, const elfel_hip_problem *pb, int method, float *ke,
                                           int64_t n, double *ms)
 {
-    if (!c) return ELFEL_EINVAL;
-    c->err.clear();
-    (void)hipSetDevice(c->device);
-    if (!pb || (method != ELFEL_EM_VALU && method != ELFEL_EM_MFMA))
-        return set_err(c, ELFEL_EINVAL, "elfel_hip_element_matrices: bad arguments");
-    if (pb->kind != ELFEL_ELAST || pb->fe != ELFEL_T6 || pb->rule != 3 || pb->precision != ELFEL_MIXED)
-        return set_err(c, ELFEL_ENOTSUP, "element matrices: plane elasticity on T6, 3-point rule, mixed arithmetic");
-    const Field &F = c->f[0];
-    if (!F.has_mesh || F.nen != 6 || F.sdim != 2) return set_err(c, ELFEL_ESTATE, "a T6 mesh in field 0 first");
-    const int64_t E = F.E;
-    const bool copy = ke != nullptr;  // ke NULL with n 0: the kernel alone, timed, results left on the device
-    if ((copy && n != E * 144) || (!copy && n != 0))
-        return set_err(c, ELFEL_EINVAL, "element matrices: ke must hold 144 E values");
-    if (E * 144 >= INT32_MAX * 8LL) return set_err(c, ELFEL_ENOTSUP, "element matrices: too many elements");
-    int rc = ensure(c, c->geom, sizeof(float) * 144 * (size_t)std::max<int64_t>(E, 1));
-    if (rc) return rc;
-    Coef cf{};
-    for (int i = 0; i < 9; i++) cf.D[i] = pb->D[i];
-    hipEvent_t a, b;
-    HIPCHK(c, hipEventCreate(&a));
-    HIPCHK(c, hipEventCreate(&b));
-    HIPCHK(c, hipEventRecord(a, c->stream));
-    if (E > 0) {
-        if (method == ELFEL_EM_VALU)
-            hipLaunchKernelGGL(k_ke_valu, dim3(ceil_div(E, 256)), dim3(256), 0, c->stream, E, (const int32_t *)F.conn.p,
-                               (const double2 *)F.xy.p, cf, (float *)c->geom.p);
-        else  // 16 elements per wave, 64 per workgroup
-            hipLaunchKernelGGL(k_ke_mfma, dim3(ceil_div(E, 64)), dim3(256), 0, c->stream, E, (const int32_t *)F.conn.p,
-                               (const double2 *)F.xy.p, cf, (float *)c->geom.p);
-        HIPCHK(c, hipGetLastError());
-    }
-    HIPCHK(c, hipEventRecord(b, c->stream));
-    HIPCHK(c, hipEventSynchronize(b));
-    float t = 0.f;
-    hipEventElapsedTime(&t, a, b);
-    hipEventDestroy(a);
-    hipEventDestroy(b);
-    if (ms) *ms = t;
-    // element-major out of the device's entry-major layout
-    if (copy && E > 0) {
-        std::vector<float> h((size_t)E * 144);
-        HIPCHK(c, hipMemcpy(h.data(), c->geom.p, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
-        for (int64_t m = 0; m < 144; m++)
-            for (int64_t e = 0; e < E; e++) ke[e * 144 + m] = h[(size_t)(m * E + e)];
-    }
-    return ELFEL_OK;
+    return host_guard(c, [&]() -> int {
+        if (!c) return ELFEL_EINVAL;
+        c->err.clear();
+        (void)hipSetDevice(c->device);
+        if (!pb || (method != ELFEL_EM_VALU && method != ELFEL_EM_MFMA))
+            return set_err(c, ELFEL_EINVAL, "elfel_hip_element_matrices: bad arguments");
+        if (pb->kind != ELFEL_ELAST || pb->fe != ELFEL_T6 || pb->rule != 3 || pb->precision != ELFEL_MIXED)
+            return set_err(c, ELFEL_ENOTSUP, "element matrices: plane elasticity on T6, 3-point rule, mixed arithmetic");
+        const Field &F = c->f[0];
+        if (!F.has_mesh || F.nen != 6 || F.sdim != 2) return set_err(c, ELFEL_ESTATE, "a T6 mesh in field 0 first");
+        const int64_t E = F.E;
+        const bool copy = ke != nullptr;  // ke NULL with n 0: the kernel alone, timed, results left on the device
+        if ((copy && n != E * 144) || (!copy && n != 0))
+            return set_err(c, ELFEL_EINVAL, "element matrices: ke must hold 144 E values");
+        if (E * 144 >= INT32_MAX * 8LL) return set_err(c, ELFEL_ENOTSUP, "element matrices: too many elements");
+        int rc = ensure(c, c->geom, sizeof(float) * 144 * (size_t)std::max<int64_t>(E, 1));
+        if (rc) return rc;
+        Coef cf{};
+        for (int i = 0; i < 9; i++) cf.D[i] = pb->D[i];
+        hipEvent_t a, b;
+        HIPCHK(c, hipEventCreate(&a));
+        HIPCHK(c, hipEventCreate(&b));
+        HIPCHK(c, hipEventRecord(a, c->stream));
+        if (E > 0) {
+            if (method == ELFEL_EM_VALU)
+                hipLaunchKernelGGL(k_ke_valu, dim3(ceil_div(E, 256)), dim3(256), 0, c->stream, E, (const int32_t *)F.conn.p,
+                                   (const double2 *)F.xy.p, cf, (float *)c->geom.p);
+            else  // 16 elements per wave, 64 per workgroup
+                hipLaunchKernelGGL(k_ke_mfma, dim3(ceil_div(E, 64)), dim3(256), 0, c->stream, E, (const int32_t *)F.conn.p,
+                                   (const double2 *)F.xy.p, cf, (float *)c->geom.p);
+            HIPCHK(c, hipGetLastError());
+        }
+        HIPCHK(c, hipEventRecord(b, c->stream));
+        HIPCHK(c, hipEventSynchronize(b));
+        float t = 0.f;
+        hipEventElapsedTime(&t, a, b);
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        if (ms) *ms = t;
+        // element-major out of the device's entry-major layout
+        if (copy && E > 0) {
+            std::vector<float> h((size_t)E * 144);
+            HIPCHK(c, hipMemcpy(h.data(), c->geom.p, sizeof(float) * h.size(), hipMemcpyDeviceToHost));
+            for (int64_t m = 0; m < 144; m++)
+                for (int64_t e = 0; e < E; e++) ke[e * 144 + m] = h[(size_t)(m * E + e)];
+        }
+        return ELFEL_OK;
+    });
 }

@@ -174,132 +174,140 @@ int finish_field_mesh(elfel_hip_ctx *c, Field &f, int nen, int64_t E, int64_t Nn
 extern "C" int elfel_hip_block_mesh(elfel_hip_ctx *c, int field, int fe, double A, double B, int64_t nA, int64_t nB,
                                     int64_t *E_out, int64_t *Nn_out)
 {
-    if (!c) return ELFEL_EINVAL;
-    c->err.clear();
-    (void)hipSetDevice(c->device);
-    if (field < 0 || field > 1 || nA < 1 || nB < 1 || (fe != ELFEL_T3 && fe != ELFEL_Q4 && fe != ELFEL_T6))
-        return set_err(c, ELFEL_EINVAL, "elfel_hip_block_mesh: bad arguments");
-    const int nen = fe == ELFEL_T3 ? 3 : fe == ELFEL_Q4 ? 4 : 6;
-    const int64_t cells = nA * nB;
-    const int64_t E = fe == ELFEL_Q4 ? cells : 2 * cells;
-    const int64_t Nv = (nA + 1) * (nB + 1);
-    const int64_t Nn = fe == ELFEL_T6 ? Nv + nA * (nB + 1) + (nA + 1) * nB + cells : Nv;
-    if (E * nen >= INT32_MAX || Nn >= INT32_MAX) return set_err(c, ELFEL_ENOTSUP, "mesh beyond int32 indexing");
-    Field &f = c->f[field];
-    f.has_mesh = false;
-    int rc;
-    if ((rc = ensure(c, f.conn, 4 * (size_t)(E * nen))) || (rc = ensure(c, f.xy, 16 * (size_t)Nn))) return rc;
-    hipLaunchKernelGGL(k_block_nodes, dim3(ceil_div(Nv, 256)), dim3(256), 0, c->stream, nA, nB, A, B,
-                       (double2 *)f.xy.p);
-    const dim3 g(ceil_div(cells, 256)), b(256);
-    if (fe == ELFEL_T3)
-        hipLaunchKernelGGL(k_block_cells<ELFEL_T3>, g, b, 0, c->stream, nA, nB, A, B, (int32_t *)f.conn.p, (double2 *)f.xy.p);
-    else if (fe == ELFEL_Q4)
-        hipLaunchKernelGGL(k_block_cells<ELFEL_Q4>, g, b, 0, c->stream, nA, nB, A, B, (int32_t *)f.conn.p, (double2 *)f.xy.p);
-    else
-        hipLaunchKernelGGL(k_block_cells<ELFEL_T6>, g, b, 0, c->stream, nA, nB, A, B, (int32_t *)f.conn.p, (double2 *)f.xy.p);
-    HIPCHK(c, hipGetLastError());
-    if ((rc = finish_field_mesh(c, f, nen, E, Nn))) return rc;
-    if (E_out) *E_out = E;
-    if (Nn_out) *Nn_out = Nn;
-    return ELFEL_OK;
+    return host_guard(c, [&]() -> int {
+        if (!c) return ELFEL_EINVAL;
+        c->err.clear();
+        (void)hipSetDevice(c->device);
+        if (field < 0 || field > 1 || nA < 1 || nB < 1 || (fe != ELFEL_T3 && fe != ELFEL_Q4 && fe != ELFEL_T6))
+            return set_err(c, ELFEL_EINVAL, "elfel_hip_block_mesh: bad arguments");
+        const int nen = fe == ELFEL_T3 ? 3 : fe == ELFEL_Q4 ? 4 : 6;
+        const int64_t cells = nA * nB;
+        const int64_t E = fe == ELFEL_Q4 ? cells : 2 * cells;
+        const int64_t Nv = (nA + 1) * (nB + 1);
+        const int64_t Nn = fe == ELFEL_T6 ? Nv + nA * (nB + 1) + (nA + 1) * nB + cells : Nv;
+        if (E * nen >= INT32_MAX || Nn >= INT32_MAX) return set_err(c, ELFEL_ENOTSUP, "mesh beyond int32 indexing");
+        Field &f = c->f[field];
+        f.has_mesh = false;
+        int rc;
+        if ((rc = ensure(c, f.conn, 4 * (size_t)(E * nen))) || (rc = ensure(c, f.xy, 16 * (size_t)Nn))) return rc;
+        hipLaunchKernelGGL(k_block_nodes, dim3(ceil_div(Nv, 256)), dim3(256), 0, c->stream, nA, nB, A, B,
+                           (double2 *)f.xy.p);
+        const dim3 g(ceil_div(cells, 256)), b(256);
+        if (fe == ELFEL_T3)
+            hipLaunchKernelGGL(k_block_cells<ELFEL_T3>, g, b, 0, c->stream, nA, nB, A, B, (int32_t *)f.conn.p, (double2 *)f.xy.p);
+        else if (fe == ELFEL_Q4)
+            hipLaunchKernelGGL(k_block_cells<ELFEL_Q4>, g, b, 0, c->stream, nA, nB, A, B, (int32_t *)f.conn.p, (double2 *)f.xy.p);
+        else
+            hipLaunchKernelGGL(k_block_cells<ELFEL_T6>, g, b, 0, c->stream, nA, nB, A, B, (int32_t *)f.conn.p, (double2 *)f.xy.p);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = finish_field_mesh(c, f, nen, E, Nn))) return rc;
+        if (E_out) *E_out = E;
+        if (Nn_out) *Nn_out = Nn;
+        return ELFEL_OK;
+    });
 }
 
 extern "C" int elfel_hip_t6_to_t3(elfel_hip_ctx *c, int src, int dst, int64_t *Nn_out)
 {
-    if (!c) return ELFEL_EINVAL;
-    c->err.clear();
-    (void)hipSetDevice(c->device);
-    if (src < 0 || src > 1 || dst < 0 || dst > 1 || src == dst)
-        return set_err(c, ELFEL_EINVAL, "elfel_hip_t6_to_t3: bad fields");
-    Field &s = c->f[src], &d = c->f[dst];
-    if (!s.has_mesh || s.nen != 6 || s.sdim != 2) return set_err(c, ELFEL_ESTATE, "elfel_hip_t6_to_t3: source field has no T6 mesh");
-    const int64_t E = s.E, Nn6 = s.Nn;
-    int rc;
-    // flags (scanned in place), a copy of the flags, the total
-    if ((rc = ensure(c, c->flags, 4 * (size_t)(2 * Nn6 + 2)))) return rc;
-    uint32_t *num = (uint32_t *)c->flags.p, *isc = num + Nn6 + 1, *total = isc + Nn6;
-    hipLaunchKernelGGL(k_zero_u32, dim3(ceil_div(Nn6 + 1, 256)), dim3(256), 0, c->stream, num, Nn6 + 1);
-    hipLaunchKernelGGL(k_mark_corners, dim3(ceil_div(std::max<int64_t>(E * 3, 1), 256)), dim3(256), 0, c->stream, E,
-                       (const int32_t *)s.conn.p, num);
-    hipLaunchKernelGGL(k_copy_u32_n, dim3(ceil_div(Nn6, 256)), dim3(256), 0, c->stream, (const uint32_t *)num, isc,
-                       Nn6);
-    HIPCHK(c, hipGetLastError());
-    if ((rc = exclusive_scan_u32(c, num, Nn6, total))) return rc;
-    uint32_t h_total = 0;
-    HIPCHK(c, hipMemcpyAsync(&h_total, total, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    const int64_t Nn3 = h_total;
-    d.has_mesh = false;
-    if ((rc = ensure(c, d.conn, 4 * (size_t)(E * 3))) || (rc = ensure(c, d.xy, 16 * (size_t)std::max<int64_t>(Nn3, 1))))
-        return rc;
-    hipLaunchKernelGGL(k_corner_xy, dim3(ceil_div(Nn6, 256)), dim3(256), 0, c->stream, Nn6, (const uint32_t *)num,
-                       (const uint32_t *)isc, (const double2 *)s.xy.p, (double2 *)d.xy.p);
-    hipLaunchKernelGGL(k_corner_conn, dim3(ceil_div(std::max<int64_t>(E * 3, 1), 256)), dim3(256), 0, c->stream, E,
-                       (const int32_t *)s.conn.p, (const uint32_t *)num, (int32_t *)d.conn.p);
-    HIPCHK(c, hipGetLastError());
-    if ((rc = finish_field_mesh(c, d, 3, E, Nn3))) return rc;
-    if (Nn_out) *Nn_out = Nn3;
-    return ELFEL_OK;
+    return host_guard(c, [&]() -> int {
+        if (!c) return ELFEL_EINVAL;
+        c->err.clear();
+        (void)hipSetDevice(c->device);
+        if (src < 0 || src > 1 || dst < 0 || dst > 1 || src == dst)
+            return set_err(c, ELFEL_EINVAL, "elfel_hip_t6_to_t3: bad fields");
+        Field &s = c->f[src], &d = c->f[dst];
+        if (!s.has_mesh || s.nen != 6 || s.sdim != 2) return set_err(c, ELFEL_ESTATE, "elfel_hip_t6_to_t3: source field has no T6 mesh");
+        const int64_t E = s.E, Nn6 = s.Nn;
+        int rc;
+        // flags (scanned in place), a copy of the flags, the total
+        if ((rc = ensure(c, c->flags, 4 * (size_t)(2 * Nn6 + 2)))) return rc;
+        uint32_t *num = (uint32_t *)c->flags.p, *isc = num + Nn6 + 1, *total = isc + Nn6;
+        hipLaunchKernelGGL(k_zero_u32, dim3(ceil_div(Nn6 + 1, 256)), dim3(256), 0, c->stream, num, Nn6 + 1);
+        hipLaunchKernelGGL(k_mark_corners, dim3(ceil_div(std::max<int64_t>(E * 3, 1), 256)), dim3(256), 0, c->stream, E,
+                           (const int32_t *)s.conn.p, num);
+        hipLaunchKernelGGL(k_copy_u32_n, dim3(ceil_div(Nn6, 256)), dim3(256), 0, c->stream, (const uint32_t *)num, isc,
+                           Nn6);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = exclusive_scan_u32(c, num, Nn6, total))) return rc;
+        uint32_t h_total = 0;
+        HIPCHK(c, hipMemcpyAsync(&h_total, total, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const int64_t Nn3 = h_total;
+        d.has_mesh = false;
+        if ((rc = ensure(c, d.conn, 4 * (size_t)(E * 3))) || (rc = ensure(c, d.xy, 16 * (size_t)std::max<int64_t>(Nn3, 1))))
+            return rc;
+        hipLaunchKernelGGL(k_corner_xy, dim3(ceil_div(Nn6, 256)), dim3(256), 0, c->stream, Nn6, (const uint32_t *)num,
+                           (const uint32_t *)isc, (const double2 *)s.xy.p, (double2 *)d.xy.p);
+        hipLaunchKernelGGL(k_corner_conn, dim3(ceil_div(std::max<int64_t>(E * 3, 1), 256)), dim3(256), 0, c->stream, E,
+                           (const int32_t *)s.conn.p, (const uint32_t *)num, (int32_t *)d.conn.p);
+        HIPCHK(c, hipGetLastError());
+        if ((rc = finish_field_mesh(c, d, 3, E, Nn3))) return rc;
+        if (Nn_out) *Nn_out = Nn3;
+        return ELFEL_OK;
+    });
 }
 
 extern "C" int elfel_hip_boundary_nodes(elfel_hip_ctx *c, int field, int64_t Nn, uint8_t *flags, int64_t *count)
 {
-    if (!c) return ELFEL_EINVAL;
-    c->err.clear();
-    (void)hipSetDevice(c->device);
-    if (field < 0 || field > 1) return set_err(c, ELFEL_EINVAL, "elfel_hip_boundary_nodes: bad field");
-    const Field &f = c->f[field];
-    if (!f.has_mesh) return set_err(c, ELFEL_ESTATE, "elfel_hip_boundary_nodes: no mesh");
-    if (f.sdim != 2) return set_err(c, ELFEL_ENOTSUP, "elfel_hip_boundary_nodes: 2D meshes only");
-    if (!flags || Nn != f.Nn) return set_err(c, ELFEL_EINVAL, "elfel_hip_boundary_nodes: flags must hold Nn entries");
-    const int ncorner = f.nen == 6 ? 3 : f.nen;  // T3/T6: 3 corner edges, Q4: 4
-    if (f.nen != 3 && f.nen != 4 && f.nen != 6) return set_err(c, ELFEL_ENOTSUP, "boundary of this element shape");
-    const int64_t n = f.E * ncorner;
-    int rc;
-    if ((rc = ensure(c, c->keys[0], 8 * (size_t)std::max<int64_t>(n, 1))) ||
-        (rc = ensure(c, c->keys[1], 8 * (size_t)std::max<int64_t>(n, 1))) ||
-        (rc = ensure(c, c->vals[0], 8 * (size_t)std::max<int64_t>(n, 1))) ||
-        (rc = ensure(c, c->vals[1], 8 * (size_t)std::max<int64_t>(n, 1))) ||
-        (rc = ensure(c, c->stage64, (size_t)std::max<int64_t>(f.Nn, 1))))
-        return rc;
-    uint64_t *k0 = (uint64_t *)c->keys[0].p, *k1 = (uint64_t *)c->keys[1].p;
-    uint64_t *v0 = (uint64_t *)c->vals[0].p, *v1 = (uint64_t *)c->vals[1].p;
-    uint8_t *dflag = (uint8_t *)c->stage64.p;
-    HIPCHK(c, hipMemsetAsync(dflag, 0, (size_t)f.Nn, c->stream));
-    if (n > 0) {
-        hipLaunchKernelGGL(k_edge_keys, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, f.E, f.nen, ncorner, f.Nn,
-                           (const int32_t *)f.conn.p, k0, v0);
-        HIPCHK(c, hipGetLastError());
-        const int bits = bits_for(f.Nn * f.Nn);
-        if ((rc = radix_sort_pairs(c, k0, v0, k1, v1, n, bits))) return rc;
-        hipLaunchKernelGGL(k_mark_boundary, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, n, f.nen, ncorner,
-                           (const int32_t *)f.conn.p, (const uint64_t *)k0, (const uint64_t *)v0, dflag);
-        HIPCHK(c, hipGetLastError());
-    }
-    HIPCHK(c, hipMemcpyAsync(flags, dflag, (size_t)f.Nn, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    int64_t cnt = 0;
-    for (int64_t i = 0; i < f.Nn; i++) cnt += flags[i] != 0;
-    if (count) *count = cnt;
-    return ELFEL_OK;
+    return host_guard(c, [&]() -> int {
+        if (!c) return ELFEL_EINVAL;
+        c->err.clear();
+        (void)hipSetDevice(c->device);
+        if (field < 0 || field > 1) return set_err(c, ELFEL_EINVAL, "elfel_hip_boundary_nodes: bad field");
+        const Field &f = c->f[field];
+        if (!f.has_mesh) return set_err(c, ELFEL_ESTATE, "elfel_hip_boundary_nodes: no mesh");
+        if (f.sdim != 2) return set_err(c, ELFEL_ENOTSUP, "elfel_hip_boundary_nodes: 2D meshes only");
+        if (!flags || Nn != f.Nn) return set_err(c, ELFEL_EINVAL, "elfel_hip_boundary_nodes: flags must hold Nn entries");
+        const int ncorner = f.nen == 6 ? 3 : f.nen;  // T3/T6: 3 corner edges, Q4: 4
+        if (f.nen != 3 && f.nen != 4 && f.nen != 6) return set_err(c, ELFEL_ENOTSUP, "boundary of this element shape");
+        const int64_t n = f.E * ncorner;
+        int rc;
+        if ((rc = ensure(c, c->keys[0], 8 * (size_t)std::max<int64_t>(n, 1))) ||
+            (rc = ensure(c, c->keys[1], 8 * (size_t)std::max<int64_t>(n, 1))) ||
+            (rc = ensure(c, c->vals[0], 8 * (size_t)std::max<int64_t>(n, 1))) ||
+            (rc = ensure(c, c->vals[1], 8 * (size_t)std::max<int64_t>(n, 1))) ||
+            (rc = ensure(c, c->stage64, (size_t)std::max<int64_t>(f.Nn, 1))))
+            return rc;
+        uint64_t *k0 = (uint64_t *)c->keys[0].p, *k1 = (uint64_t *)c->keys[1].p;
+        uint64_t *v0 = (uint64_t *)c->vals[0].p, *v1 = (uint64_t *)c->vals[1].p;
+        uint8_t *dflag = (uint8_t *)c->stage64.p;
+        HIPCHK(c, hipMemsetAsync(dflag, 0, (size_t)f.Nn, c->stream));
+        if (n > 0) {
+            hipLaunchKernelGGL(k_edge_keys, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, f.E, f.nen, ncorner, f.Nn,
+                               (const int32_t *)f.conn.p, k0, v0);
+            HIPCHK(c, hipGetLastError());
+            const int bits = bits_for(f.Nn * f.Nn);
+            if ((rc = radix_sort_pairs(c, k0, v0, k1, v1, n, bits))) return rc;
+            hipLaunchKernelGGL(k_mark_boundary, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, n, f.nen, ncorner,
+                               (const int32_t *)f.conn.p, (const uint64_t *)k0, (const uint64_t *)v0, dflag);
+            HIPCHK(c, hipGetLastError());
+        }
+        HIPCHK(c, hipMemcpyAsync(flags, dflag, (size_t)f.Nn, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        int64_t cnt = 0;
+        for (int64_t i = 0; i < f.Nn; i++) cnt += flags[i] != 0;
+        if (count) *count = cnt;
+        return ELFEL_OK;
+    });
 }
 
 extern "C" int elfel_hip_copy_mesh(elfel_hip_ctx *c, int field, int64_t E, int64_t *conn, int64_t Nn, double *xy)
 {
-    if (!c) return ELFEL_EINVAL;
-    c->err.clear();
-    (void)hipSetDevice(c->device);
-    if (field < 0 || field > 1) return set_err(c, ELFEL_EINVAL, "elfel_hip_copy_mesh: bad field");
-    const Field &f = c->f[field];
-    if (!f.has_mesh) return set_err(c, ELFEL_ESTATE, "elfel_hip_copy_mesh: no mesh");
-    if (E != f.E || Nn != f.Nn) return set_err(c, ELFEL_EINVAL, "elfel_hip_copy_mesh: sizes differ from the mesh");
-    if (xy && Nn > 0)
-        HIPCHK(c, hipMemcpyAsync(xy, f.xy.p, 8 * (size_t)f.sdim * Nn, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (conn)
-        for (int64_t i = 0; i < E * f.nen; i++) conn[i] = (int64_t)f.h_conn[i] + 1;
-    return ELFEL_OK;
+    return host_guard(c, [&]() -> int {
+        if (!c) return ELFEL_EINVAL;
+        c->err.clear();
+        (void)hipSetDevice(c->device);
+        if (field < 0 || field > 1) return set_err(c, ELFEL_EINVAL, "elfel_hip_copy_mesh: bad field");
+        const Field &f = c->f[field];
+        if (!f.has_mesh) return set_err(c, ELFEL_ESTATE, "elfel_hip_copy_mesh: no mesh");
+        if (E != f.E || Nn != f.Nn) return set_err(c, ELFEL_EINVAL, "elfel_hip_copy_mesh: sizes differ from the mesh");
+        if (xy && Nn > 0)
+            HIPCHK(c, hipMemcpyAsync(xy, f.xy.p, 8 * (size_t)f.sdim * Nn, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (conn)
+            for (int64_t i = 0; i < E * f.nen; i++) conn[i] = (int64_t)f.h_conn[i] + 1;
+        return ELFEL_OK;
+    });
 }
 
 // ---- mesh files ----------------------------------------------------------------
@@ -344,40 +352,42 @@ std::string trim(const char *s)
 extern "C" int elfel_hip_load_mesh(elfel_hip_ctx *c, int field, const char *path, int32_t *nen_out, int64_t *E_out,
                                    int64_t *Nn_out)
 {
-    if (!c) return ELFEL_EINVAL;
-    c->err.clear();
-    if (!path || field < 0 || field > 1) return set_err(c, ELFEL_EINVAL, "elfel_hip_load_mesh: bad arguments");
-    FILE *fh = fopen(path, "r");
-    if (!fh) return set_err(c, ELFEL_EINVAL, std::string("cannot open ") + path);
-    std::vector<std::string> lines;
-    char buf[4096];
-    while (fgets(buf, sizeof buf, fh)) {
-        const std::string t = trim(buf);
-        if (!t.empty()) lines.push_back(t);
-    }
-    fclose(fh);
-    if (lines.size() < 3) return set_err(c, ELFEL_EINVAL, "mesh manifest needs three lines: xyz file, shape, conn file");
-    const std::string p(path);
-    const size_t slash = p.find_last_of('/');
-    const std::string dir = slash == std::string::npos ? std::string() : p.substr(0, slash + 1);
-    const std::string &shape = lines[1];
-    const int nen = shape == "T3" ? 3 : shape == "Q4" ? 4 : shape == "T6" ? 6 : 0;
-    if (!nen) return set_err(c, ELFEL_ENOTSUP, "mesh shape " + shape + " (T3, Q4 and T6 are supported)");
-    std::vector<std::vector<double>> xyz, cn;
-    if (!read_table(dir + lines[0], 2, xyz)) return set_err(c, ELFEL_EINVAL, "cannot read " + dir + lines[0]);
-    if (!read_table(dir + lines[2], nen, cn)) return set_err(c, ELFEL_EINVAL, "cannot read " + dir + lines[2]);
-    const int64_t Nn = (int64_t)xyz.size(), E = (int64_t)cn.size();
-    std::vector<double> xy(2 * (size_t)Nn);
-    for (int64_t n = 0; n < Nn; n++) { xy[2 * n] = xyz[n][0]; xy[2 * n + 1] = xyz[n][1]; }
-    std::vector<int64_t> conn((size_t)(E * nen));
-    for (int64_t e = 0; e < E; e++) {
-        if ((int)cn[e].size() != nen) return set_err(c, ELFEL_EINVAL, "connectivity row with a wrong node count");
-        for (int k = 0; k < nen; k++) conn[e * nen + k] = (int64_t)cn[e][k];
-    }
-    const int rc = elfel_hip_set_mesh(c, field, nen, E, conn.data(), Nn, xy.data());
-    if (rc) return rc;
-    if (nen_out) *nen_out = nen;
-    if (E_out) *E_out = E;
-    if (Nn_out) *Nn_out = Nn;
-    return ELFEL_OK;
+    return host_guard(c, [&]() -> int {
+        if (!c) return ELFEL_EINVAL;
+        c->err.clear();
+        if (!path || field < 0 || field > 1) return set_err(c, ELFEL_EINVAL, "elfel_hip_load_mesh: bad arguments");
+        FILE *fh = fopen(path, "r");
+        if (!fh) return set_err(c, ELFEL_EINVAL, std::string("cannot open ") + path);
+        std::vector<std::string> lines;
+        char buf[4096];
+        while (fgets(buf, sizeof buf, fh)) {
+            const std::string t = trim(buf);
+            if (!t.empty()) lines.push_back(t);
+        }
+        fclose(fh);
+        if (lines.size() < 3) return set_err(c, ELFEL_EINVAL, "mesh manifest needs three lines: xyz file, shape, conn file");
+        const std::string p(path);
+        const size_t slash = p.find_last_of('/');
+        const std::string dir = slash == std::string::npos ? std::string() : p.substr(0, slash + 1);
+        const std::string &shape = lines[1];
+        const int nen = shape == "T3" ? 3 : shape == "Q4" ? 4 : shape == "T6" ? 6 : 0;
+        if (!nen) return set_err(c, ELFEL_ENOTSUP, "mesh shape " + shape + " (T3, Q4 and T6 are supported)");
+        std::vector<std::vector<double>> xyz, cn;
+        if (!read_table(dir + lines[0], 2, xyz)) return set_err(c, ELFEL_EINVAL, "cannot read " + dir + lines[0]);
+        if (!read_table(dir + lines[2], nen, cn)) return set_err(c, ELFEL_EINVAL, "cannot read " + dir + lines[2]);
+        const int64_t Nn = (int64_t)xyz.size(), E = (int64_t)cn.size();
+        std::vector<double> xy(2 * (size_t)Nn);
+        for (int64_t n = 0; n < Nn; n++) { xy[2 * n] = xyz[n][0]; xy[2 * n + 1] = xyz[n][1]; }
+        std::vector<int64_t> conn((size_t)(E * nen));
+        for (int64_t e = 0; e < E; e++) {
+            if ((int)cn[e].size() != nen) return set_err(c, ELFEL_EINVAL, "connectivity row with a wrong node count");
+            for (int k = 0; k < nen; k++) conn[e * nen + k] = (int64_t)cn[e][k];
+        }
+        const int rc = elfel_hip_set_mesh(c, field, nen, E, conn.data(), Nn, xy.data());
+        if (rc) return rc;
+        if (nen_out) *nen_out = nen;
+        if (E_out) *E_out = E;
+        if (Nn_out) *Nn_out = Nn;
+        return ELFEL_OK;
+    });
 }

@@ -70,54 +70,56 @@ extern "C" int elfel_hip_sparse_noalloc(elfel_hip_ctx *c, int64_t nrow, int64_t 
                                         const int64_t *col, const double *val, int layout, int32_t *ptr,
                                         int32_t *ind, double *out, int64_t *nnz)
 {
-    if (!c) return ELFEL_EINVAL;
-    c->err.clear();
-    (void)hipSetDevice(c->device);
-    if (nrow < 0 || ncol < 0 || n < 0 || (n > 0 && (!row || !col || !val || !ind || !out)) || !ptr || !nnz ||
-        (layout != ELFEL_CSR && layout != ELFEL_CSC))
-        return set_err(c, ELFEL_EINVAL, "elfel_hip_sparse_noalloc: bad arguments");
-    if (n > NMAX) {
-        char msg[128];
-        snprintf(msg, sizeof msg, "elfel_hip_sparse_noalloc: %lld entries exceed the limit of %lld", (long long)n,
-                 (long long)NMAX);
-        return set_err(c, ELFEL_EINVAL, msg);
-    }
-    if (nrow >= INT32_MAX || ncol >= INT32_MAX)
-        return set_err(c, ELFEL_ENOTSUP, "elfel_hip_sparse_noalloc: dimensions beyond int32");
-    unsigned char *base = (unsigned char *)noalloc_scratch(c->device);
-    if (!base) return set_err(c, ELFEL_ESTATE, "elfel_hip_sparse_noalloc: no scratch address for this device");
-    const int csc = layout == ELFEL_CSC;
-    const int64_t nmaj = csc ? ncol : nrow, nmin = csc ? nrow : ncol;
-    const int minor_bits = bits_for(nmin), major_bits = bits_for(nmaj);
-    uint64_t *k0 = (uint64_t *)(base + O_K0), *k1 = (uint64_t *)(base + O_K1);
-    uint64_t *v0 = (uint64_t *)(base + O_V0), *v1 = (uint64_t *)(base + O_V1);
-    SortScratch s;
-    s.hist = (uint32_t *)(base + O_HIST);
-    s.hist_cap = (int64_t)(B_HIST / 4);
-    s.scan = (uint32_t *)(base + O_SCAN);
-    s.scan_cap = (int64_t)(B_SCAN / 4);
-    s.flags = (uint32_t *)(base + O_FLAGS);
-    s.segpos = (uint32_t *)(base + O_SEG);
-    s.n_cap = NMAX;
-    int32_t *nnz_dev = (int32_t *)(base + O_NNZ);
-    int rc;
-    if (n > 0) HIPCHK(c, hipMemcpyAsync(v0, val, 8 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
-    if ((rc = plain_coo_keys(c, row, col, n, nrow, ncol, minor_bits, csc, k0))) return rc;
-    // the range flags live in the context's device scalars (set at init)
-    int flags = 0;
-    HIPCHK(c, hipMemcpyAsync(&flags, (int *)c->dscal.p + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (flags) {
-        HIPCHK(c, hipMemsetAsync((int *)c->dscal.p + 1, 0, sizeof(int), c->stream));
-        if (flags & DERR_RANGE) return set_err(c, ELFEL_ERANGE, "row or column index out of range");
-    }
-    if ((rc = radix_sort_pairs(c, k0, v0, k1, v1, n, major_bits + minor_bits, &s))) return rc;
-    if ((rc = build_csr_from_sorted(c, k0, (const double *)v0, n, minor_bits, nmaj, ptr, ind, out, nnz_dev, true,
-                                    &s)))
-        return rc;
-    int32_t h = 0;
-    HIPCHK(c, hipMemcpyAsync(&h, nnz_dev, 4, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    *nnz = h;
-    return ELFEL_OK;
+    return host_guard(c, [&]() -> int {
+        if (!c) return ELFEL_EINVAL;
+        c->err.clear();
+        (void)hipSetDevice(c->device);
+        if (nrow < 0 || ncol < 0 || n < 0 || (n > 0 && (!row || !col || !val || !ind || !out)) || !ptr || !nnz ||
+            (layout != ELFEL_CSR && layout != ELFEL_CSC))
+            return set_err(c, ELFEL_EINVAL, "elfel_hip_sparse_noalloc: bad arguments");
+        if (n > NMAX) {
+            char msg[128];
+            snprintf(msg, sizeof msg, "elfel_hip_sparse_noalloc: %lld entries exceed the limit of %lld", (long long)n,
+                     (long long)NMAX);
+            return set_err(c, ELFEL_EINVAL, msg);
+        }
+        if (nrow >= INT32_MAX || ncol >= INT32_MAX)
+            return set_err(c, ELFEL_ENOTSUP, "elfel_hip_sparse_noalloc: dimensions beyond int32");
+        unsigned char *base = (unsigned char *)noalloc_scratch(c->device);
+        if (!base) return set_err(c, ELFEL_ESTATE, "elfel_hip_sparse_noalloc: no scratch address for this device");
+        const int csc = layout == ELFEL_CSC;
+        const int64_t nmaj = csc ? ncol : nrow, nmin = csc ? nrow : ncol;
+        const int minor_bits = bits_for(nmin), major_bits = bits_for(nmaj);
+        uint64_t *k0 = (uint64_t *)(base + O_K0), *k1 = (uint64_t *)(base + O_K1);
+        uint64_t *v0 = (uint64_t *)(base + O_V0), *v1 = (uint64_t *)(base + O_V1);
+        SortScratch s;
+        s.hist = (uint32_t *)(base + O_HIST);
+        s.hist_cap = (int64_t)(B_HIST / 4);
+        s.scan = (uint32_t *)(base + O_SCAN);
+        s.scan_cap = (int64_t)(B_SCAN / 4);
+        s.flags = (uint32_t *)(base + O_FLAGS);
+        s.segpos = (uint32_t *)(base + O_SEG);
+        s.n_cap = NMAX;
+        int32_t *nnz_dev = (int32_t *)(base + O_NNZ);
+        int rc;
+        if (n > 0) HIPCHK(c, hipMemcpyAsync(v0, val, 8 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+        if ((rc = plain_coo_keys(c, row, col, n, nrow, ncol, minor_bits, csc, k0))) return rc;
+        // the range flags live in the context's device scalars (set at init)
+        int flags = 0;
+        HIPCHK(c, hipMemcpyAsync(&flags, (int *)c->dscal.p + 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (flags) {
+            HIPCHK(c, hipMemsetAsync((int *)c->dscal.p + 1, 0, sizeof(int), c->stream));
+            if (flags & DERR_RANGE) return set_err(c, ELFEL_ERANGE, "row or column index out of range");
+        }
+        if ((rc = radix_sort_pairs(c, k0, v0, k1, v1, n, major_bits + minor_bits, &s))) return rc;
+        if ((rc = build_csr_from_sorted(c, k0, (const double *)v0, n, minor_bits, nmaj, ptr, ind, out, nnz_dev, true,
+                                        &s)))
+            return rc;
+        int32_t h = 0;
+        HIPCHK(c, hipMemcpyAsync(&h, nnz_dev, 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *nnz = h;
+        return ELFEL_OK;
+    });
 }

@@ -123,34 +123,36 @@ __global__ void k_sum_pairs(const double *__restrict__ part, int n, double *__re
 
 extern "C" int elfel_hip_colliding_errors(elfel_hip_ctx *c, int64_t ndofs, const double *U, double *ep, double *ev)
 {
-    if (!c) return ELFEL_EINVAL;
-    c->err.clear();
-    (void)hipSetDevice(c->device);
-    const Field &V = c->f[0], &P = c->f[1];
-    if (!V.has_mesh || !V.has_dofs || !P.has_mesh || !P.has_dofs)
-        return set_err(c, ELFEL_ESTATE, "colliding_errors: set_mesh/set_dofs of both fields first");
-    if (V.nen != 6 || V.ncomp != 2 || P.nen != 3 || P.ncomp != 1 || V.sdim != 2 || P.sdim != 2)
-        return set_err(c, ELFEL_EINVAL, "colliding_errors: field 0 must be vector T6, field 1 scalar T3");
-    if (!U || ndofs < std::max(V.ndofs, P.ndofs)) return set_err(c, ELFEL_EINVAL, "colliding_errors: bad vector");
-    QTable t3, t6;
-    if (make_qtable(ELFEL_T3, 3, &t3) || make_qtable(ELFEL_T6, 3, &t6))
-        return set_err(c, ELFEL_EINVAL, "no quadrature table");
-    const int64_t E = std::max(V.E, P.E);
-    const int nb = ceil_div(std::max<int64_t>(E, 1), 256);
-    int rc = ensure(c, c->pcg, 8 * ((size_t)ndofs + 2 * (size_t)nb + 2) + 256);
-    if (rc) return rc;
-    double *d_U = (double *)c->pcg.p, *part = d_U + ndofs, *out = part + 2 * nb;
-    hipStream_t st = c->stream;
-    HIPCHK(c, hipMemcpyAsync(d_U, U, 8 * (size_t)ndofs, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_colliding, dim3(nb), dim3(256), 0, st, P.E, (const int32_t *)P.conn.p,
-                       (const double2 *)P.xy.p, (const int32_t *)P.dof.p, V.E, (const int32_t *)V.conn.p,
-                       (const double2 *)V.xy.p, (const int32_t *)V.dof.p, (const double *)d_U, t3, t6, part);
-    hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(1024), 0, st, (const double *)part, nb, out);
-    HIPCHK(c, hipGetLastError());
-    double h[2];
-    HIPCHK(c, hipMemcpyAsync(h, out, 16, hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipStreamSynchronize(st));
-    if (ep) *ep = sqrt(h[0]);
-    if (ev) *ev = sqrt(h[1]);
-    return ELFEL_OK;
+    return host_guard(c, [&]() -> int {
+        if (!c) return ELFEL_EINVAL;
+        c->err.clear();
+        (void)hipSetDevice(c->device);
+        const Field &V = c->f[0], &P = c->f[1];
+        if (!V.has_mesh || !V.has_dofs || !P.has_mesh || !P.has_dofs)
+            return set_err(c, ELFEL_ESTATE, "colliding_errors: set_mesh/set_dofs of both fields first");
+        if (V.nen != 6 || V.ncomp != 2 || P.nen != 3 || P.ncomp != 1 || V.sdim != 2 || P.sdim != 2)
+            return set_err(c, ELFEL_EINVAL, "colliding_errors: field 0 must be vector T6, field 1 scalar T3");
+        if (!U || ndofs < std::max(V.ndofs, P.ndofs)) return set_err(c, ELFEL_EINVAL, "colliding_errors: bad vector");
+        QTable t3, t6;
+        if (make_qtable(ELFEL_T3, 3, &t3) || make_qtable(ELFEL_T6, 3, &t6))
+            return set_err(c, ELFEL_EINVAL, "no quadrature table");
+        const int64_t E = std::max(V.E, P.E);
+        const int nb = ceil_div(std::max<int64_t>(E, 1), 256);
+        int rc = ensure(c, c->pcg, 8 * ((size_t)ndofs + 2 * (size_t)nb + 2) + 256);
+        if (rc) return rc;
+        double *d_U = (double *)c->pcg.p, *part = d_U + ndofs, *out = part + 2 * nb;
+        hipStream_t st = c->stream;
+        HIPCHK(c, hipMemcpyAsync(d_U, U, 8 * (size_t)ndofs, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_colliding, dim3(nb), dim3(256), 0, st, P.E, (const int32_t *)P.conn.p,
+                           (const double2 *)P.xy.p, (const int32_t *)P.dof.p, V.E, (const int32_t *)V.conn.p,
+                           (const double2 *)V.xy.p, (const int32_t *)V.dof.p, (const double *)d_U, t3, t6, part);
+        hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(1024), 0, st, (const double *)part, nb, out);
+        HIPCHK(c, hipGetLastError());
+        double h[2];
+        HIPCHK(c, hipMemcpyAsync(h, out, 16, hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        if (ep) *ep = sqrt(h[0]);
+        if (ev) *ev = sqrt(h[1]);
+        return ELFEL_OK;
+    });
 }

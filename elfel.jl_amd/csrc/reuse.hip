@@ -19,6 +19,7 @@
 #include <atomic>
 #include <cmath>
 #include <chrono>
+#include <mutex>
 #include <thread>
 
 #include "reuse_kernel.h"
@@ -565,20 +566,35 @@ int tile_tables(elfel_hip_ctx *c, bool stokes, const Field &F0, const Field *F1,
     const int nthreads = (int)std::max<int64_t>(1, std::min<int64_t>(
         nranges, std::min<int64_t>(16, std::max(1u, std::thread::hardware_concurrency()))));
     std::atomic<int64_t> next{0};
+    // an exception on a thread (a failed host allocation) fails the build;
+    // the other threads still drain their ranges
+    std::mutex exc_mu;
+    int exc_rc = 0;
+    std::string exc_err;
     auto worker = [&]() {
-        // the last tile each element joined (element-staged tiles)
-        std::vector<int32_t> stamp(I.staged ? F0.E : 0, -1);
-        int tile_id = 0;
-        for (;;) {
-            const int64_t k = next.fetch_add(1);
-            if (k >= nranges) break;
-            prc[k] = tile_range(I, cut[k], cut[k + 1], part[k], stamp, tile_id, perr[k]);
+        try {
+            // the last tile each element joined (element-staged tiles)
+            std::vector<int32_t> stamp(I.staged ? F0.E : 0, -1);
+            int tile_id = 0;
+            for (;;) {
+                const int64_t k = next.fetch_add(1);
+                if (k >= nranges) break;
+                prc[k] = tile_range(I, cut[k], cut[k + 1], part[k], stamp, tile_id, perr[k]);
+            }
+        } catch (...) {
+            std::string m;
+            const int code = exception_code(&m);
+            std::lock_guard<std::mutex> lk(exc_mu);
+            if (!exc_rc) { exc_rc = code; exc_err = m; }
         }
     };
     std::vector<std::thread> pool;
-    for (int t = 1; t < nthreads; t++) pool.emplace_back(worker);
-    worker();
-    for (auto &th : pool) th.join();
+    {
+        auto joiner = join_on_exit(pool, [] {});
+        for (int t = 1; t < nthreads; t++) pool.emplace_back(worker);
+        worker();
+    }
+    if (exc_rc) return fail(exc_rc, exc_err);
     for (int64_t k = 0; k < nranges; k++)
         if (prc[k]) return fail(prc[k], perr[k]);
     // concatenation: tile task offsets become global

@@ -29,7 +29,7 @@ enum {
     ELFEL_OK = 0,
     ELFEL_EINVAL = -1,   /* bad argument (null pointer, bad size, bad enum) */
     ELFEL_ERANGE = -2,   /* index out of range: sparse() ArgumentError / BoundsError */
-    ELFEL_ENOMEM = -3,   /* device allocation failed */
+    ELFEL_ENOMEM = -3,   /* device or host allocation failed */
     ELFEL_EHIP = -4,     /* HIP runtime error */
     ELFEL_ESTATE = -5,   /* call out of order (no mesh, no dofs, no result) */
     ELFEL_EGEOM = -6,    /* element with non-positive Jacobian determinant */
