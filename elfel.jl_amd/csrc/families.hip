@@ -661,9 +661,13 @@ __device__ __forceinline__ double run_sum(const int32_t *__restrict__ perm, cons
 // and each work-item then sums its run from LDS; a longer range gathers run by
 // run (run_sum).  Gathering run by run everywhere left most lanes idle in the
 // waves that hold a long run (a heat T4 diagonal entry sums 24 values, an
-// off-diagonal one 1 to 6): heat T4 64^3 0.20 ms.  (Contiguous eighths of the
-// entries per XCD, so that neighbouring rows share one XCD's L2, measured
-// slower still: 0.48 ms.)
+// off-diagonal one 1 to 6): heat T4 64^3 0.20 ms.  Each work-item issues up to
+// eight of its gathers before it stores any (heat T4 64^3 kernel 145 -> 124 us,
+// Reddy P1b/P1 512^2 330 -> 306 us, together with the non-temporal stores of
+// the sums; four at a time 133 us, sixteen 124 us).  (Contiguous eighths of
+// the entries per XCD, so that neighbouring rows share one XCD's L2, measured
+// slower still: 0.48 ms; non-temporal loads of the permutation 210 us;
+// non-temporal stores of the element blocks slowed the element kernel.)
 template <typename VT>
 __global__ __launch_bounds__(256) void k_fam_runs(const int32_t *__restrict__ perm, const int32_t *__restrict__ seg,
                                                   const VT *__restrict__ coo, int64_t nnz, double *__restrict__ val,
@@ -684,7 +688,17 @@ __global__ __launch_bounds__(256) void k_fam_runs(const int32_t *__restrict__ pe
     const int32_t np = sg[e1] - p0;
     const bool staged = np <= RUNS_CAP;  // the same in the whole workgroup
     if (staged) {
-        for (int32_t k = threadIdx.x; k < np; k += 256) sv[k] = v[pm[p0 + k]];
+        for (int32_t k = threadIdx.x; k < np; k += 8 * 256) {
+            int32_t q[8];
+            VT x[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) q[u] = k + u * 256 < np ? pm[p0 + k + u * 256] : 0;
+#pragma unroll
+            for (int u = 0; u < 8; u++) x[u] = k + u * 256 < np ? v[q[u]] : VT(0);
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (k + u * 256 < np) sv[k + u * 256] = x[u];
+        }
         __syncthreads();
     }
     const int64_t s = e0 + threadIdx.x;
@@ -699,8 +713,8 @@ __global__ __launch_bounds__(256) void k_fam_runs(const int32_t *__restrict__ pe
             acc = run_sum<8>(pm, v, i0, i1);
         }
     }
-    if (mat) val[s] = acc;
-    else F[s] = i0 == i1 ? 0.0 : 0.0 + acc;  // start! zeros the vector, assemble! adds
+    if (mat) __builtin_nontemporal_store(acc, &val[s]);
+    else __builtin_nontemporal_store(i0 == i1 ? 0.0 : 0.0 + acc, &F[s]);  // start! zeros the vector, assemble! adds
 }
 
 // The element kernel of the problem (its weak form, element and rule) with
