@@ -308,9 +308,10 @@ class Context:
         return ke, ms.value
 
     def assembly_info(self):
-        """(method, precision, note) of what the last assemble ran: the REUSE
-        method falls back to the plain FP64 method on meshes its tiles do not
-        hold (note says why)."""
+        """(method, precision, note) of what the last assemble ran: on meshes
+        its tiles do not hold, the REUSE method sums the runs of the plain
+        method's stream in its cached sort order, in FP64 (REUSE, FP64; the
+        note says why)."""
         m, p = ctypes.c_int(), ctypes.c_int()
         buf = ctypes.create_string_buffer(256)
         self._chk(self.lib.elfel_hip_assembly_info(self.h, ctypes.byref(m), ctypes.byref(p), buf, 256),
