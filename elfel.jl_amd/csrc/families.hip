@@ -265,6 +265,35 @@ __device__ __forceinline__ void store_block(VT *dst, const VT (&v)[N])
     }
 }
 
+// The workgroup's blocks of N values (elements e0 .. e0 + 255, contiguous in
+// dst) written through LDS: each work-item's N values into its LDS row (N + 1
+// apart, against bank conflicts), then the range out with consecutive lanes
+// on consecutive 16-byte words.  Every work-item of the workgroup calls it.
+template <int N, typename VT>
+__device__ __forceinline__ void staged_store(VT *dst, int64_t e0, int64_t E, const VT (&v)[N], VT *st)
+{
+    constexpr int W = 16 / (int)sizeof(VT);
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < N; k++) st[t * (N + 1) + k] = v[k];
+    __syncthreads();
+    const int nv = (int)(E - e0 < 256 ? E - e0 : 256) * N;
+    VT *d = dst + e0 * N;
+    if constexpr (N % W == 0) {
+        for (int i = t * W; i < nv; i += 256 * W) {
+            VT q[W];
+#pragma unroll
+            for (int w = 0; w < W; w++) q[w] = st[((i + w) / N) * (N + 1) + (i + w) % N];
+            uint4 u;
+            __builtin_memcpy(&u, q, 16);
+            *(uint4 *)(d + i) = u;
+        }
+    } else {
+        for (int i = t; i < nv; i += 256) d[i] = st[(i / N) * (N + 1) + i % N];
+    }
+    __syncthreads();
+}
+
 __device__ __forceinline__ uint64_t coo_key(int32_t r, int32_t col, int minor_bits, int csc)
 {
     const uint64_t mj = (uint64_t)(csc ? col : r), mn = (uint64_t)(csc ? r : col);
@@ -290,13 +319,16 @@ __global__ __launch_bounds__(256) void k_fam_scalar(int64_t E, const int32_t *__
 {
     constexpr bool F32 = sizeof(VT) == 4;
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= E) return;
+    // past the end: a copy of the last element, computed and not stored (the
+    // workgroup's stores may go through LDS together)
+    const bool valid = e < E;
+    const int64_t ec = valid ? e : E - 1;
     double X[NB * M];
     int32_t d[NB];
     const bool want_keys = keys || fkeys;
 #pragma unroll
     for (int k = 0; k < NB; k++) {
-        const int32_t n = conn[e * NB + k];
+        const int32_t n = conn[ec * NB + k];
 #pragma unroll
         for (int s = 0; s < M; s++) X[k * M + s] = x[(int64_t)n * M + s];
         d[k] = want_keys ? dof[n] : 0;
@@ -347,20 +379,31 @@ __global__ __launch_bounds__(256) void k_fam_scalar(int64_t E, const int32_t *__
             }
         }
     }
-    if (keys) {
+    if (keys && valid) {
         int64_t o = e * NB * NB;
 #pragma unroll
         for (int j = 0; j < NB; j++)
 #pragma unroll
             for (int i = 0; i < NB; i++) keys[o++] = coo_key(d[i], d[j], minor_bits, csc);
     }
-    store_block<NB * NB>(vals + e * NB * NB, ke);
     if constexpr (KIND == ELFEL_HEAT) {
-        if (fkeys) {
+        if (fkeys && valid) {
 #pragma unroll
             for (int j = 0; j < NB; j++) fkeys[e * NB + j] = (uint64_t)d[j];
         }
-        store_block<NB>(fvals + e * NB, fe);
+    }
+    // values through LDS where the stage takes at most 40 KiB (heat T4 64^3
+    // element kernel 51 -> 40 us); the larger blocks (FP64 T6 mass, 76 KiB)
+    // keep direct stores and their occupancy
+    constexpr bool STAGE = (size_t)256 * (NB * NB + 1) * sizeof(VT) <= 40 * 1024;
+    if constexpr (STAGE) {
+        __shared__ VT stage[256 * (NB * NB + 1)];
+        const int64_t e0 = (int64_t)blockIdx.x * blockDim.x;
+        staged_store<NB * NB>(vals, e0, E, ke, stage);
+        if constexpr (KIND == ELFEL_HEAT) staged_store<NB>(fvals, e0, E, fe, stage);
+    } else if (valid) {
+        store_block<NB * NB>(vals + e * NB * NB, ke);
+        if constexpr (KIND == ELFEL_HEAT) store_block<NB>(fvals + e * NB, fe);
     }
 }
 
@@ -661,13 +704,9 @@ __device__ __forceinline__ double run_sum(const int32_t *__restrict__ perm, cons
 // and each work-item then sums its run from LDS; a longer range gathers run by
 // run (run_sum).  Gathering run by run everywhere left most lanes idle in the
 // waves that hold a long run (a heat T4 diagonal entry sums 24 values, an
-// off-diagonal one 1 to 6): heat T4 64^3 0.20 ms.  Each work-item issues up to
-// eight of its gathers before it stores any (heat T4 64^3 kernel 145 -> 124 us,
-// Reddy P1b/P1 512^2 330 -> 306 us, together with the non-temporal stores of
-// the sums; four at a time 133 us, sixteen 124 us).  (Contiguous eighths of
-// the entries per XCD, so that neighbouring rows share one XCD's L2, measured
-// slower still: 0.48 ms; non-temporal loads of the permutation 210 us;
-// non-temporal stores of the element blocks slowed the element kernel.)
+// off-diagonal one 1 to 6): heat T4 64^3 0.20 ms.  (Contiguous eighths of the
+// entries per XCD, so that neighbouring rows share one XCD's L2, measured
+// slower still: 0.48 ms.)
 template <typename VT>
 __global__ __launch_bounds__(256) void k_fam_runs(const int32_t *__restrict__ perm, const int32_t *__restrict__ seg,
                                                   const VT *__restrict__ coo, int64_t nnz, double *__restrict__ val,
